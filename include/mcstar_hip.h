@@ -1,0 +1,176 @@
+/*
+ * mcstar_hip.h -- C ABI of libmcstar_hip.so, the MI355X (gfx950) PFC time step.
+ *
+ * Plain C: pointers, sizes and status codes only.  Every entry point returns
+ * MCS_OK (0) or an MCS_ERR_* code; mcs_error_string() names the code and
+ * mcs_last_error() gives "file:line message" of the most recent failure.
+ * The C++ host headers under include/cuda and include/common keep the
+ * reference's host API and call only these functions.
+ *
+ * Array conventions (cuda/array_abstraction.hpp:37-70):
+ *   ext[3] = {nx, ny, nz}, real extents, x fastest (int3 real_vext()).
+ *   real view:    (i2*ny + i1)*nx + i0, unpadded
+ *   complex view: (i2*ny + i1)*(nx/2+1) + i0, interleaved re,im
+ *   one buffer of (nx/2+1)*ny*nz*2*sizeof(F) bytes backs both views.
+ * Precision: prec = 32 (float) or 64 (double).
+ * Synchronisation: every entry point that hands data back to the host, or
+ * that the reference made synchronous (cufft_wrapper.hpp:85, every
+ * call_kernel_* wrapper), returns with its device work complete.
+ */
+#ifndef MCSTAR_HIP_H
+#define MCSTAR_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status */
+#define MCS_OK 0
+#define MCS_ERR_INVALID 1     /* bad argument (null pointer, extent < 1, mismatch) */
+#define MCS_ERR_HIP 2         /* HIP runtime failure */
+#define MCS_ERR_UNSUPPORTED 3 /* size or mode outside what the kernels implement */
+#define MCS_ERR_NOMEM 4       /* device or host allocation failed */
+#define MCS_ERR_CAPACITY 5    /* output capacity too small; count still reported */
+
+#define MCS_F32 32
+#define MCS_F64 64
+
+/* Replaces cudaGetErrorString / cufftGetErrorString (common/cu_error.hpp:5-11,
+   cuda/cufft_wrapper.hpp:7-29). */
+const char* mcs_error_string(int code);
+const char* mcs_last_error(void);
+
+/* Replaces cudaSetDevice(0) (cufft_unittest/main.cu:29). */
+int mcs_init(int device);
+int mcs_sync(void);
+
+/* ---------------------------------------------------------------- memory
+ * GPUArray / CPUArray storage (cuda/array_abstraction.hpp:74-131):
+ * cudaMalloc / cudaFree / cudaHostAlloc / cudaFreeHost / cudaMemcpy. */
+#define MCS_H2D 1
+#define MCS_D2H 2
+#define MCS_D2D 3
+#define MCS_H2H 4
+int mcs_malloc(void** ptr, size_t bytes);
+int mcs_free(void* ptr);
+int mcs_host_alloc(void** ptr, size_t bytes);
+int mcs_host_free(void* ptr);
+int mcs_memcpy(void* dst, const void* src, size_t bytes, int kind);
+int mcs_memset(void* dst, int value, size_t bytes);
+
+/* ---------------------------------------------------------------- FFT
+ * FFT::CufftPlan<F, R2C|C2R> (cuda/cufft_wrapper.hpp:61-86).  Rank 1/2/3
+ * from the extents (72-74), unnormalised both ways, out-of-place.  The C2R
+ * input is left unmodified (the plan owns a scratch buffer).  Axis lengths
+ * must be 2,3,5-smooth (cuda/math.hpp:125-149), at most 16384 (f32) / 8192
+ * (f64).  C2R of a non-Hermitian spectrum follows numpy.fft.irfftn: the
+ * imaginary parts of the x = 0 and x = nx/2 entries are ignored. */
+#define MCS_R2C 0
+#define MCS_C2R 1
+typedef struct mcs_fft_plan mcs_fft_plan;
+int mcs_fft_plan_create(mcs_fft_plan** plan, const int ext[3], int prec, int dir);
+int mcs_fft_exec(mcs_fft_plan* plan, const void* in, void* out);
+int mcs_fft_plan_destroy(mcs_fft_plan* plan);
+
+/* ---------------------------------------------------------------- noise
+ * NoiseHostApi<F> (cuda/noise.hpp:93-150), with the cuRAND XORWOW stream
+ * replaced by counter-based Philox4x32-10: key = seed, counter =
+ * (element index lo, hi, draw number, stream tag).  Each fill advances the
+ * draw number by one. */
+typedef struct mcs_rng mcs_rng;
+int mcs_rng_create(mcs_rng** rng, uint64_t seed);
+int mcs_rng_destroy(mcs_rng* rng);
+int mcs_rng_set_draw(mcs_rng* rng, uint32_t draw);
+int mcs_rng_get_draw(mcs_rng* rng, uint32_t* draw);
+/* fill(GPUArray&, F sigma): n real N(0, sigma^2) values (noise.hpp:108-109) */
+int mcs_rng_fill_normal(mcs_rng* rng, void* dst, size_t n, double sigma, int prec);
+/* fill_kspace(GPUArray&): unit complex Gaussians over the half spectrum with
+   Hermitian repair on the x = 0, nx/2 planes (noise.hpp:110-142) */
+int mcs_rng_fill_kspace(mcs_rng* rng, void* cmpl, const int ext[3], int prec);
+/* raw Philox4x32-10 blocks: out[4*i + j] for counter {c0 + i (64-bit lo,hi), c2, c3} */
+int mcs_philox_blocks(uint64_t seed, uint64_t c0, uint32_t c2, uint32_t c3, size_t n, uint32_t* out_dev);
+
+/* ---------------------------------------------------------------- pointwise
+ * call_kernel_psi3_split (cuda/kernels/psi3_split.hpp:15-26):
+ *   a <- (mul*a)^3 - mul*a - mu [+ pot]; mul = 1/N when norm != 0. */
+int mcs_psi3_split(void* real, const void* pot, double mu, int norm, const int ext[3], int prec);
+/* call_kernel_norm (cuda/kernels/norm.hpp:5-10): a <- a / N */
+int mcs_norm(void* real, const int ext[3], int prec);
+/* call_kernel_update_eom (cuda/kernels/eom.hpp:60-86), reference split:
+ *   kpsi <- (kpsi - dt k^2 knonlin + dt namp sqrt(N) |k| knoise) / (1 + dt k^2 (1 - eps + (1-k^2)^2));
+ *   knoise may be NULL (noise off). */
+int mcs_update_eom(void* kpsi, const void* knonlin, const void* knoise, const int ext[3], double namp,
+                   double dt, double eps, const double lens[3], int prec);
+/* PFC split selector for the fused paths */
+#define MCS_SPLIT_NORTHSTAR 0 /* N = psi^3,       D = 1 + dt k^2 (-eps + (1-k^2)^2) */
+#define MCS_SPLIT_REFERENCE 1 /* N = psi^3 - psi, D = 1 + dt k^2 (1 - eps + (1-k^2)^2) */
+/* the same update with conserved noise drawn in registers:
+ *   eta_k = sqrt(2 T dt / dV) sqrt(N) |k| xi_k(seed, draw), xi_k = mcs_rng_fill_kspace's values. */
+int mcs_update_pfc(void* kpsi, const void* knl, const int ext[3], const double lens[3], double dt, double eps,
+                   double temp, uint64_t seed, uint32_t draw, int split, int prec);
+/* psi = psi0 + amp (2u - 1), u a 53-bit uniform from Philox(seed, cell index) (computed in FP64) */
+int mcs_init_field(void* real, const int ext[3], double psi0, double amp, uint64_t seed, int prec);
+/* FP64 reductions of a real field: mean density and free energy
+ *   F = dV sum[psi/2 (-eps + (1+lap)^2) psi + psi^4/4]  (cuda/kernels/eom.hpp:111-141) */
+int mcs_field_stats(const void* real, const int ext[3], const double h[3], double eps, int prec,
+                    double* mean, double* free_energy);
+
+/* ---------------------------------------------------------------- stepper
+ * The fused PFC step: the state lives in a packed spectrum (x pairs packed
+ * into one complex value), each step is four passes in 3D (two in 2D):
+ *   x: FFT, semi-implicit update + Philox noise, inverse FFT
+ *   y: inverse FFT | z: inverse FFT, nonlinearity, FFT | y: FFT
+ * nx even; axis lengths nx/2, ny, nz of the form 2^a, 3*2^a or 5*2^a (>= 4). */
+typedef struct {
+    int ext[3];     /* nx, ny, nz (nz = 1 for 2D) */
+    double h[3];    /* grid spacing */
+    double eps, dt;
+    double temp;    /* T, 0 = noise off */
+    uint64_t seed;  /* noise key */
+    int split;      /* MCS_SPLIT_* */
+    int prec;       /* MCS_F32 / MCS_F64 */
+} mcs_pfc_params;
+typedef struct mcs_stepper mcs_stepper;
+int mcs_stepper_create(mcs_stepper** st, const mcs_pfc_params* p);
+/* load the real field (device, reference layout, precision p->prec) */
+int mcs_stepper_set_psi(mcs_stepper* st, const void* psi_dev);
+int mcs_stepper_get_psi(mcs_stepper* st, void* psi_dev);
+int mcs_stepper_step(mcs_stepper* st, int nsteps);
+/* steps taken so far (the Philox draw number of the next step) */
+int mcs_stepper_get_step(mcs_stepper* st, int64_t* step);
+int mcs_stepper_set_step(mcs_stepper* st, int64_t step);
+/* launch nsteps on the library stream without waiting (for timing); mcs_sync() to finish */
+int mcs_stepper_step_async(mcs_stepper* st, int nsteps);
+/* mean density from the FP64 k=0 bookkeeping */
+int mcs_stepper_mass(mcs_stepper* st, double* mean);
+int mcs_stepper_destroy(mcs_stepper* st);
+
+/* ---------------------------------------------------------------- peaks
+ * call_kernel_findpeaks / _2d (cuda/kernels/findpeaks.hpp:7-63,
+ * findpeaks_2d.hpp:7-56): strict periodic local maxima (26 neighbours in 3D,
+ * 8 in 2D) with value >= ths.  Output ordered by linear cell index.
+ * idx_out (host) receives up to cap linear indices, vals_out (host, may be
+ * NULL) the field values as double; *count the number found (MCS_ERR_CAPACITY
+ * if count > cap). */
+int mcs_findpeaks(const void* real, const int ext[3], double ths, int64_t* idx_out, double* vals_out,
+                  int64_t cap, int64_t* count, int prec);
+
+/* ---------------------------------------------------------------- bond order
+ * bond_order_analysis<T> (bond-order-analysis/bond-order.cpp:289-349):
+ * neighbours within rlim under the minimum image of a cubic box, q4 q6 q8 w4
+ * q4b q6b q8b w4b xi per point (out9[9*i + ...], NaN when nb[i] == 0).
+ * Host arrays in and out; computed on the GPU in FP64. */
+int mcs_bond_order_cutoff(const double* xyz, int npts, double box, double rlim, int* nb, double* out9);
+/* k nearest neighbours on a periodic cubic grid of side n cells (spacing h):
+ * integer squared distances, ties broken by the smaller index.  nbr[k*i+j],
+ * out4[4*i + {q4,q6,q8,w4}], optional 64-bin histograms of q4 and q6 over [0,1). */
+int mcs_bond_order_knn(const int* ijk, int npts, int n, double h, int k, int* nbr, double* out4,
+                       int64_t* hist_q4, int64_t* hist_q6, int nbins);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MCSTAR_HIP_H */

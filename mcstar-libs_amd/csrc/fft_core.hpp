@@ -1,0 +1,213 @@
+// Stockham autosort FFT engines for CDNA4 (gfx950), written for this library.
+//
+// Pass p of a length-L transform with radix R and sub-length Ns (product of
+// the earlier radices) maps butterfly j in [0, L/R):
+//     k   = j mod Ns
+//     v_q = x[j + q*L/R] * w_{Ns*R}^{q*k}             q in [0, R)
+//     y[(j/Ns)*Ns*R + k + s*Ns] = DFT_R(v)_s           s in [0, R)
+// which takes natural-order input to natural-order output.
+//
+// Register engine: a line of L points is owned by S = L/P threads, thread t
+// holding the P points t + m*S (m in [0, P)).  For every radix R dividing P
+// the inputs of butterflies j = t + b*S (b in [0, P/R)) are exactly those
+// points, so each pass reads registers, writes its outputs to LDS, and the
+// exchange reads the same pattern back.  The last pass lands its outputs in
+// the thread's own registers at natural positions t + m*S, so an inverse
+// transform, a pointwise op and a forward transform chain without LDS.
+//
+// Generic engine: ping-pong LDS buffers and a work-item loop, for the
+// 2,3,5-smooth lengths the register engine does not take.
+#pragma once
+
+#include "mcs_common.hpp"
+
+#define MCS_MAX_PASSES 16
+
+struct FftPlanDev {
+    int L;
+    int np;
+    int radix[MCS_MAX_PASSES];
+    int ns[MCS_MAX_PASSES];
+    int twoff[MCS_MAX_PASSES];
+};
+
+// ------------------------------------------------------------ butterflies
+// In-place DFT of size R over v[0..R-1]: X_s = sum_q v_q exp(DIR*2*pi*i*q*s/R).
+template <typename F, int DIR> MCS_DEV void dft2(C2<F>* v) {
+    const C2<F> a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+}
+
+template <typename F, int DIR> MCS_DEV void dft3(C2<F>* v) {
+    const F c = F(-0.5);
+    const F s = F(DIR) * F(0.86602540378443864676);
+    const C2<F> t = cadd(v[1], v[2]);
+    const C2<F> u = csub(v[1], v[2]);
+    const C2<F> m = mk<F>(v[0].x + c * t.x, v[0].y + c * t.y);
+    const C2<F> iu = mk<F>(-s * u.y, s * u.x);  // i*s*u
+    v[0] = cadd(v[0], t);
+    v[1] = cadd(m, iu);
+    v[2] = csub(m, iu);
+}
+
+template <typename F, int DIR> MCS_DEV void dft4(C2<F>* v) {
+    const C2<F> t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
+    const C2<F> t2 = cadd(v[1], v[3]);
+    const C2<F> t3 = cmuli<DIR>(csub(v[1], v[3]));
+    v[0] = cadd(t0, t2);
+    v[2] = csub(t0, t2);
+    v[1] = cadd(t1, t3);
+    v[3] = csub(t1, t3);
+}
+
+template <typename F, int DIR> MCS_DEV void dft5(C2<F>* v) {
+    const F c1 = F(0.30901699437494742410), c2 = F(-0.80901699437494742410);
+    const F s1 = F(DIR) * F(0.95105651629515357212), s2 = F(DIR) * F(0.58778525229247312917);
+    const C2<F> t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]);
+    const C2<F> u1 = csub(v[1], v[4]), u2 = csub(v[2], v[3]);
+    const C2<F> a0 = v[0];
+    const C2<F> m1 = mk<F>(a0.x + c1 * t1.x + c2 * t2.x, a0.y + c1 * t1.y + c2 * t2.y);
+    const C2<F> m2 = mk<F>(a0.x + c2 * t1.x + c1 * t2.x, a0.y + c2 * t1.y + c1 * t2.y);
+    const C2<F> n1 = mk<F>(s1 * u1.x + s2 * u2.x, s1 * u1.y + s2 * u2.y);
+    const C2<F> n2 = mk<F>(s2 * u1.x - s1 * u2.x, s2 * u1.y - s1 * u2.y);
+    const C2<F> in1 = mk<F>(-n1.y, n1.x), in2 = mk<F>(-n2.y, n2.x);
+    v[0] = mk<F>(a0.x + t1.x + t2.x, a0.y + t1.y + t2.y);
+    v[1] = cadd(m1, in1);
+    v[4] = csub(m1, in1);
+    v[2] = cadd(m2, in2);
+    v[3] = csub(m2, in2);
+}
+
+template <typename F, int DIR> MCS_DEV void dft8(C2<F>* v) {
+    C2<F> e[4] = {v[0], v[2], v[4], v[6]};
+    C2<F> o[4] = {v[1], v[3], v[5], v[7]};
+    dft4<F, DIR>(e);
+    dft4<F, DIR>(o);
+    const F r = F(0.70710678118654752440);
+    // o1 *= w8, o2 *= w8^2 = DIR*i, o3 *= w8^3
+    const C2<F> o1 = mk<F>(r * (o[1].x - F(DIR) * o[1].y), r * (o[1].y + F(DIR) * o[1].x));
+    const C2<F> o2 = cmuli<DIR>(o[2]);
+    const C2<F> o3 = mk<F>(r * (-o[3].x - F(DIR) * o[3].y), r * (-o[3].y + F(DIR) * o[3].x));
+    v[0] = cadd(e[0], o[0]);
+    v[4] = csub(e[0], o[0]);
+    v[1] = cadd(e[1], o1);
+    v[5] = csub(e[1], o1);
+    v[2] = cadd(e[2], o2);
+    v[6] = csub(e[2], o2);
+    v[3] = cadd(e[3], o3);
+    v[7] = csub(e[3], o3);
+}
+
+template <typename F, int R, int DIR> MCS_DEV void dft(C2<F>* v) {
+    if constexpr (R == 2) dft2<F, DIR>(v);
+    else if constexpr (R == 3) dft3<F, DIR>(v);
+    else if constexpr (R == 4) dft4<F, DIR>(v);
+    else if constexpr (R == 5) dft5<F, DIR>(v);
+    else if constexpr (R == 8) dft8<F, DIR>(v);
+}
+
+// twiddle for pass with sub-length Ns: tw[k*(R-1) + q-1] = exp(-2 pi i q k / (Ns R)) (forward)
+template <typename F, int R, int DIR>
+MCS_DEV void apply_tw(C2<F>* v, const C2<F>* __restrict__ tw, int k) {
+#pragma unroll
+    for (int q = 1; q < R; ++q) {
+        C2<F> w = tw[k * (R - 1) + (q - 1)];
+        if (DIR > 0) w.y = -w.y;
+        v[q] = cmul(v[q], w);
+    }
+}
+
+// ------------------------------------------------------------ register engine
+template <typename F, int P, int R, int DIR>
+MCS_DEV void rpass(C2<F> (&a)[P], int t, int S, int Ns, const C2<F>* __restrict__ tw, C2<F>* lds, int lbase,
+                   int lps, bool last) {
+    constexpr int B = P / R;
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+        C2<F> v[R];
+#pragma unroll
+        for (int q = 0; q < R; ++q) v[q] = a[b + q * B];
+        const int j = t + b * S;
+        const int k = j % Ns;
+        if (Ns > 1) apply_tw<F, R, DIR>(v, tw, k);
+        dft<F, R, DIR>(v);
+        if (last) {
+#pragma unroll
+            for (int q = 0; q < R; ++q) a[b + q * B] = v[q];
+        } else {
+            const int base = (j - k) * R + k;
+#pragma unroll
+            for (int q = 0; q < R; ++q) lds[lbase + (base + q * Ns) * lps] = v[q];
+        }
+    }
+}
+
+template <typename F, int P, int DIR>
+MCS_DEV void reg_fft(C2<F> (&a)[P], const FftPlanDev& pl, const C2<F>* __restrict__ tw, int t, C2<F>* lds,
+                     int lbase, int lps) {
+    const int S = pl.L / P;
+    for (int p = 0; p < pl.np; ++p) {
+        const bool last = (p == pl.np - 1);
+        const int R = pl.radix[p];
+        const int Ns = pl.ns[p];
+        const C2<F>* twp = tw + pl.twoff[p];
+        switch (R) {
+            case 2: if constexpr (P % 2 == 0) rpass<F, P, 2, DIR>(a, t, S, Ns, twp, lds, lbase, lps, last); break;
+            case 3: if constexpr (P % 3 == 0) rpass<F, P, 3, DIR>(a, t, S, Ns, twp, lds, lbase, lps, last); break;
+            case 4: if constexpr (P % 4 == 0) rpass<F, P, 4, DIR>(a, t, S, Ns, twp, lds, lbase, lps, last); break;
+            case 5: if constexpr (P % 5 == 0) rpass<F, P, 5, DIR>(a, t, S, Ns, twp, lds, lbase, lps, last); break;
+            case 8: if constexpr (P % 8 == 0) rpass<F, P, 8, DIR>(a, t, S, Ns, twp, lds, lbase, lps, last); break;
+            default: break;
+        }
+        if (!last) {
+            __syncthreads();
+#pragma unroll
+            for (int m = 0; m < P; ++m) a[m] = lds[lbase + (t + m * S) * lps];
+            __syncthreads();
+        }
+    }
+}
+
+// ------------------------------------------------------------ generic engine
+// Lines live in LDS as buf[c*Lp + pos], c in [0, G).
+template <typename F, int R, int DIR>
+MCS_DEV void gpass(const C2<F>* in, C2<F>* out, int L, int Lp, int G, int Ns, const C2<F>* __restrict__ tw, int tid,
+                   int nthr) {
+    const int nb = L / R;
+    for (int w = tid; w < G * nb; w += nthr) {
+        const int c = w / nb, j = w - c * nb;
+        C2<F> v[R];
+#pragma unroll
+        for (int q = 0; q < R; ++q) v[q] = in[c * Lp + j + q * nb];
+        const int k = j % Ns;
+        if (Ns > 1) apply_tw<F, R, DIR>(v, tw, k);
+        dft<F, R, DIR>(v);
+        const int base = (j - k) * R + k;
+#pragma unroll
+        for (int q = 0; q < R; ++q) out[c * Lp + base + q * Ns] = v[q];
+    }
+}
+
+// returns the buffer holding the result
+template <typename F, int DIR>
+MCS_DEV C2<F>* lds_fft(C2<F>* b0, C2<F>* b1, const FftPlanDev& pl, const C2<F>* __restrict__ tw, int Lp, int G,
+                       int tid, int nthr) {
+    C2<F>* in = b0;
+    C2<F>* out = b1;
+    for (int p = 0; p < pl.np; ++p) {
+        const int R = pl.radix[p], Ns = pl.ns[p];
+        const C2<F>* twp = tw + pl.twoff[p];
+        switch (R) {
+            case 2: gpass<F, 2, DIR>(in, out, pl.L, Lp, G, Ns, twp, tid, nthr); break;
+            case 3: gpass<F, 3, DIR>(in, out, pl.L, Lp, G, Ns, twp, tid, nthr); break;
+            case 4: gpass<F, 4, DIR>(in, out, pl.L, Lp, G, Ns, twp, tid, nthr); break;
+            case 5: gpass<F, 5, DIR>(in, out, pl.L, Lp, G, Ns, twp, tid, nthr); break;
+            case 8: gpass<F, 8, DIR>(in, out, pl.L, Lp, G, Ns, twp, tid, nthr); break;
+            default: break;
+        }
+        __syncthreads();
+        C2<F>* tmp = in; in = out; out = tmp;
+    }
+    return in;
+}

@@ -1,0 +1,144 @@
+// Line plans: factorisation, radix ordering and twiddle generation.
+#include "fft_plan.hpp"
+
+#include <cmath>
+#include <vector>
+
+static bool factor235(int L, int& a, int& b, int& c) {
+    a = b = c = 0;
+    int m = L;
+    while (m % 2 == 0) { m /= 2; ++a; }
+    while (m % 3 == 0) { m /= 3; ++b; }
+    while (m % 5 == 0) { m /= 5; ++c; }
+    return m == 1;
+}
+
+int line_plan_reg_P(int L) {
+    int a, b, c;
+    if (L < 4 || !factor235(L, a, b, c)) return 0;
+    int P = 0;
+    if (b == 0 && c == 0) P = (a >= 3) ? 8 : (a == 2 ? 4 : 0);
+    else if (b == 1 && c == 0 && a >= 2) P = 12;
+    else if (b == 0 && c == 1 && a >= 2) P = 20;
+    if (P == 0) return 0;
+    if (L / P > 1024) return 0;
+    return P;
+}
+
+static void reg_radices(int L, int P, std::vector<int>& r) {
+    int a, b, c;
+    factor235(L, a, b, c);
+    r.clear();
+    if (P == 8 || P == 4) {
+        if (a % 3 == 1) r.push_back(2);
+        if (a % 3 == 2) r.push_back(4);
+        for (int i = 0; i < a / 3; ++i) r.push_back(8);
+    } else {
+        r.push_back(P == 12 ? 3 : 5);
+        if (a % 2 == 1) r.push_back(2);
+        for (int i = 0; i < a / 2; ++i) r.push_back(4);
+    }
+}
+
+static void gen_radices(int L, std::vector<int>& r) {
+    int a, b, c;
+    factor235(L, a, b, c);
+    r.clear();
+    if (a % 3 == 1) r.push_back(2);
+    if (a % 3 == 2) r.push_back(4);
+    for (int i = 0; i < a / 3; ++i) r.push_back(8);
+    for (int i = 0; i < b; ++i) r.push_back(3);
+    for (int i = 0; i < c; ++i) r.push_back(5);
+}
+
+int line_plan_gen_Lp(const LinePlan& lp) { return lp.L + 1; }
+
+size_t line_plan_gen_lds(const LinePlan& lp, int G) {
+    const size_t e = lp.prec == 64 ? 16 : 8;
+    return 2 * (size_t)G * (size_t)line_plan_gen_Lp(lp) * e;
+}
+
+template <typename F> static int upload(void** dev, const std::vector<double>& re, const std::vector<double>& im) {
+    std::vector<C2<F>> h(re.size());
+    for (size_t i = 0; i < re.size(); ++i) { h[i].x = (F)re[i]; h[i].y = (F)im[i]; }
+    if (h.empty()) { *dev = nullptr; return MCS_OK; }
+    MCS_HIP_TRY(hipMalloc(dev, h.size() * sizeof(C2<F>)));
+    MCS_HIP_TRY(hipMemcpy(*dev, h.data(), h.size() * sizeof(C2<F>), hipMemcpyHostToDevice));
+    return MCS_OK;
+}
+
+// exp(-2 pi i j / n) in FP64, exact at the quarter and half turns
+static void phase(long j, long n, double& c, double& s) {
+    j %= n;
+    if (j < 0) j += n;
+    const double a = 2.0 * M_PI * (double)j / (double)n;
+    c = std::cos(a);
+    s = -std::sin(a);
+    if (n % 4 == 0) {
+        const long q = n / 4;
+        if (j == 0) { c = 1; s = 0; }
+        else if (j == q) { c = 0; s = -1; }
+        else if (j == 2 * q) { c = -1; s = 0; }
+        else if (j == 3 * q) { c = 0; s = 1; }
+    } else if (n % 2 == 0 && j == n / 2) { c = -1; s = 0; }
+}
+
+int line_plan_build(LinePlan* lp, int L, int prec, bool allow_reg) {
+    MCS_CHECK(L >= 1, MCS_ERR_INVALID, "FFT length must be >= 1");
+    MCS_CHECK(prec == 32 || prec == 64, MCS_ERR_INVALID, "precision must be 32 or 64");
+    int a, b, c;
+    MCS_CHECK(factor235(L, a, b, c), MCS_ERR_UNSUPPORTED, "FFT length must be 2,3,5-smooth");
+    lp->L = L;
+    lp->prec = prec;
+    std::vector<int> rad;
+    const int P = allow_reg ? line_plan_reg_P(L) : 0;
+    if (P) {
+        lp->engine = ENGINE_REG;
+        lp->P = P;
+        reg_radices(L, P, rad);
+        const size_t e = prec == 64 ? 16 : 8;
+        MCS_CHECK((size_t)L * e <= MCS_LDS_MAX, MCS_ERR_UNSUPPORTED, "FFT length too large for LDS");
+    } else {
+        lp->engine = ENGINE_GEN;
+        lp->P = 0;
+        gen_radices(L, rad);
+        MCS_CHECK(line_plan_gen_lds(*lp, 1) <= MCS_LDS_MAX, MCS_ERR_UNSUPPORTED, "FFT length too large for LDS");
+    }
+    MCS_CHECK((int)rad.size() <= MCS_MAX_PASSES, MCS_ERR_UNSUPPORTED, "too many FFT passes");
+    FftPlanDev& d = lp->dev;
+    d = FftPlanDev{};
+    d.L = L;
+    d.np = (int)rad.size();
+    std::vector<double> re, im;
+    int Ns = 1;
+    for (int p = 0; p < d.np; ++p) {
+        const int R = rad[p];
+        d.radix[p] = R;
+        d.ns[p] = Ns;
+        d.twoff[p] = (int)re.size();
+        if (Ns > 1) {
+            for (int k = 0; k < Ns; ++k)
+                for (int q = 1; q < R; ++q) {
+                    double cr, ci;
+                    phase((long)q * k, (long)Ns * R, cr, ci);
+                    re.push_back(cr);
+                    im.push_back(ci);
+                }
+        }
+        Ns *= R;
+    }
+    lp->twcount = re.size();
+    int st = prec == 64 ? upload<double>(&lp->tw, re, im) : upload<float>(&lp->tw, re, im);
+    return st;
+}
+
+void line_plan_free(LinePlan* lp) {
+    if (lp->tw) (void)hipFree(lp->tw);
+    lp->tw = nullptr;
+}
+
+int make_phase_table(void** dev, int n, int count, int prec) {
+    std::vector<double> re(count), im(count);
+    for (int k = 0; k < count; ++k) phase(k, n, re[k], im[k]);
+    return prec == 64 ? upload<double>(dev, re, im) : upload<float>(dev, re, im);
+}

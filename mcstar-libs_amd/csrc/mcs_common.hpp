@@ -1,0 +1,56 @@
+// Shared device/host helpers for the MI355X (gfx950) PFC library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+
+#include "../../include/mcstar_hip.h"
+
+#define MCS_DEV __device__ __forceinline__
+#define MCS_HD __host__ __device__ __forceinline__
+
+// ---------------------------------------------------------------- complex
+template <typename F> struct cplx_of;
+template <> struct cplx_of<float> { using type = float2; };
+template <> struct cplx_of<double> { using type = double2; };
+template <typename F> using C2 = typename cplx_of<F>::type;
+
+template <typename F> MCS_HD C2<F> mk(F x, F y) { C2<F> r; r.x = x; r.y = y; return r; }
+template <typename T> MCS_HD T cadd(T a, T b) { T r; r.x = a.x + b.x; r.y = a.y + b.y; return r; }
+template <typename T> MCS_HD T csub(T a, T b) { T r; r.x = a.x - b.x; r.y = a.y - b.y; return r; }
+template <typename T> MCS_HD T cmul(T a, T b) { T r; r.x = a.x * b.x - a.y * b.y; r.y = a.x * b.y + a.y * b.x; return r; }
+// a * conj(b)
+template <typename T> MCS_HD T cmulc(T a, T b) { T r; r.x = a.x * b.x + a.y * b.y; r.y = a.y * b.x - a.x * b.y; return r; }
+template <typename T> MCS_HD T cconj(T a) { T r; r.x = a.x; r.y = -a.y; return r; }
+template <typename T, typename S> MCS_HD T cscale(T a, S s) { T r; r.x = a.x * s; r.y = a.y * s; return r; }
+// multiply by +i (s = +1) or -i (s = -1)
+template <int S, typename T> MCS_HD T cmuli(T a) {
+    T r;
+    if (S > 0) { r.x = -a.y; r.y = a.x; } else { r.x = a.y; r.y = -a.x; }
+    return r;
+}
+
+// ---------------------------------------------------------------- errors
+#define MCS_HIP_TRY(expr)                                                                  \
+    do {                                                                                   \
+        hipError_t _e = (expr);                                                            \
+        if (_e != hipSuccess) {                                                            \
+            mcs_set_last_error(__FILE__, __LINE__, hipGetErrorString(_e));                 \
+            return MCS_ERR_HIP;                                                            \
+        }                                                                                  \
+    } while (0)
+
+#define MCS_CHECK(cond, code, msg)                                                         \
+    do {                                                                                   \
+        if (!(cond)) {                                                                     \
+            mcs_set_last_error(__FILE__, __LINE__, msg);                                   \
+            return (code);                                                                 \
+        }                                                                                  \
+    } while (0)
+
+void mcs_set_last_error(const char* file, int line, const char* msg);
+hipStream_t mcs_stream();  // library stream (default priority)
+
+static inline unsigned mcs_div_up(long a, long b) { return (unsigned)((a + b - 1) / b); }

@@ -1,0 +1,560 @@
+// Fused PFC time step (the hot path).
+//
+// One step of the reference's loop (cuda/kernels/psi3_split.hpp:15-26,
+// cuda/kernels/eom.hpp:60-86, cuda/noise.hpp:110-142, cufft_wrapper.hpp:81-85)
+// is c2r -> psi^3 -> r2c -> noise -> update.  Here the real field psi(x,y,z)
+// is packed as u(m,y,z) = psi(2m,y,z) + i psi(2m+1,y,z) and the state is
+// U = FFT3(u) over the (nx/2) x ny x nz grid: N/2 complex values, rows of nx/2
+// aligned elements, no Hermitian redundancy.  The full half spectrum follows
+// from a pair of packed modes:
+//     E = (U(p,y,z) + conj U(-p,-y,-z)) / 2,  O = -i (U(p,..) - conj U(-p,..)) / 2
+//     Psi(p, y, z) = E + w^p O,  Psi(p + nx/2, y, z) = E - w^p O,  w = exp(-2 pi i / nx)
+// so the semi-implicit update runs on row pairs (y,z) ~ (-y,-z) in the x pass.
+//
+// Step (3D), each a single sweep over HBM except the first:
+//   S3  x rows:  FFT_x(W) -> unpack, update with Philox noise, repack -> U;  IFFT_x -> W
+//   S2  y cols:  IFFT_y(W)
+//   S1  z cols:  IFFT_z(W), nonlinearity (psi^3 or psi^3 - psi), FFT_z
+//   S2  y cols:  FFT_y(W)
+// 10 sweeps of N/2 complex values per step (S3 reads W and U, writes U and W).
+// 2D: S3 on x rows, S1 on y columns (6 sweeps).
+#include "fft_kernels.hpp"
+#include "pfc_math.hpp"
+#include "philox.hpp"
+#include "stepper.hpp"
+
+#include <cmath>
+#include <vector>
+
+// ------------------------------------------------------------------ plain row pass (packed rows)
+template <typename F, int P, int DIR>
+__global__ void __launch_bounds__(1024) k_row_reg(const C2<F>* src, C2<F>* dst, long nrows, int G, FftPlanDev pl,
+                                                  const C2<F>* __restrict__ tw) {
+    extern __shared__ unsigned char smem_raw[];
+    C2<F>* lds = reinterpret_cast<C2<F>*>(smem_raw);
+    const int L = pl.L, S = L / P;
+    const int t = threadIdx.x % S, g = threadIdx.x / S;
+    const long row = (long)blockIdx.x * G + g;
+    const bool valid = row < nrows;
+    C2<F> a[P];
+#pragma unroll
+    for (int m = 0; m < P; ++m) a[m] = valid ? src[row * L + t + m * S] : mk<F>(0, 0);
+    reg_fft<F, P, DIR>(a, pl, tw, t, lds, g * L, 1);
+    if (valid) {
+#pragma unroll
+        for (int m = 0; m < P; ++m) dst[row * L + t + m * S] = a[m];
+    }
+}
+
+// ------------------------------------------------------------------ S1: column pass with the nonlinearity
+template <typename F, int P, int SPLIT>
+__global__ void __launch_bounds__(1024) k_col_nl(C2<F>* W, ColGeom g, int G, FftPlanDev pl,
+                                                 const C2<F>* __restrict__ tw, F scale, int64_t* d_step) {
+    extern __shared__ unsigned char smem_raw[];
+    C2<F>* lds = reinterpret_cast<C2<F>*>(smem_raw);
+    const int S = pl.L / P;
+    const int tilesx = (g.nxc + G - 1) / G;
+    const int tile = blockIdx.x % tilesx, outer = blockIdx.x / tilesx;
+    const int c = threadIdx.x % G, t = threadIdx.x / G;
+    const int x = tile * G + c;
+    const bool valid = x < g.nxc;
+    const long base = (long)outer * g.os + x;
+    C2<F> a[P];
+#pragma unroll
+    for (int m = 0; m < P; ++m) a[m] = valid ? W[base + (long)(t + m * S) * g.ls] : mk<F>(0, 0);
+    reg_fft<F, P, +1>(a, pl, tw, t, lds, c, G);
+#pragma unroll
+    for (int m = 0; m < P; ++m) {
+        const F u = a[m].x * scale, v = a[m].y * scale;
+        if (SPLIT == MCS_SPLIT_REFERENCE) a[m] = mk<F>(u * u * u - u, v * v * v - v);
+        else a[m] = mk<F>(u * u * u, v * v * v);
+    }
+    reg_fft<F, P, -1>(a, pl, tw, t, lds, c, G);
+    if (valid) {
+#pragma unroll
+        for (int m = 0; m < P; ++m) W[base + (long)(t + m * S) * g.ls] = a[m];
+    }
+    if (d_step && blockIdx.x == 0 && threadIdx.x == 0) *d_step += 1;
+}
+
+// ------------------------------------------------------------------ S3: x rows, update, inverse
+template <typename F>
+struct XArgs {
+    int nx, ny, nz, L;
+    F lx, ly, lz, dt, lin0, amp;
+    uint32_t key0, key1;
+    int noise;
+    double mass;  // FP64 k=0 mode (sum of psi), kept exactly
+};
+
+template <typename F, int SPLIT>
+MCS_DEV C2<F> xupd_elem(C2<F> nA, C2<F> nBr, C2<F> pA, C2<F> pBr, int p, int iy, int iz, int my, int mz, C2<F> w,
+                        const XArgs<F>& ar, uint32_t draw) {
+    const C2<F> nB = cconj(nBr), pB = cconj(pBr);
+    const F h = F(0.5);
+    const C2<F> eN = cscale(cadd(nA, nB), h), oN = cscale(cmuli<-1>(csub(nA, nB)), h);
+    const C2<F> eP = cscale(cadd(pA, pB), h), oP = cscale(cmuli<-1>(csub(pA, pB)), h);
+    const C2<F> woN = cmul(w, oN), woP = cmul(w, oP);
+    const C2<F> n1 = cadd(eN, woN), n2 = csub(eN, woN);
+    const C2<F> p1 = cadd(eP, woP), p2 = csub(eP, woP);
+    const F ky = kwave<F>(iy, ar.ny, ar.ly), kz = kwave<F>(iz, ar.nz, ar.lz);
+    const F kyz = ky * ky + kz * kz;
+    const F kx1 = kwave<F>(p, ar.nx, ar.lx);
+    const F kx2 = kwave<F>(ar.L - p, ar.nx, ar.lx);  // |k_x| of index p + nx/2
+    C2<F> xi1 = mk<F>(0, 0), xi2 = mk<F>(0, 0);
+    if (ar.noise) {
+        xi1 = noise_xi<F>(p, iy, iz, ar.nx, ar.ny, ar.nz, ar.key0, ar.key1, draw);
+        xi2 = p == 0 ? noise_xi<F>(ar.L, iy, iz, ar.nx, ar.ny, ar.nz, ar.key0, ar.key1, draw)
+                     : cconj(noise_xi<F>(ar.L - p, my, mz, ar.nx, ar.ny, ar.nz, ar.key0, ar.key1, draw));
+    }
+    const C2<F> q1 = pfc_mode_update<F>(p1, n1, xi1, kx1 * kx1 + kyz, ar.dt, ar.lin0, ar.amp);
+    const C2<F> q2 = pfc_mode_update<F>(p2, n2, xi2, kx2 * kx2 + kyz, ar.dt, ar.lin0, ar.amp);
+    if (p == 0 && iy == 0 && iz == 0) {
+        // k = 0: the mean density is carried in FP64 and re-imposed exactly
+        const double m = ar.mass;
+        const double er = 0.5 * (m + (double)q2.x), ei = 0.5 * (double)q2.y;
+        const double or_ = 0.5 * (m - (double)q2.x), oi = -0.5 * (double)q2.y;
+        return mk<F>((F)(er - oi), (F)(ei + or_));
+    }
+    const C2<F> e = cscale(cadd(q1, q2), h);
+    const C2<F> o = cscale(cmulc(csub(q1, q2), w), h);
+    return cadd(e, cmuli<+1>(o));
+}
+
+template <typename F, int P, int SPLIT>
+__global__ void __launch_bounds__(256) k_xupd(C2<F>* __restrict__ W, C2<F>* __restrict__ U, const int2* __restrict__ pairs,
+                                              int npairs, int G, XArgs<F> ar, FftPlanDev pl,
+                                              const C2<F>* __restrict__ tw, const C2<F>* __restrict__ wph,
+                                              const int64_t* __restrict__ d_step) {
+    extern __shared__ unsigned char smem_raw[];
+    C2<F>* lds = reinterpret_cast<C2<F>*>(smem_raw);
+    const int L = pl.L, S = L / P;
+    const int t = threadIdx.x % S, g = threadIdx.x / S;
+    const int q = blockIdx.x * G + g;
+    const bool valid = q < npairs;
+    const int2 pr = pairs[valid ? q : 0];
+    const long rc = pr.x, rm = pr.y;
+    const int base = g * 4 * L;
+    C2<F>* Nc = lds + base;
+    C2<F>* Nm = Nc + L;
+    C2<F>* Pc = Nm + L;
+    C2<F>* Pm = Pc + L;
+    const uint32_t draw = ar.noise ? (uint32_t)*d_step : 0u;
+
+    C2<F> a[P], b[P];
+#pragma unroll
+    for (int m = 0; m < P; ++m) {
+        a[m] = valid ? W[rc * L + t + m * S] : mk<F>(0, 0);
+        b[m] = valid ? W[rm * L + t + m * S] : mk<F>(0, 0);
+    }
+    reg_fft<F, P, -1>(a, pl, tw, t, lds, base, 1);
+    reg_fft<F, P, -1>(b, pl, tw, t, lds, base + L, 1);
+#pragma unroll
+    for (int m = 0; m < P; ++m) {
+        const int p = t + m * S;
+        Nc[p] = a[m];
+        Nm[p] = b[m];
+        Pc[p] = valid ? U[rc * L + p] : mk<F>(0, 0);
+        Pm[p] = valid ? U[rm * L + p] : mk<F>(0, 0);
+    }
+    __syncthreads();
+    const int iyc = (int)(rc % ar.ny), izc = (int)(rc / ar.ny);
+    const int iym = (int)(rm % ar.ny), izm = (int)(rm / ar.ny);
+#pragma unroll
+    for (int m = 0; m < P; ++m) {
+        const int p = t + m * S;
+        const int pp = p == 0 ? 0 : L - p;
+        const C2<F> w = wph[p];
+        a[m] = xupd_elem<F, SPLIT>(Nc[p], Nm[pp], Pc[p], Pm[pp], p, iyc, izc, iym, izm, w, ar, draw);
+        b[m] = xupd_elem<F, SPLIT>(Nm[p], Nc[pp], Pm[p], Pc[pp], p, iym, izm, iyc, izc, w, ar, draw);
+    }
+    __syncthreads();
+    if (valid) {
+#pragma unroll
+        for (int m = 0; m < P; ++m) {
+            U[rc * L + t + m * S] = a[m];
+            U[rm * L + t + m * S] = b[m];
+        }
+    }
+    reg_fft<F, P, +1>(a, pl, tw, t, lds, base, 1);
+    reg_fft<F, P, +1>(b, pl, tw, t, lds, base + L, 1);
+    if (valid) {
+#pragma unroll
+        for (int m = 0; m < P; ++m) {
+            W[rc * L + t + m * S] = a[m];
+            W[rm * L + t + m * S] = b[m];
+        }
+    }
+}
+
+// ------------------------------------------------------------------ helpers
+template <typename F>
+__global__ void __launch_bounds__(256) k_scale_copy(const F* src, F* dst, long n, F s) {
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) dst[i] = src[i] * s;
+}
+
+template <typename F>
+__global__ void __launch_bounds__(256) k_sum_f64(const F* a, long n, double* part) {
+    __shared__ double s[256];
+    double acc = 0;
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) acc += (double)a[i];
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = s[0];
+}
+
+static int set_lds_attr(const void* fn, size_t bytes) {
+    if (bytes > 65536) MCS_HIP_TRY(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+    return MCS_OK;
+}
+
+template <typename F, int P, int DIR>
+static int launch_row_reg_t(const LinePlan& lp, const void* src, void* dst, long nrows, hipStream_t s) {
+    const int S = lp.L / P;
+    int G = S >= 256 ? 1 : 256 / S;
+    const size_t esz = sizeof(C2<F>);
+    while (G > 1 && (size_t)G * lp.L * esz > 65536) --G;
+    const size_t lds = (size_t)G * lp.L * esz;
+    auto fn = k_row_reg<F, P, DIR>;
+    int st = set_lds_attr((const void*)fn, lds);
+    if (st) return st;
+    hipLaunchKernelGGL(fn, dim3((unsigned)((nrows + G - 1) / G)), dim3(G * S), lds, s, (const C2<F>*)src, (C2<F>*)dst,
+                       nrows, G, lp.dev, (const C2<F>*)lp.tw);
+    MCS_HIP_TRY(hipGetLastError());
+    return MCS_OK;
+}
+
+template <typename F, int DIR>
+static int launch_row_reg_d(const LinePlan& lp, const void* src, void* dst, long nrows, hipStream_t s) {
+    switch (lp.P) {
+        case 4: return launch_row_reg_t<F, 4, DIR>(lp, src, dst, nrows, s);
+        case 8: return launch_row_reg_t<F, 8, DIR>(lp, src, dst, nrows, s);
+        case 12: return launch_row_reg_t<F, 12, DIR>(lp, src, dst, nrows, s);
+        case 20: return launch_row_reg_t<F, 20, DIR>(lp, src, dst, nrows, s);
+        default: mcs_set_last_error(__FILE__, __LINE__, "row plan not register-eligible"); return MCS_ERR_UNSUPPORTED;
+    }
+}
+
+int launch_row_packed(const LinePlan& lp, const void* src, void* dst, long nrows, int dir, hipStream_t s) {
+    if (lp.prec == 64)
+        return dir < 0 ? launch_row_reg_d<double, -1>(lp, src, dst, nrows, s) : launch_row_reg_d<double, +1>(lp, src, dst, nrows, s);
+    return dir < 0 ? launch_row_reg_d<float, -1>(lp, src, dst, nrows, s) : launch_row_reg_d<float, +1>(lp, src, dst, nrows, s);
+}
+
+template <typename F, int P, int SPLIT>
+static int launch_col_nl_t(const LinePlan& lp, void* W, const ColGeom& g, double scale, int64_t* d_step,
+                           hipStream_t s) {
+    const int S = lp.L / P;
+    const size_t esz = sizeof(C2<F>);
+    int G = (int)(128 / esz);
+    if (G > 1024 / S) G = 1024 / S;
+    while (G > 1 && (size_t)G * lp.L * esz > 65536) --G;
+    if (G < 1) G = 1;
+    const size_t lds = (size_t)G * lp.L * esz;
+    auto fn = k_col_nl<F, P, SPLIT>;
+    int st = set_lds_attr((const void*)fn, lds);
+    if (st) return st;
+    const long tiles = (long)((g.nxc + G - 1) / G) * g.nouter;
+    hipLaunchKernelGGL(fn, dim3((unsigned)tiles), dim3(G * S), lds, s, (C2<F>*)W, g, G, lp.dev, (const C2<F>*)lp.tw,
+                       (F)scale, d_step);
+    MCS_HIP_TRY(hipGetLastError());
+    return MCS_OK;
+}
+
+template <typename F, int SPLIT>
+static int launch_col_nl_d(const LinePlan& lp, void* W, const ColGeom& g, double scale, int64_t* d_step,
+                           hipStream_t s) {
+    switch (lp.P) {
+        case 4: return launch_col_nl_t<F, 4, SPLIT>(lp, W, g, scale, d_step, s);
+        case 8: return launch_col_nl_t<F, 8, SPLIT>(lp, W, g, scale, d_step, s);
+        case 12: return launch_col_nl_t<F, 12, SPLIT>(lp, W, g, scale, d_step, s);
+        case 20: return launch_col_nl_t<F, 20, SPLIT>(lp, W, g, scale, d_step, s);
+        default: mcs_set_last_error(__FILE__, __LINE__, "column plan not register-eligible"); return MCS_ERR_UNSUPPORTED;
+    }
+}
+
+int launch_col_nl(const LinePlan& lp, void* W, const ColGeom& g, double scale, int split, int64_t* d_step,
+                  hipStream_t s) {
+    if (lp.prec == 64)
+        return split == MCS_SPLIT_REFERENCE ? launch_col_nl_d<double, MCS_SPLIT_REFERENCE>(lp, W, g, scale, d_step, s)
+                                            : launch_col_nl_d<double, MCS_SPLIT_NORTHSTAR>(lp, W, g, scale, d_step, s);
+    return split == MCS_SPLIT_REFERENCE ? launch_col_nl_d<float, MCS_SPLIT_REFERENCE>(lp, W, g, scale, d_step, s)
+                                        : launch_col_nl_d<float, MCS_SPLIT_NORTHSTAR>(lp, W, g, scale, d_step, s);
+}
+
+template <typename F, int P, int SPLIT>
+static int launch_xupd_t(const StepGeom& sg, void* W, void* U, const XArgs<F>& ar, hipStream_t s) {
+    const LinePlan& lp = sg.px;
+    const int S = lp.L / P;
+    const size_t esz = sizeof(C2<F>);
+    int G = S >= 256 ? 1 : 256 / S;
+    while (G > 1 && (size_t)G * 4 * lp.L * esz > 65536) --G;
+    const size_t lds = (size_t)G * 4 * lp.L * esz;
+    auto fn = k_xupd<F, P, SPLIT>;
+    int st = set_lds_attr((const void*)fn, lds);
+    if (st) return st;
+    hipLaunchKernelGGL(fn, dim3((unsigned)((sg.npairs + G - 1) / G)), dim3(G * S), lds, s, (C2<F>*)W, (C2<F>*)U,
+                       sg.pairs, sg.npairs, G, ar, lp.dev, (const C2<F>*)lp.tw, (const C2<F>*)sg.wph, sg.d_step);
+    MCS_HIP_TRY(hipGetLastError());
+    return MCS_OK;
+}
+
+template <typename F, int SPLIT>
+static int launch_xupd_d(const StepGeom& sg, void* W, void* U, const XArgs<F>& ar, hipStream_t s) {
+    switch (sg.px.P) {
+        case 4: return launch_xupd_t<F, 4, SPLIT>(sg, W, U, ar, s);
+        case 8: return launch_xupd_t<F, 8, SPLIT>(sg, W, U, ar, s);
+        case 12: return launch_xupd_t<F, 12, SPLIT>(sg, W, U, ar, s);
+        case 20: return launch_xupd_t<F, 20, SPLIT>(sg, W, U, ar, s);
+        default: mcs_set_last_error(__FILE__, __LINE__, "x plan not register-eligible"); return MCS_ERR_UNSUPPORTED;
+    }
+}
+
+template <typename F>
+static XArgs<F> make_xargs(const StepGeom& sg, const StepPhys& ph) {
+    XArgs<F> ar;
+    ar.nx = sg.nx; ar.ny = sg.ny; ar.nz = sg.nz; ar.L = sg.L;
+    ar.lx = (F)(sg.nx * ph.h[0]); ar.ly = (F)(sg.ny * ph.h[1]); ar.lz = (F)(sg.nz * ph.h[2]);
+    ar.dt = (F)ph.dt;
+    ar.lin0 = (F)(ph.split == MCS_SPLIT_REFERENCE ? 1.0 - ph.eps : -ph.eps);
+    const int ext[3] = {sg.nx, sg.ny, sg.nz};
+    ar.amp = (F)pfc_noise_amp(ext, ph.h, ph.dt, ph.temp);
+    ar.key0 = (uint32_t)ph.seed; ar.key1 = (uint32_t)(ph.seed >> 32);
+    ar.noise = ph.temp > 0;
+    ar.mass = ph.mass;
+    return ar;
+}
+
+int launch_xupd(const StepGeom& sg, const StepPhys& ph, void* W, void* U, hipStream_t s) {
+    if (sg.prec == 64) {
+        const XArgs<double> ar = make_xargs<double>(sg, ph);
+        return ph.split == MCS_SPLIT_REFERENCE ? launch_xupd_d<double, MCS_SPLIT_REFERENCE>(sg, W, U, ar, s)
+                                               : launch_xupd_d<double, MCS_SPLIT_NORTHSTAR>(sg, W, U, ar, s);
+    }
+    const XArgs<float> ar = make_xargs<float>(sg, ph);
+    return ph.split == MCS_SPLIT_REFERENCE ? launch_xupd_d<float, MCS_SPLIT_REFERENCE>(sg, W, U, ar, s)
+                                           : launch_xupd_d<float, MCS_SPLIT_NORTHSTAR>(sg, W, U, ar, s);
+}
+
+// ------------------------------------------------------------------ geometry
+int step_geom_build(StepGeom* sg, const int ext[3], int prec) {
+    MCS_CHECK(ext[0] >= 8 && ext[0] % 2 == 0, MCS_ERR_UNSUPPORTED, "stepper needs an even nx >= 8");
+    MCS_CHECK(ext[1] >= 4, MCS_ERR_UNSUPPORTED, "stepper needs ny >= 4");
+    MCS_CHECK(ext[2] == 1 || ext[2] >= 4, MCS_ERR_UNSUPPORTED, "stepper needs nz == 1 or nz >= 4");
+    sg->nx = ext[0]; sg->ny = ext[1]; sg->nz = ext[2];
+    sg->L = ext[0] / 2;
+    sg->prec = prec;
+    sg->is3d = ext[2] > 1;
+    MCS_CHECK(line_plan_reg_P(sg->L) && line_plan_reg_P(sg->ny) && (!sg->is3d || line_plan_reg_P(sg->nz)),
+              MCS_ERR_UNSUPPORTED, "stepper axis lengths must be 2^a, 3*2^a or 5*2^a (nx/2, ny, nz >= 4)");
+    int st = line_plan_build(&sg->px, sg->L, prec);
+    if (!st) st = line_plan_build(&sg->py, sg->ny, prec);
+    if (!st && sg->is3d) st = line_plan_build(&sg->pz, sg->nz, prec);
+    if (!st) st = make_phase_table(&sg->wph, sg->nx, sg->L, prec);
+    if (st) return st;
+    // canonical row pairs (r, mirror r) with r <= mirror
+    std::vector<int2> pr;
+    for (int iz = 0; iz < sg->nz; ++iz)
+        for (int iy = 0; iy < sg->ny; ++iy) {
+            const int my = iy == 0 ? 0 : sg->ny - iy, mz = iz == 0 ? 0 : sg->nz - iz;
+            const int r = iz * sg->ny + iy, m = mz * sg->ny + my;
+            if (r <= m) pr.push_back(make_int2(r, m));
+        }
+    sg->npairs = (int)pr.size();
+    MCS_HIP_TRY(hipMalloc((void**)&sg->pairs, pr.size() * sizeof(int2)));
+    MCS_HIP_TRY(hipMemcpy(sg->pairs, pr.data(), pr.size() * sizeof(int2), hipMemcpyHostToDevice));
+    MCS_HIP_TRY(hipMalloc((void**)&sg->d_step, sizeof(int64_t)));
+    MCS_HIP_TRY(hipMemset(sg->d_step, 0, sizeof(int64_t)));
+    return MCS_OK;
+}
+
+void step_geom_free(StepGeom* sg) {
+    line_plan_free(&sg->px);
+    line_plan_free(&sg->py);
+    line_plan_free(&sg->pz);
+    if (sg->wph) (void)hipFree(sg->wph);
+    if (sg->pairs) (void)hipFree(sg->pairs);
+    if (sg->d_step) (void)hipFree(sg->d_step);
+    sg->wph = nullptr; sg->pairs = nullptr; sg->d_step = nullptr;
+}
+
+// column geometries on the packed [nz][ny][L] array
+static ColGeom geom_y(const StepGeom& sg) { return ColGeom{sg.L, (long)sg.L, sg.nz, (long)sg.L * sg.ny}; }
+static ColGeom geom_z(const StepGeom& sg) { return ColGeom{sg.L, (long)sg.L * sg.ny, sg.ny, (long)sg.L}; }
+
+// W holds IFFT_x(U) on entry; leaves FFT_yz(N(psi)) in W
+int step_nonlinear_half(const StepGeom& sg, const StepPhys& ph, void* W, int64_t* d_step, hipStream_t s) {
+    const double scale = 2.0 / ((double)sg.nx * sg.ny * sg.nz);
+    int st;
+    if (sg.is3d) {
+        st = launch_col(sg.py, W, W, geom_y(sg), +1, s);
+        if (!st) st = launch_col_nl(sg.pz, W, geom_z(sg), scale, ph.split, d_step, s);
+        if (!st) st = launch_col(sg.py, W, W, geom_y(sg), -1, s);
+    } else {
+        st = launch_col_nl(sg.py, W, geom_y(sg), scale, ph.split, d_step, s);
+    }
+    return st;
+}
+
+int step_run(const StepGeom& sg, const StepPhys& ph, void* W, void* U, int nsteps, hipStream_t s) {
+    for (int i = 0; i < nsteps; ++i) {
+        int st = launch_xupd(sg, ph, W, U, s);
+        if (!st) st = step_nonlinear_half(sg, ph, W, sg.d_step, s);
+        if (st) return st;
+    }
+    return MCS_OK;
+}
+
+// psi (real, reference layout, N values) -> U; W primed with the nonlinear term; returns sum(psi) in *mass
+int step_import(const StepGeom& sg, const StepPhys& ph, const void* psi, void* W, void* U, double* mass,
+                hipStream_t s) {
+    const long rows = (long)sg.ny * sg.nz;
+    const long n = (long)sg.nx * sg.ny * sg.nz;
+    int st = launch_row_packed(sg.px, psi, U, rows, -1, s);
+    if (!st && sg.is3d) st = launch_col(sg.pz, U, U, geom_z(sg), -1, s);
+    if (!st) st = launch_col(sg.py, U, U, geom_y(sg), -1, s);
+    if (st) return st;
+    double* part = nullptr;
+    MCS_HIP_TRY(hipMalloc((void**)&part, 1024 * sizeof(double)));
+    if (sg.prec == 64) hipLaunchKernelGGL(k_sum_f64<double>, dim3(1024), dim3(256), 0, s, (const double*)psi, n, part);
+    else hipLaunchKernelGGL(k_sum_f64<float>, dim3(1024), dim3(256), 0, s, (const float*)psi, n, part);
+    std::vector<double> hp(1024);
+    st = hipMemcpyAsync(hp.data(), part, 1024 * sizeof(double), hipMemcpyDeviceToHost, s) == hipSuccess &&
+                 hipStreamSynchronize(s) == hipSuccess
+             ? MCS_OK
+             : MCS_ERR_HIP;
+    (void)hipFree(part);
+    if (st) { mcs_set_last_error(__FILE__, __LINE__, "mass reduction failed"); return st; }
+    double m = 0;
+    for (double v : hp) m += v;
+    *mass = m;
+    st = launch_row_packed(sg.px, U, W, rows, +1, s);
+    if (!st) st = step_nonlinear_half(sg, ph, W, nullptr, s);
+    return st;
+}
+
+// U -> psi (real, reference layout); tmp must hold N/2 complex values
+int step_export(const StepGeom& sg, const void* U, void* psi, void* tmp, hipStream_t s) {
+    const long rows = (long)sg.ny * sg.nz;
+    const long n = (long)sg.nx * sg.ny * sg.nz;
+    int st = launch_row_packed(sg.px, U, tmp, rows, +1, s);
+    if (!st) st = launch_col(sg.py, tmp, tmp, geom_y(sg), +1, s);
+    if (!st && sg.is3d) st = launch_col(sg.pz, tmp, tmp, geom_z(sg), +1, s);
+    if (st) return st;
+    const double sc = 1.0 / (double)n;
+    if (sg.prec == 64)
+        hipLaunchKernelGGL(k_scale_copy<double>, dim3(2048), dim3(256), 0, s, (const double*)tmp, (double*)psi, n, sc);
+    else
+        hipLaunchKernelGGL(k_scale_copy<float>, dim3(2048), dim3(256), 0, s, (const float*)tmp, (float*)psi, n,
+                           (float)sc);
+    MCS_HIP_TRY(hipGetLastError());
+    return MCS_OK;
+}
+
+// ------------------------------------------------------------------ C ABI
+struct mcs_stepper {
+    mcs_pfc_params p;
+    StepGeom sg;
+    StepPhys ph;
+    void* U = nullptr;
+    void* W = nullptr;
+    bool loaded = false;
+    int64_t step = 0;
+};
+
+extern "C" int mcs_stepper_create(mcs_stepper** out, const mcs_pfc_params* p) {
+    MCS_CHECK(out && p, MCS_ERR_INVALID, "null argument");
+    MCS_CHECK(p->ext[0] >= 1 && p->ext[1] >= 1 && p->ext[2] >= 1, MCS_ERR_INVALID, "extent must be >=1");
+    MCS_CHECK(p->prec == 32 || p->prec == 64, MCS_ERR_INVALID, "precision must be 32 or 64");
+    MCS_CHECK(p->split == MCS_SPLIT_NORTHSTAR || p->split == MCS_SPLIT_REFERENCE, MCS_ERR_INVALID, "bad split");
+    MCS_CHECK(p->dt > 0 && p->h[0] > 0 && p->h[1] > 0 && p->h[2] > 0, MCS_ERR_INVALID, "dt and h must be positive");
+    (void)mcs_stream();
+    auto* st = new mcs_stepper();
+    st->p = *p;
+    int rc = step_geom_build(&st->sg, p->ext, p->prec);
+    const size_t bytes = (size_t)p->ext[0] * p->ext[1] * p->ext[2] * (p->prec == 64 ? 8 : 4);
+    if (!rc && (hipMalloc(&st->U, bytes) != hipSuccess || hipMalloc(&st->W, bytes) != hipSuccess)) {
+        mcs_set_last_error(__FILE__, __LINE__, "stepper state allocation failed");
+        rc = MCS_ERR_NOMEM;
+    }
+    if (rc) { mcs_stepper_destroy(st); return rc; }
+    st->ph.h[0] = p->h[0]; st->ph.h[1] = p->h[1]; st->ph.h[2] = p->h[2];
+    st->ph.eps = p->eps; st->ph.dt = p->dt; st->ph.temp = p->temp; st->ph.seed = p->seed; st->ph.split = p->split;
+    st->ph.mass = 0;
+    *out = st;
+    return MCS_OK;
+}
+
+extern "C" int mcs_stepper_destroy(mcs_stepper* st) {
+    if (!st) return MCS_OK;
+    step_geom_free(&st->sg);
+    if (st->U) (void)hipFree(st->U);
+    if (st->W) (void)hipFree(st->W);
+    delete st;
+    return MCS_OK;
+}
+
+extern "C" int mcs_stepper_set_psi(mcs_stepper* st, const void* psi) {
+    MCS_CHECK(st && psi, MCS_ERR_INVALID, "null argument");
+    hipStream_t s = mcs_stream();
+    // the draw number read by the x pass is the step counter
+    MCS_HIP_TRY(hipMemcpyAsync(st->sg.d_step, &st->step, sizeof(int64_t), hipMemcpyHostToDevice, s));
+    int rc = step_import(st->sg, st->ph, psi, st->W, st->U, &st->ph.mass, s);
+    if (rc) return rc;
+    MCS_HIP_TRY(hipStreamSynchronize(s));
+    st->loaded = true;
+    return MCS_OK;
+}
+
+extern "C" int mcs_stepper_get_psi(mcs_stepper* st, void* psi) {
+    MCS_CHECK(st && psi, MCS_ERR_INVALID, "null argument");
+    MCS_CHECK(st->loaded, MCS_ERR_INVALID, "stepper has no field (call mcs_stepper_set_psi)");
+    hipStream_t s = mcs_stream();
+    const size_t bytes = (size_t)st->sg.nx * st->sg.ny * st->sg.nz * (st->p.prec == 64 ? 8 : 4);
+    void* tmp = nullptr;
+    MCS_HIP_TRY(hipMalloc(&tmp, bytes));
+    int rc = step_export(st->sg, st->U, psi, tmp, s);
+    if (!rc && hipStreamSynchronize(s) != hipSuccess) rc = MCS_ERR_HIP;
+    (void)hipFree(tmp);
+    return rc;
+}
+
+extern "C" int mcs_stepper_step_async(mcs_stepper* st, int nsteps) {
+    MCS_CHECK(st && nsteps >= 0, MCS_ERR_INVALID, "bad argument");
+    MCS_CHECK(st->loaded, MCS_ERR_INVALID, "stepper has no field (call mcs_stepper_set_psi)");
+    int rc = step_run(st->sg, st->ph, st->W, st->U, nsteps, mcs_stream());
+    if (rc) return rc;
+    st->step += nsteps;
+    return MCS_OK;
+}
+
+extern "C" int mcs_stepper_step(mcs_stepper* st, int nsteps) {
+    int rc = mcs_stepper_step_async(st, nsteps);
+    if (rc) return rc;
+    MCS_HIP_TRY(hipStreamSynchronize(mcs_stream()));
+    return MCS_OK;
+}
+
+extern "C" int mcs_stepper_get_step(mcs_stepper* st, int64_t* step) {
+    MCS_CHECK(st && step, MCS_ERR_INVALID, "null argument");
+    *step = st->step;
+    return MCS_OK;
+}
+
+extern "C" int mcs_stepper_set_step(mcs_stepper* st, int64_t step) {
+    MCS_CHECK(st && step >= 0, MCS_ERR_INVALID, "bad argument");
+    st->step = step;
+    MCS_HIP_TRY(hipMemcpyAsync(st->sg.d_step, &st->step, sizeof(int64_t), hipMemcpyHostToDevice, mcs_stream()));
+    MCS_HIP_TRY(hipStreamSynchronize(mcs_stream()));
+    return MCS_OK;
+}
+
+extern "C" int mcs_stepper_mass(mcs_stepper* st, double* mean) {
+    MCS_CHECK(st && mean, MCS_ERR_INVALID, "null argument");
+    *mean = st->ph.mass / ((double)st->sg.nx * st->sg.ny * st->sg.nz);
+    return MCS_OK;
+}

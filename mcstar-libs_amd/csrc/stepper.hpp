@@ -1,0 +1,31 @@
+// Fused-step building blocks shared by the single-grid stepper and the ensemble.
+#pragma once
+
+#include "fft_kernels.hpp"
+
+struct StepGeom {
+    int nx = 0, ny = 0, nz = 0, L = 0, prec = 32;
+    bool is3d = false;
+    LinePlan px, py, pz;    // x: nx/2 (packed), y: ny, z: nz
+    void* wph = nullptr;    // exp(-2 pi i p / nx), p in [0, nx/2)
+    int2* pairs = nullptr;  // canonical (row, mirror row) pairs
+    int npairs = 0;
+    int64_t* d_step = nullptr;  // device step counter (Philox draw number)
+};
+
+struct StepPhys {
+    double h[3];
+    double eps, dt, temp;
+    uint64_t seed;
+    int split;
+    double mass;  // sum of psi over the grid (the k = 0 mode), FP64
+};
+
+int step_geom_build(StepGeom* sg, const int ext[3], int prec);
+void step_geom_free(StepGeom* sg);
+int step_import(const StepGeom& sg, const StepPhys& ph, const void* psi, void* W, void* U, double* mass, hipStream_t s);
+int step_export(const StepGeom& sg, const void* U, void* psi, void* tmp, hipStream_t s);
+int step_run(const StepGeom& sg, const StepPhys& ph, void* W, void* U, int nsteps, hipStream_t s);
+int launch_row_packed(const LinePlan& lp, const void* src, void* dst, long nrows, int dir, hipStream_t s);
+
+double pfc_noise_amp(const int ext[3], const double h[3], double dt, double temp);

@@ -1,0 +1,188 @@
+"""ctypes binding of the FP64 CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, always as the checker.  The product path never
+imports this module.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+_d = ctypes.POINTER(ctypes.c_double)
+_i = ctypes.POINTER(ctypes.c_int)
+_i64 = ctypes.POINTER(ctypes.c_int64)
+_u32 = ctypes.POINTER(ctypes.c_uint32)
+
+SPLIT_NORTHSTAR = 0
+SPLIT_REFERENCE = 1
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE, "liboracle.so"], check=True)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        L = ctypes.CDLL(path)
+        L.orc_fft_c2c.argtypes = [_d, ctypes.c_int, ctypes.c_int]
+        L.orc_r2c.argtypes = [_d, _d, _i]
+        L.orc_c2r.argtypes = [_d, _d, _i]
+        L.orc_philox4x32_10.argtypes = [_u32, _u32, _u32]
+        L.orc_philox_stream.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                        ctypes.c_int64, _u32]
+        L.orc_kspace_noise.argtypes = [_d, _i, ctypes.c_uint64, ctypes.c_uint32]
+        L.orc_init_field.argtypes = [_d, _i, ctypes.c_double, ctypes.c_double, ctypes.c_uint64]
+        L.orc_update.argtypes = [_d, _d, _d, _i, _d, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                 ctypes.c_int]
+        L.orc_pfc_run.argtypes = [_d, _i, _d, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                  ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_int, _d]
+        L.orc_free_energy.argtypes = [_d, _i, _d, ctypes.c_double]
+        L.orc_free_energy.restype = ctypes.c_double
+        L.orc_findpeaks.argtypes = [_d, _i, ctypes.c_double, _i64, ctypes.c_int64]
+        L.orc_findpeaks.restype = ctypes.c_int64
+        L.orc_bond_order_cutoff.argtypes = [_d, ctypes.c_int, ctypes.c_double, ctypes.c_double, _i, _d]
+        L.orc_bond_order_knn.argtypes = [_i, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, _i, _d]
+        L.orc_ylm.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, _d, _d]
+        L.orc_wigner3j.argtypes = [ctypes.c_int] * 6
+        L.orc_wigner3j.restype = ctypes.c_double
+        _LIB = L
+    return _LIB
+
+
+def _p(a, t=_d):
+    return a.ctypes.data_as(t)
+
+
+def _dims(shape_zyx):
+    """numpy shape (nz, ny, nx) / (ny, nx) / (nx,) -> int[3] {nx, ny, nz}"""
+    s = list(shape_zyx)[::-1] + [1] * (3 - len(shape_zyx))
+    return np.array(s, dtype=np.int32)
+
+
+def half_shape(shape_zyx):
+    s = list(shape_zyx)
+    s[-1] = s[-1] // 2 + 1
+    return tuple(s)
+
+
+def r2c(x):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    out = np.empty(half_shape(x.shape), dtype=np.complex128)
+    lib().orc_r2c(_p(x), _p(out), _p(_dims(x.shape), _i))
+    return out
+
+
+def c2r(k, shape_zyx):
+    k = np.ascontiguousarray(k, dtype=np.complex128)
+    out = np.empty(tuple(shape_zyx), dtype=np.float64)
+    lib().orc_c2r(_p(k), _p(out), _p(_dims(shape_zyx), _i))
+    return out
+
+
+def fft_c2c(x, sign=-1):
+    a = np.ascontiguousarray(x, dtype=np.complex128).copy()
+    lib().orc_fft_c2c(_p(a), a.size, sign)
+    return a
+
+
+def philox(ctr, key):
+    c = np.ascontiguousarray(ctr, dtype=np.uint32)
+    k = np.ascontiguousarray(key, dtype=np.uint32)
+    o = np.empty(4, dtype=np.uint32)
+    lib().orc_philox4x32_10(_p(c, _u32), _p(k, _u32), _p(o, _u32))
+    return o
+
+
+def philox_stream(seed, c0, c2, c3, n):
+    o = np.empty(4 * n, dtype=np.uint32)
+    lib().orc_philox_stream(seed, c0, c2, c3, n, _p(o, _u32))
+    return o
+
+
+def kspace_noise(shape_zyx, seed, step):
+    out = np.empty(half_shape(shape_zyx), dtype=np.complex128)
+    lib().orc_kspace_noise(_p(out), _p(_dims(shape_zyx), _i), seed, step)
+    return out
+
+
+def init_field(shape_zyx, psi0, amp, seed):
+    out = np.empty(tuple(shape_zyx), dtype=np.float64)
+    lib().orc_init_field(_p(out), _p(_dims(shape_zyx), _i), psi0, amp, seed)
+    return out
+
+
+def update(kpsi, knl, xi, shape_zyx, lens, dt, eps, noise_amp, split):
+    k = np.ascontiguousarray(kpsi, dtype=np.complex128).copy()
+    nl = np.ascontiguousarray(knl, dtype=np.complex128)
+    xp = None if xi is None else _p(np.ascontiguousarray(xi, dtype=np.complex128))
+    ln = np.ascontiguousarray(lens, dtype=np.float64)
+    lib().orc_update(_p(k), _p(nl), xp, _p(_dims(shape_zyx), _i), _p(ln), dt, eps, noise_amp, split)
+    return k
+
+
+def pfc_run(psi, h, eps, dt, temp, seed, step0, nsteps, split=SPLIT_NORTHSTAR, want_mass=False):
+    p = np.ascontiguousarray(psi, dtype=np.float64).copy()
+    hh = np.ascontiguousarray(h, dtype=np.float64)
+    mass = np.empty(nsteps + 1, dtype=np.float64) if want_mass else None
+    lib().orc_pfc_run(_p(p), _p(_dims(p.shape), _i), _p(hh), eps, dt, temp, seed, step0, nsteps, split,
+                      None if mass is None else _p(mass))
+    return (p, mass) if want_mass else p
+
+
+def free_energy(psi, h, eps):
+    p = np.ascontiguousarray(psi, dtype=np.float64)
+    hh = np.ascontiguousarray(h, dtype=np.float64)
+    return lib().orc_free_energy(_p(p), _p(_dims(p.shape), _i), _p(hh), eps)
+
+
+def findpeaks(v, ths):
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    cap = v.size
+    out = np.empty(max(cap, 1), dtype=np.int64)
+    n = lib().orc_findpeaks(_p(v), _p(_dims(v.shape), _i), ths, _p(out, _i64), cap)
+    return out[:n].copy()
+
+
+def bond_order_cutoff(xyz, box, rlim):
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+    P = xyz.shape[0]
+    nb = np.zeros(max(P, 1), dtype=np.int32)
+    out = np.zeros((max(P, 1), 9), dtype=np.float64)
+    lib().orc_bond_order_cutoff(_p(xyz), P, box, rlim, _p(nb, _i), _p(out))
+    return nb[:P], out[:P]
+
+
+def bond_order_knn(ijk, n, h, k=12):
+    ijk = np.ascontiguousarray(ijk, dtype=np.int32)
+    P = ijk.shape[0]
+    nbr = np.zeros((max(P, 1), k), dtype=np.int32)
+    out = np.zeros((max(P, 1), 4), dtype=np.float64)
+    lib().orc_bond_order_knn(_p(ijk, _i), P, n, h, k, _p(nbr, _i), _p(out))
+    return nbr[:P], out[:P]
+
+
+def ylm(l, m, ct, phi):
+    re = ctypes.c_double()
+    im = ctypes.c_double()
+    lib().orc_ylm(l, m, ct, phi, ctypes.byref(re), ctypes.byref(im))
+    return complex(re.value, im.value)
+
+
+def wigner3j(j1, j2, j3, m1, m2, m3):
+    return lib().orc_wigner3j(j1, j2, j3, m1, m2, m3)
+
+
+def ref_bond_order_binary():
+    """Path of the reference's own bond-order analysis built from its sources
+    (bond-order-analysis/standalone.cpp + bond-order.cpp), or None."""
+    p = os.path.join(_HERE, "_ref", "bond_order_standalone")
+    return p if os.path.exists(p) else None
