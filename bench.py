@@ -1,0 +1,152 @@
+#!/usr/bin/env python3
+"""Benchmark of the fused PFC time step on one MI355X.
+
+Workload: 3D PFC 512^3 FP32 (eps=0.3, psi0=-0.25 + U[-0.05,0.05] from Philox seed
+11, dx=pi/4, dt=0.5, noise off), the configuration the cell-updates/s metric
+is quoted on.  Prints one JSON line:
+  value          cell-updates/s = 512^3 * steps / t (HIP events around the step loop)
+  roofline       the dominant pass (x_update) against the 8 TB/s HBM peak
+  fp64           the same kernels built with F = double (the all-FP64 form)
+  cpu_baseline   the FP64 CPU oracle on the host cores, a bounded sample of steps
+"""
+import argparse
+import importlib.util
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+HBM_PEAK_GBS = 8000.0
+METRIC = "cell-updates/s, 3D 512^3 FP32 vs FP64-GPU at stated L2 error; % HBM roofline"
+
+
+def load_pkg():
+    spec = importlib.util.spec_from_file_location("mcstar_libs_amd", os.path.join(REPO, "mcstar-libs_amd", "__init__.py"))
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["mcstar_libs_amd"] = mod
+    spec.loader.exec_module(mod)
+    mod.lib()
+    return mod
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def make_stepper(mcs, n, dtype, cfg):
+    shp = (n, n, n)
+    buf = mcs.DeviceBuffer(n ** 3 * np.dtype(dtype).itemsize)
+    prec = 64 if dtype == np.float64 else 32
+    mcs.check(mcs.lib().mcs_init_field(buf.ptr, mcs._ext(shp), cfg["psi0"], cfg["amp"], cfg["seed"], prec))
+    st = mcs.Stepper(shp, [cfg["dx"]] * 3, cfg["eps"], cfg["dt"], temp=0.0, seed=0, dtype=dtype)
+    st.set_psi_device(buf)
+    return st, buf
+
+
+def timed_steps(mcs, st, k):
+    mcs.sync()
+    with mcs.Timer() as t:
+        st.step_async(k)
+    return t.ms
+
+
+def cpu_baseline(n, cfg, steps):
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle
+    oracle.lib()
+    psi = oracle.init_field((n, n, n), cfg["psi0"], cfg["amp"], cfg["seed"])
+    h = [cfg["dx"]] * 3
+    t0 = time.perf_counter()
+    oracle.pfc_run(psi, h, cfg["eps"], cfg["dt"], 0.0, 0, 0, 0, 0)  # transforms in and out only
+    t1 = time.perf_counter()
+    oracle.pfc_run(psi, h, cfg["eps"], cfg["dt"], 0.0, 0, 0, steps, 0)
+    t2 = time.perf_counter()
+    per_step = max((t2 - t1) - (t1 - t0), 1e-9) / steps
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": n ** 3 / per_step, "unit": "cell-updates/s", "cores": cores, "kind": "port",
+            "sample": f"{steps} steps of the FP64 oracle (own mixed-radix FFT, OpenMP) at {n}^3, "
+                      f"minus a 0-step run for the transforms in and out; {per_step:.2f} s/step"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (the PFC step runs on a single "
+              f"MI355X; see DESIGN.md for how it would shard)", file=sys.stderr)
+        sys.exit(2)
+
+    mcs = load_pkg()
+    n = a.n
+    cfg = dict(eps=0.3, psi0=-0.25, amp=0.05, seed=11, dx=np.pi / 4, dt=0.5)
+    N = n ** 3
+    log(f"[bench] fp32 stepper {n}^3: warmup {a.warmup}")
+    st, _ = make_stepper(mcs, n, np.float32, cfg)
+    st.step(a.warmup)
+    log(f"[bench] fp32 timed {a.steps} steps")
+    ms = timed_steps(mcs, st, a.steps)
+    ms_step = ms / a.steps
+    value = N * a.steps / (ms / 1e3)
+    passes = st.pass_times(10)
+    log(f"[bench] fp32 {ms_step:.4f} ms/step, passes {passes}")
+    steps32 = a.warmup + a.steps + 10
+
+    # algorithmic bytes per pass: every pass streams the packed state (N/2 complex = 4N bytes in FP32)
+    # once in and once out; x_update also reads and writes the state spectrum
+    sweep = 4 * N
+    pass_bytes = {"x_update": 4 * sweep, "y_inverse": 2 * sweep, "z_nonlinear": 2 * sweep, "y_forward": 2 * sweep}
+    dom = max(passes, key=passes.get)
+    achieved = pass_bytes[dom] / (passes[dom] / 1e3) / 1e9
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "bytes_per_launch": pass_bytes[dom], "launch_ms": round(passes[dom], 5)}
+    # step-level model: 11 Cb + 2 Rb bytes per 3D step (half-spectrum FFT passes with fused pointwise work)
+    cb = (n // 2 + 1) * n * n * 8
+    rb = N * 4
+    b_step = 11 * cb + 2 * rb
+    step_frac = b_step / (ms_step / 1e3) / (HBM_PEAK_GBS * 1e9)
+    out = {"metric": METRIC, "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": a.steps,
+           "warmup": a.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32", "data": "synthetic (Philox-seeded initial field)",
+           "config": {"workload": f"3D PFC {n}^3 FP32, eps=0.3, psi0=-0.25+U[-0.05,0.05] seed 11, dx=pi/4, "
+                                  f"dt=0.5, noise off", "n": n, "precision": "fp32", "parallelism": "1 GPU"},
+           "steps_per_s": 1e3 / ms_step, "hbm_roofline_frac_step": round(step_frac, 4),
+           "model_bytes_per_step": b_step, "pass_ms": {k: round(v, 5) for k, v in passes.items()},
+           "roofline": roofline}
+
+    if not a.no_fp64:
+        log("[bench] fp64 form")
+        st64, _ = make_stepper(mcs, n, np.float64, cfg)
+        st64.step(a.warmup)
+        k64 = min(a.steps, 50)
+        ms64 = timed_steps(mcs, st64, k64)
+        st64.step(steps32 - a.warmup - k64)
+        mcs.sync()
+        v64 = N * k64 / (ms64 / 1e3)
+        p32 = st.get_psi().astype(np.float64)
+        p64 = st64.get_psi()
+        rel = float(np.linalg.norm(p32 - p64) / np.linalg.norm(p64))
+        out["fp64"] = {"value": v64, "ms_per_step": ms64 / k64, "speedup_fp32_over_fp64": value / v64,
+                       "rel_l2_fp32_vs_fp64_gpu": rel, "max_abs_fp32_vs_fp64_gpu": float(np.abs(p32 - p64).max()),
+                       "steps_compared": steps32}
+        del st64, p32, p64
+    if not a.no_cpu_baseline:
+        log("[bench] cpu baseline")
+        out["cpu_baseline"] = cpu_baseline(n, cfg, a.cpu_steps)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -147,6 +147,15 @@ int mcs_stepper_step_async(mcs_stepper* st, int nsteps);
 /* mean density from the FP64 k=0 bookkeeping */
 int mcs_stepper_mass(mcs_stepper* st, double* mean);
 int mcs_stepper_destroy(mcs_stepper* st);
+/* Instrumentation (no counterpart in the reference): run nsteps with a HIP
+ * event between consecutive passes on the library stream and return the mean
+ * milliseconds of each pass (x_update, y_inverse, z_nonlinear, y_forward in
+ * 3D; x_update, y_nonlinear in 2D) in ms[0..*npass). */
+int mcs_stepper_pass_times(mcs_stepper* st, int nsteps, double* ms, int cap, int* npass);
+
+/* HIP-event timer on the library stream (instrumentation, no reference counterpart) */
+int mcs_timer_start(void);
+int mcs_timer_stop(double* ms);
 
 /* ---------------------------------------------------------------- peaks
  * call_kernel_findpeaks / _2d (cuda/kernels/findpeaks.hpp:7-63,

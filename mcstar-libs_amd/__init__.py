@@ -1,0 +1,428 @@
+"""Python binding of libmcstar_hip.so (the MI355X PFC time step) over its C ABI.
+
+The product is the C ABI in include/mcstar_hip.h and the C++ host headers
+that keep the reference's API (include/cuda/*.hpp).  This module binds the
+same C ABI with ctypes for tests and benchmarks.  It has no fallback: if the
+shared library is missing, loading fails loudly.
+
+Arrays on the device are DeviceBuffer objects (mcs_malloc'd); numpy arrays
+move in and out with upload()/download().  Shapes follow numpy order
+(nz, ny, nx); the C ABI takes ext = {nx, ny, nz}.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libmcstar_hip.so")
+HEADER = os.path.join(os.path.dirname(_HERE), "include", "mcstar_hip.h")
+
+OK, ERR_INVALID, ERR_HIP, ERR_UNSUPPORTED, ERR_NOMEM, ERR_CAPACITY = 0, 1, 2, 3, 4, 5
+H2D, D2H, D2D, H2H = 1, 2, 3, 4
+R2C, C2R = 0, 1
+SPLIT_NORTHSTAR, SPLIT_REFERENCE = 0, 1
+
+_vp = ctypes.c_void_p
+_ip = ctypes.POINTER(ctypes.c_int)
+_dp = ctypes.POINTER(ctypes.c_double)
+_i64p = ctypes.POINTER(ctypes.c_int64)
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+
+
+class McsError(RuntimeError):
+    def __init__(self, code, where, detail):
+        super().__init__(f"{where}: {detail} (code {code})")
+        self.code = code
+
+
+class PfcParams(ctypes.Structure):
+    _fields_ = [("ext", ctypes.c_int * 3), ("h", ctypes.c_double * 3), ("eps", ctypes.c_double),
+                ("dt", ctypes.c_double), ("temp", ctypes.c_double), ("seed", ctypes.c_uint64),
+                ("split", ctypes.c_int), ("prec", ctypes.c_int)]
+
+
+_SIGS = {
+    "mcs_error_string": ([ctypes.c_int], ctypes.c_char_p),
+    "mcs_last_error": ([], ctypes.c_char_p),
+    "mcs_init": ([ctypes.c_int], ctypes.c_int),
+    "mcs_sync": ([], ctypes.c_int),
+    "mcs_malloc": ([ctypes.POINTER(_vp), ctypes.c_size_t], ctypes.c_int),
+    "mcs_free": ([_vp], ctypes.c_int),
+    "mcs_host_alloc": ([ctypes.POINTER(_vp), ctypes.c_size_t], ctypes.c_int),
+    "mcs_host_free": ([_vp], ctypes.c_int),
+    "mcs_memcpy": ([_vp, _vp, ctypes.c_size_t, ctypes.c_int], ctypes.c_int),
+    "mcs_memset": ([_vp, ctypes.c_int, ctypes.c_size_t], ctypes.c_int),
+    "mcs_fft_plan_create": ([ctypes.POINTER(_vp), _ip, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+    "mcs_fft_exec": ([_vp, _vp, _vp], ctypes.c_int),
+    "mcs_fft_plan_destroy": ([_vp], ctypes.c_int),
+    "mcs_rng_create": ([ctypes.POINTER(_vp), ctypes.c_uint64], ctypes.c_int),
+    "mcs_rng_destroy": ([_vp], ctypes.c_int),
+    "mcs_rng_set_draw": ([_vp, ctypes.c_uint32], ctypes.c_int),
+    "mcs_rng_get_draw": ([_vp, _u32p], ctypes.c_int),
+    "mcs_rng_fill_normal": ([_vp, _vp, ctypes.c_size_t, ctypes.c_double, ctypes.c_int], ctypes.c_int),
+    "mcs_rng_fill_kspace": ([_vp, _vp, _ip, ctypes.c_int], ctypes.c_int),
+    "mcs_philox_blocks": ([ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_size_t,
+                           _vp], ctypes.c_int),
+    "mcs_psi3_split": ([_vp, _vp, ctypes.c_double, ctypes.c_int, _ip, ctypes.c_int], ctypes.c_int),
+    "mcs_norm": ([_vp, _ip, ctypes.c_int], ctypes.c_int),
+    "mcs_update_eom": ([_vp, _vp, _vp, _ip, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp,
+                        ctypes.c_int], ctypes.c_int),
+    "mcs_update_pfc": ([_vp, _vp, _ip, _dp, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_uint64,
+                        ctypes.c_uint32, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+    "mcs_init_field": ([_vp, _ip, ctypes.c_double, ctypes.c_double, ctypes.c_uint64, ctypes.c_int], ctypes.c_int),
+    "mcs_field_stats": ([_vp, _ip, _dp, ctypes.c_double, ctypes.c_int, _dp, _dp], ctypes.c_int),
+    "mcs_stepper_create": ([ctypes.POINTER(_vp), ctypes.POINTER(PfcParams)], ctypes.c_int),
+    "mcs_stepper_set_psi": ([_vp, _vp], ctypes.c_int),
+    "mcs_stepper_get_psi": ([_vp, _vp], ctypes.c_int),
+    "mcs_stepper_step": ([_vp, ctypes.c_int], ctypes.c_int),
+    "mcs_stepper_step_async": ([_vp, ctypes.c_int], ctypes.c_int),
+    "mcs_stepper_get_step": ([_vp, _i64p], ctypes.c_int),
+    "mcs_stepper_set_step": ([_vp, ctypes.c_int64], ctypes.c_int),
+    "mcs_stepper_mass": ([_vp, _dp], ctypes.c_int),
+    "mcs_stepper_destroy": ([_vp], ctypes.c_int),
+    "mcs_stepper_pass_times": ([_vp, ctypes.c_int, _dp, ctypes.c_int, _ip], ctypes.c_int),
+    "mcs_timer_start": ([], ctypes.c_int),
+    "mcs_timer_stop": ([_dp], ctypes.c_int),
+    "mcs_findpeaks": ([_vp, _ip, ctypes.c_double, _i64p, _dp, ctypes.c_int64, _i64p, ctypes.c_int], ctypes.c_int),
+    "mcs_bond_order_cutoff": ([_dp, ctypes.c_int, ctypes.c_double, ctypes.c_double, _ip, _dp], ctypes.c_int),
+    "mcs_bond_order_knn": ([_ip, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, _ip, _dp, _i64p,
+                            _i64p, ctypes.c_int], ctypes.c_int),
+}
+
+_LIB = None
+
+
+def lib():
+    """Load libmcstar_hip.so (raises if it was not built)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `make lib` (no fallback path exists)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (args, res) in _SIGS.items():
+            f = getattr(L, name, None)
+            if f is None:
+                continue
+            f.argtypes = args
+            f.restype = res
+        _LIB = L
+    return _LIB
+
+
+def declared_symbols():
+    """Function names declared in include/mcstar_hip.h."""
+    import re
+    txt = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\*?\s+\**(mcs_\w+)\s*\(", txt, re.M)))
+
+
+def check(code, where=""):
+    if code != OK:
+        L = lib()
+        raise McsError(code, where or L.mcs_error_string(code).decode(), L.mcs_last_error().decode())
+    return code
+
+
+def _ext(shape_zyx):
+    s = list(shape_zyx)[::-1] + [1] * (3 - len(shape_zyx))
+    return (ctypes.c_int * 3)(*s)
+
+
+def _prec(dtype):
+    return 64 if np.dtype(dtype) in (np.float64, np.complex128) else 32
+
+
+def half_shape(shape_zyx):
+    s = list(shape_zyx)
+    s[-1] = s[-1] // 2 + 1
+    return tuple(s)
+
+
+def buffer_bytes(shape_zyx, dtype):
+    """Bytes of a GPUArray for a real shape: (nx/2+1)*ny*nz complex values."""
+    return int(np.prod(half_shape(shape_zyx))) * 2 * np.dtype(dtype).itemsize
+
+
+class DeviceBuffer:
+    """Owning device allocation (mcs_malloc / mcs_free)."""
+
+    def __init__(self, nbytes):
+        self.nbytes = int(nbytes)
+        p = _vp()
+        check(lib().mcs_malloc(ctypes.byref(p), max(self.nbytes, 1)), "mcs_malloc")
+        self.ptr = p
+
+    def upload(self, arr):
+        a = np.ascontiguousarray(arr)
+        assert a.nbytes <= self.nbytes, "upload larger than buffer"
+        check(lib().mcs_memcpy(self.ptr, a.ctypes.data_as(_vp), a.nbytes, H2D), "mcs_memcpy H2D")
+        return self
+
+    def download(self, shape, dtype):
+        out = np.empty(shape, dtype=dtype)
+        assert out.nbytes <= self.nbytes, "download larger than buffer"
+        check(lib().mcs_memcpy(out.ctypes.data_as(_vp), self.ptr, out.nbytes, D2H), "mcs_memcpy D2H")
+        return out
+
+    def zero(self):
+        check(lib().mcs_memset(self.ptr, 0, self.nbytes), "mcs_memset")
+        return self
+
+    def free(self):
+        if self.ptr is not None and self.ptr.value:
+            lib().mcs_free(self.ptr)
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def to_device(arr, nbytes=None):
+    a = np.ascontiguousarray(arr)
+    return DeviceBuffer(nbytes or a.nbytes).upload(a)
+
+
+class FFTPlan:
+    """FFT::CufftPlan<F, R2C|C2R> over the C ABI."""
+
+    def __init__(self, shape_zyx, dtype, direction):
+        self.shape = tuple(shape_zyx)
+        self.prec = _prec(dtype)
+        p = _vp()
+        check(lib().mcs_fft_plan_create(ctypes.byref(p), _ext(shape_zyx), self.prec, direction),
+              "mcs_fft_plan_create")
+        self.ptr = p
+
+    def execute(self, din, dout):
+        check(lib().mcs_fft_exec(self.ptr, din.ptr, dout.ptr), "mcs_fft_exec")
+
+    def __del__(self):
+        try:
+            lib().mcs_fft_plan_destroy(self.ptr)
+        except Exception:
+            pass
+
+
+def rfftn(x):
+    """Unnormalised R2C of a real numpy array on the GPU (reference layout)."""
+    x = np.ascontiguousarray(x)
+    ct = np.complex128 if x.dtype == np.float64 else np.complex64
+    nb = buffer_bytes(x.shape, x.dtype)
+    din = DeviceBuffer(nb).upload(x)
+    dout = DeviceBuffer(nb)
+    FFTPlan(x.shape, x.dtype, R2C).execute(din, dout)
+    return dout.download(half_shape(x.shape), ct)
+
+
+def irfftn(k, shape_zyx):
+    """Unnormalised C2R of a half spectrum on the GPU."""
+    k = np.ascontiguousarray(k)
+    rt = np.float64 if k.dtype == np.complex128 else np.float32
+    nb = buffer_bytes(shape_zyx, rt)
+    din = DeviceBuffer(nb).upload(k)
+    dout = DeviceBuffer(nb)
+    FFTPlan(shape_zyx, rt, C2R).execute(din, dout)
+    return dout.download(tuple(shape_zyx), rt)
+
+
+def update_eom(kpsi, knl, knoise, shape_zyx, namp, dt, eps, lens):
+    prec = _prec(kpsi.dtype)
+    nb = buffer_bytes(shape_zyx, np.float64 if prec == 64 else np.float32)
+    dk, dn = to_device(kpsi, nb), to_device(knl, nb)
+    dz = to_device(knoise, nb) if knoise is not None else None
+    ln = (ctypes.c_double * 3)(*lens)
+    check(lib().mcs_update_eom(dk.ptr, dn.ptr, dz.ptr if dz else None, _ext(shape_zyx), namp, dt, eps, ln, prec),
+          "mcs_update_eom")
+    return dk.download(kpsi.shape, kpsi.dtype)
+
+
+def update_pfc(kpsi, knl, shape_zyx, lens, dt, eps, temp, seed, draw, split):
+    prec = _prec(kpsi.dtype)
+    nb = buffer_bytes(shape_zyx, np.float64 if prec == 64 else np.float32)
+    dk, dn = to_device(kpsi, nb), to_device(knl, nb)
+    ln = (ctypes.c_double * 3)(*lens)
+    check(lib().mcs_update_pfc(dk.ptr, dn.ptr, _ext(shape_zyx), ln, dt, eps, temp, seed, draw, split, prec),
+          "mcs_update_pfc")
+    return dk.download(kpsi.shape, kpsi.dtype)
+
+
+def psi3_split(a, mu=0.0, norm=False, pot=None):
+    prec = _prec(a.dtype)
+    d = to_device(a)
+    dp = to_device(pot) if pot is not None else None
+    check(lib().mcs_psi3_split(d.ptr, dp.ptr if dp else None, mu, int(norm), _ext(a.shape), prec), "mcs_psi3_split")
+    return d.download(a.shape, a.dtype)
+
+
+def norm(a):
+    d = to_device(a)
+    check(lib().mcs_norm(d.ptr, _ext(a.shape), _prec(a.dtype)), "mcs_norm")
+    return d.download(a.shape, a.dtype)
+
+
+def kspace_noise(shape_zyx, seed, draw, dtype=np.float32):
+    prec = _prec(dtype)
+    ct = np.complex128 if prec == 64 else np.complex64
+    d = DeviceBuffer(buffer_bytes(shape_zyx, dtype))
+    r = _vp()
+    check(lib().mcs_rng_create(ctypes.byref(r), seed), "mcs_rng_create")
+    try:
+        check(lib().mcs_rng_set_draw(r, draw), "mcs_rng_set_draw")
+        check(lib().mcs_rng_fill_kspace(r, d.ptr, _ext(shape_zyx), prec), "mcs_rng_fill_kspace")
+    finally:
+        lib().mcs_rng_destroy(r)
+    return d.download(half_shape(shape_zyx), ct)
+
+
+def fill_normal(n, sigma, seed, draw, dtype=np.float32):
+    d = DeviceBuffer(n * np.dtype(dtype).itemsize)
+    r = _vp()
+    check(lib().mcs_rng_create(ctypes.byref(r), seed), "mcs_rng_create")
+    try:
+        check(lib().mcs_rng_set_draw(r, draw), "mcs_rng_set_draw")
+        check(lib().mcs_rng_fill_normal(r, d.ptr, n, sigma, _prec(dtype)), "mcs_rng_fill_normal")
+    finally:
+        lib().mcs_rng_destroy(r)
+    return d.download((n,), dtype)
+
+
+def philox_blocks(seed, c0, c2, c3, n):
+    d = DeviceBuffer(16 * n)
+    check(lib().mcs_philox_blocks(seed, c0, c2, c3, n, d.ptr), "mcs_philox_blocks")
+    return d.download((4 * n,), np.uint32)
+
+
+def init_field(shape_zyx, psi0, amp, seed, dtype=np.float32):
+    d = DeviceBuffer(int(np.prod(shape_zyx)) * np.dtype(dtype).itemsize)
+    check(lib().mcs_init_field(d.ptr, _ext(shape_zyx), psi0, amp, seed, _prec(dtype)), "mcs_init_field")
+    return d.download(tuple(shape_zyx), dtype)
+
+
+def field_stats(psi, h, eps):
+    d = to_device(psi)
+    hh = (ctypes.c_double * 3)(*h)
+    mean, fen = ctypes.c_double(), ctypes.c_double()
+    check(lib().mcs_field_stats(d.ptr, _ext(psi.shape), hh, eps, _prec(psi.dtype), ctypes.byref(mean),
+                                ctypes.byref(fen)), "mcs_field_stats")
+    return mean.value, fen.value
+
+
+def findpeaks(field, ths, cap=None, with_values=False):
+    field = np.ascontiguousarray(field)
+    d = to_device(field)
+    cap = field.size if cap is None else cap
+    idx = np.empty(max(cap, 1), dtype=np.int64)
+    vals = np.empty(max(cap, 1), dtype=np.float64) if with_values else None
+    cnt = ctypes.c_int64()
+    rc = lib().mcs_findpeaks(d.ptr, _ext(field.shape), ths, idx.ctypes.data_as(_i64p),
+                             vals.ctypes.data_as(_dp) if with_values else None, cap, ctypes.byref(cnt),
+                             _prec(field.dtype))
+    if rc == ERR_CAPACITY:
+        raise McsError(rc, "mcs_findpeaks", f"capacity {cap} < count {cnt.value}")
+    check(rc, "mcs_findpeaks")
+    n = cnt.value
+    return (idx[:n].copy(), vals[:n].copy()) if with_values else idx[:n].copy()
+
+
+def bond_order_cutoff(xyz, box, rlim):
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+    P = xyz.shape[0]
+    nb = np.zeros(max(P, 1), dtype=np.int32)
+    out = np.zeros((max(P, 1), 9), dtype=np.float64)
+    check(lib().mcs_bond_order_cutoff(xyz.ctypes.data_as(_dp), P, box, rlim, nb.ctypes.data_as(_ip),
+                                      out.ctypes.data_as(_dp)), "mcs_bond_order_cutoff")
+    return nb[:P], out[:P]
+
+
+def bond_order_knn(ijk, n, h, k=12, nbins=64):
+    ijk = np.ascontiguousarray(ijk, dtype=np.int32)
+    P = ijk.shape[0]
+    nbr = np.zeros((max(P, 1), k), dtype=np.int32)
+    out = np.zeros((max(P, 1), 4), dtype=np.float64)
+    h4 = np.zeros(nbins, dtype=np.int64)
+    h6 = np.zeros(nbins, dtype=np.int64)
+    check(lib().mcs_bond_order_knn(ijk.ctypes.data_as(_ip), P, n, h, k, nbr.ctypes.data_as(_ip),
+                                   out.ctypes.data_as(_dp), h4.ctypes.data_as(_i64p), h6.ctypes.data_as(_i64p),
+                                   nbins), "mcs_bond_order_knn")
+    return nbr[:P], out[:P], h4, h6
+
+
+class Stepper:
+    """The fused PFC stepper (mcs_stepper_*)."""
+
+    def __init__(self, shape_zyx, h, eps, dt, temp=0.0, seed=0, split=SPLIT_NORTHSTAR, dtype=np.float32):
+        self.shape = tuple(shape_zyx)
+        self.dtype = np.dtype(dtype)
+        p = PfcParams()
+        p.ext = _ext(shape_zyx)
+        p.h = (ctypes.c_double * 3)(*h)
+        p.eps, p.dt, p.temp, p.seed, p.split, p.prec = eps, dt, temp, seed, split, _prec(dtype)
+        self.params = p
+        ptr = _vp()
+        check(lib().mcs_stepper_create(ctypes.byref(ptr), ctypes.byref(p)), "mcs_stepper_create")
+        self.ptr = ptr
+        self.buf = DeviceBuffer(int(np.prod(shape_zyx)) * self.dtype.itemsize)
+
+    def set_psi(self, psi):
+        self.buf.upload(np.ascontiguousarray(psi, dtype=self.dtype))
+        check(lib().mcs_stepper_set_psi(self.ptr, self.buf.ptr), "mcs_stepper_set_psi")
+
+    def set_psi_device(self, dbuf):
+        check(lib().mcs_stepper_set_psi(self.ptr, dbuf.ptr), "mcs_stepper_set_psi")
+
+    def get_psi(self):
+        check(lib().mcs_stepper_get_psi(self.ptr, self.buf.ptr), "mcs_stepper_get_psi")
+        return self.buf.download(self.shape, self.dtype)
+
+    def step(self, n=1):
+        check(lib().mcs_stepper_step(self.ptr, n), "mcs_stepper_step")
+
+    def step_async(self, n=1):
+        check(lib().mcs_stepper_step_async(self.ptr, n), "mcs_stepper_step_async")
+
+    def mass(self):
+        m = ctypes.c_double()
+        check(lib().mcs_stepper_mass(self.ptr, ctypes.byref(m)), "mcs_stepper_mass")
+        return m.value
+
+    def steps_taken(self):
+        s = ctypes.c_int64()
+        check(lib().mcs_stepper_get_step(self.ptr, ctypes.byref(s)), "mcs_stepper_get_step")
+        return s.value
+
+    def pass_times(self, n):
+        """mean milliseconds per pass over n steps (HIP events between passes)"""
+        ms = (ctypes.c_double * 8)()
+        npass = ctypes.c_int()
+        check(lib().mcs_stepper_pass_times(self.ptr, n, ms, 8, ctypes.byref(npass)), "mcs_stepper_pass_times")
+        names = ["x_update", "y_inverse", "z_nonlinear", "y_forward"] if len(self.shape) == 3 else \
+            ["x_update", "y_nonlinear"]
+        return dict(zip(names, [ms[i] for i in range(npass.value)]))
+
+    def __del__(self):
+        try:
+            lib().mcs_stepper_destroy(self.ptr)
+        except Exception:
+            pass
+
+
+def sync():
+    check(lib().mcs_sync(), "mcs_sync")
+
+
+class Timer:
+    """HIP events on the library stream around a region."""
+
+    def __enter__(self):
+        check(lib().mcs_timer_start(), "mcs_timer_start")
+        return self
+
+    def __exit__(self, *exc):
+        ms = ctypes.c_double()
+        check(lib().mcs_timer_stop(ctypes.byref(ms)), "mcs_timer_stop")
+        self.ms = ms.value
+        return False

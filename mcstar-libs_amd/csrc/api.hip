@@ -122,3 +122,26 @@ extern "C" int mcs_memset(void* dst, int value, size_t bytes) {
     MCS_HIP_TRY(hipStreamSynchronize(g_stream));
     return MCS_OK;
 }
+
+static hipEvent_t g_t0 = nullptr, g_t1 = nullptr;
+
+extern "C" int mcs_timer_start(void) {
+    int st = ensure_init(-1);
+    if (st) return st;
+    if (!g_t0) {
+        MCS_HIP_TRY(hipEventCreate(&g_t0));
+        MCS_HIP_TRY(hipEventCreate(&g_t1));
+    }
+    MCS_HIP_TRY(hipEventRecord(g_t0, g_stream));
+    return MCS_OK;
+}
+
+extern "C" int mcs_timer_stop(double* ms) {
+    MCS_CHECK(ms && g_t0, MCS_ERR_INVALID, "timer not started");
+    MCS_HIP_TRY(hipEventRecord(g_t1, g_stream));
+    MCS_HIP_TRY(hipEventSynchronize(g_t1));
+    float f = 0;
+    MCS_HIP_TRY(hipEventElapsedTime(&f, g_t0, g_t1));
+    *ms = f;
+    return MCS_OK;
+}

@@ -351,6 +351,7 @@ int step_geom_build(StepGeom* sg, const int ext[3], int prec) {
     sg->is3d = ext[2] > 1;
     MCS_CHECK(line_plan_reg_P(sg->L) && line_plan_reg_P(sg->ny) && (!sg->is3d || line_plan_reg_P(sg->nz)),
               MCS_ERR_UNSUPPORTED, "stepper axis lengths must be 2^a, 3*2^a or 5*2^a (nx/2, ny, nz >= 4)");
+    MCS_CHECK(sg->L / line_plan_reg_P(sg->L) <= 256, MCS_ERR_UNSUPPORTED, "nx/2 too long for the x pass");
     int st = line_plan_build(&sg->px, sg->L, prec);
     if (!st) st = line_plan_build(&sg->py, sg->ny, prec);
     if (!st && sg->is3d) st = line_plan_build(&sg->pz, sg->nz, prec);
@@ -397,6 +398,42 @@ int step_nonlinear_half(const StepGeom& sg, const StepPhys& ph, void* W, int64_t
     } else {
         st = launch_col_nl(sg.py, W, geom_y(sg), scale, ph.split, d_step, s);
     }
+    return st;
+}
+
+// the same passes as step_run with an event after each; returns per-pass milliseconds summed over nsteps
+int step_run_timed(const StepGeom& sg, const StepPhys& ph, void* W, void* U, int nsteps, hipStream_t s,
+                   double* acc, int* npass) {
+    const int np = sg.is3d ? 4 : 2;
+    *npass = np;
+    hipEvent_t ev[5];
+    for (int i = 0; i <= np; ++i) MCS_HIP_TRY(hipEventCreate(&ev[i]));
+    for (int i = 0; i < np; ++i) acc[i] = 0;
+    const double scale = 2.0 / ((double)sg.nx * sg.ny * sg.nz);
+    int st = MCS_OK;
+    for (int it = 0; it < nsteps && !st; ++it) {
+        (void)hipEventRecord(ev[0], s);
+        st = launch_xupd(sg, ph, W, U, s);
+        (void)hipEventRecord(ev[1], s);
+        if (sg.is3d) {
+            if (!st) st = launch_col(sg.py, W, W, geom_y(sg), +1, s);
+            (void)hipEventRecord(ev[2], s);
+            if (!st) st = launch_col_nl(sg.pz, W, geom_z(sg), scale, ph.split, sg.d_step, s);
+            (void)hipEventRecord(ev[3], s);
+            if (!st) st = launch_col(sg.py, W, W, geom_y(sg), -1, s);
+            (void)hipEventRecord(ev[4], s);
+        } else {
+            if (!st) st = launch_col_nl(sg.py, W, geom_y(sg), scale, ph.split, sg.d_step, s);
+            (void)hipEventRecord(ev[2], s);
+        }
+        if (!st && hipEventSynchronize(ev[np]) != hipSuccess) st = MCS_ERR_HIP;
+        for (int i = 0; i < np && !st; ++i) {
+            float f = 0;
+            if (hipEventElapsedTime(&f, ev[i], ev[i + 1]) != hipSuccess) st = MCS_ERR_HIP;
+            acc[i] += f;
+        }
+    }
+    for (int i = 0; i <= np; ++i) (void)hipEventDestroy(ev[i]);
     return st;
 }
 
@@ -536,6 +573,17 @@ extern "C" int mcs_stepper_step(mcs_stepper* st, int nsteps) {
     int rc = mcs_stepper_step_async(st, nsteps);
     if (rc) return rc;
     MCS_HIP_TRY(hipStreamSynchronize(mcs_stream()));
+    return MCS_OK;
+}
+
+extern "C" int mcs_stepper_pass_times(mcs_stepper* st, int nsteps, double* ms, int cap, int* npass) {
+    MCS_CHECK(st && ms && npass && nsteps > 0 && cap >= 4, MCS_ERR_INVALID, "bad argument");
+    MCS_CHECK(st->loaded, MCS_ERR_INVALID, "stepper has no field (call mcs_stepper_set_psi)");
+    double acc[4];
+    int rc = step_run_timed(st->sg, st->ph, st->W, st->U, nsteps, mcs_stream(), acc, npass);
+    if (rc) return rc;
+    for (int i = 0; i < *npass; ++i) ms[i] = acc[i] / nsteps;
+    st->step += nsteps;
     return MCS_OK;
 }
 

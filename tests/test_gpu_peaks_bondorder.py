@@ -1,0 +1,87 @@
+"""GPU parity of the peak finder and the bond-order analysis against the oracle
+(bit-exact peak lists and neighbour sets) and the reference's own build."""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+@pytest.mark.parametrize("shp", [(7, 9, 11), (64, 64, 64), (12, 10), (256, 256), (2, 2, 2), (1, 4, 4)])
+def test_peaks_match_oracle(mcs, orc, shp, dt):
+    v = np.random.default_rng(sum(shp)).standard_normal(shp).astype(dt)
+    for ths in (-10.0, 0.5):
+        ths_f = float(np.asarray(ths, dt))
+        idx, vals = mcs.findpeaks(v, ths_f, with_values=True)
+        want = orc.findpeaks(v.astype(np.float64), ths_f)
+        assert np.array_equal(idx, want)
+        assert np.array_equal(vals, v.reshape(-1)[want].astype(np.float64))
+
+
+def test_peaks_special_cases(mcs):
+    v = np.zeros((6, 6, 6), np.float32)
+    assert mcs.findpeaks(v, -1).size == 0
+    v[5, 5, 5] = 1.0
+    assert list(mcs.findpeaks(v, 1.0)) == [215]
+    assert mcs.findpeaks(v, 1.0001).size == 0
+    c = np.zeros((8, 8), np.float64)
+    c[::2, ::2] = 1.0
+    assert mcs.findpeaks(c, 0.5).size == 16
+    with pytest.raises(mcs.McsError) as e:
+        mcs.findpeaks(c, 0.5, cap=3)
+    assert e.value.code == mcs.ERR_CAPACITY
+
+
+def test_peaks_on_crystal_field(mcs, orc):
+    # a bcc-like density wave: one peak per lattice site, all on the grid
+    n = 48
+    x = np.arange(n) * 2 * np.pi / 12
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+    f = np.cos(X) * np.cos(Y) + np.cos(Y) * np.cos(Z) + np.cos(Z) * np.cos(X)
+    idx = mcs.findpeaks(f.astype(np.float32), 0.0)
+    assert np.array_equal(idx, orc.findpeaks(f.astype(np.float32).astype(np.float64), 0.0))
+    assert idx.size > 0
+
+
+def test_bond_order_cutoff_matches_oracle_and_reference(mcs, orc):
+    g = np.load(os.path.join(GOLD, "bond_order_ref.npz"))
+    for c in range(int(g["count"])):
+        pts, box, rlim, table = g[f"pts{c}"], float(g[f"box{c}"]), float(g[f"rlim{c}"]), g[f"table{c}"]
+        nb, out = mcs.bond_order_cutoff(pts, box, rlim)
+        onb, oout = orc.bond_order_cutoff(pts, box, rlim)
+        assert np.array_equal(nb, onb)
+        assert np.array_equal(np.isnan(out), np.isnan(oout))
+        assert np.nanmax(np.abs(out - oout)) < 1e-12
+        m = nb > 0
+        mine = np.column_stack([pts[m], nb[m], out[m][:, 8], out[m][:, :8]])
+        assert np.array_equal(mine[:, :5], table[:, :5])
+        assert np.nanmax(np.abs(mine[:, 5:] - table[:, 5:])) <= 6e-7
+
+
+def test_bond_order_knn_bit_exact_neighbours(mcs, orc):
+    rng = np.random.default_rng(8)
+    n = 40
+    cells = rng.choice(n ** 3, 2000, replace=False)
+    ijk = np.column_stack([cells % n, (cells // n) % n, cells // (n * n)]).astype(np.int32)
+    nbr, out, h4, h6 = mcs.bond_order_knn(ijk, n, np.pi / 4, 12)
+    onbr, oout = orc.bond_order_knn(ijk, n, np.pi / 4, 12)
+    assert np.array_equal(nbr, onbr)
+    assert np.abs(out - oout).max() < 1e-12
+    want4 = np.bincount(np.minimum((oout[:, 0] * 64).astype(int), 63), minlength=64)
+    want6 = np.bincount(np.minimum((oout[:, 1] * 64).astype(int), 63), minlength=64)
+    assert np.array_equal(h4, want4) and np.array_equal(h6, want6)
+
+
+def test_bond_order_fcc_lattice(mcs):
+    pts = []
+    for i in range(4):
+        for j in range(4):
+            for k in range(4):
+                for (x, y, z) in [(0, 0, 0), (1, 1, 0), (1, 0, 1), (0, 1, 1)]:
+                    pts.append(((2 * i + x) % 8, (2 * j + y) % 8, (2 * k + z) % 8))
+    nbr, out, h4, h6 = mcs.bond_order_knn(np.array(pts), 8, 1.0, 12)
+    assert np.allclose(out[:, 0], 0.19094, atol=5e-5) and np.allclose(out[:, 1], 0.57452, atol=5e-5)
+    assert h4[int(0.19094 * 64)] == len(pts) and h6[int(0.57452 * 64)] == len(pts)

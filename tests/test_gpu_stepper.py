@@ -1,0 +1,105 @@
+"""GPU parity of the fused PFC stepper against the FP64 oracle on seeded inputs."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+H = np.pi / 4
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+def run_pair(mcs, orc, shp, eps, dt, psi0, amp, seed, steps, split=0, temp=0.0, noise_seed=0, dtype=np.float64):
+    psi = orc.init_field(shp, psi0, amp, seed)
+    h3 = [H, H, H]
+    st = mcs.Stepper(shp, h3, eps, dt, temp=temp, seed=noise_seed, split=split, dtype=dtype)
+    st.set_psi(psi)
+    st.step(steps)
+    got = st.get_psi()
+    want = orc.pfc_run(psi, h3, eps, dt, temp, noise_seed, 0, steps, split)
+    return got, want, psi, st
+
+
+@pytest.mark.parametrize("split", [0, 1])
+@pytest.mark.parametrize("shp", [(32, 32, 32), (24, 20, 48), (64, 64), (48, 80)])
+def test_fp64_matches_oracle(mcs, orc, shp, split):
+    got, want, psi, st = run_pair(mcs, orc, shp, 0.3, 0.5, -0.25, 0.05, 11, 20, split=split)
+    assert rel_l2(got, want) < 1e-12
+    assert abs(got.mean() - psi.mean()) < 1e-14
+
+
+@pytest.mark.parametrize("shp", [(32, 32, 32), (64, 96)])
+def test_fp64_with_noise_matches_oracle(mcs, orc, shp):
+    got, want, psi, st = run_pair(mcs, orc, shp, 0.3, 0.25, -0.2, 0.1, 7, 15, temp=1e-3, noise_seed=7)
+    assert rel_l2(got, want) < 1e-11
+    assert abs(got.mean() - psi.mean()) < 1e-14
+
+
+@pytest.mark.parametrize("shp", [(32, 32, 32), (128, 128)])
+def test_fp32_matches_oracle(mcs, orc, shp):
+    got, want, psi, st = run_pair(mcs, orc, shp, 0.3, 0.5, -0.25, 0.05, 11, 50, dtype=np.float32)
+    assert rel_l2(got, want) < 1e-5
+    assert np.abs(got - want).max() < 1e-5
+    assert abs(st.mass() - psi.astype(np.float32).astype(np.float64).mean()) < 1e-15
+    assert abs(got.astype(np.float64).mean() - st.mass()) < 2e-8
+
+
+def test_fp32_with_noise_matches_oracle(mcs, orc):
+    got, want, psi, st = run_pair(mcs, orc, (32, 32, 32), 0.3, 0.25, -0.2, 0.1, 7, 20, temp=1e-3, noise_seed=7,
+                                  dtype=np.float32)
+    assert rel_l2(got, want) < 5e-5
+
+
+def test_config1_2d_256_fp64_1000_steps(mcs, orc):
+    """2D 256^2 FP64, eps=0.25, dt=0.5, psi0=-0.25 +- U[-0.05,0.05] seed 1, noise off, 1000 steps."""
+    shp = (256, 256)
+    got, want, psi, st = run_pair(mcs, orc, shp, 0.25, 0.5, -0.25, 0.05, 1, 1000)
+    assert rel_l2(got, want) < 1e-10
+    assert abs(got.mean() - want.mean()) < 1e-14
+    pg = mcs.findpeaks(got, -1e9)
+    po = orc.findpeaks(want, -1e9)
+    assert np.array_equal(pg, po)
+    assert np.array_equal(pg, orc.findpeaks(got, -1e9))
+
+
+def test_resume_matches_uninterrupted_run(mcs, orc):
+    shp = (32, 32, 32)
+    psi = orc.init_field(shp, -0.2, 0.1, 3)
+    kw = dict(temp=1e-3, seed=5, dtype=np.float64)
+    a = mcs.Stepper(shp, [H] * 3, 0.3, 0.25, **kw)
+    a.set_psi(psi)
+    a.step(10)
+    b = mcs.Stepper(shp, [H] * 3, 0.3, 0.25, **kw)
+    b.set_psi(psi)
+    b.step(4)
+    mid = b.get_psi()
+    c = mcs.Stepper(shp, [H] * 3, 0.3, 0.25, **kw)
+    mcs.check(mcs.lib().mcs_stepper_set_step(c.ptr, 4))
+    c.set_psi(mid)
+    c.step(6)
+    assert rel_l2(c.get_psi(), a.get_psi()) < 1e-12
+    assert a.steps_taken() == 10 and c.steps_taken() == 10
+
+
+def test_single_mode_linear_decay(mcs):
+    # small-amplitude single mode, linear regime: psi_k(n) = psi_k(0) / (1 + dt k^2 L(k))^n
+    shp = (32, 64)
+    y, x = np.meshgrid(np.arange(32), np.arange(64), indexing="ij")
+    kx = 2 * np.pi * 3 / (64 * H)
+    psi = 1e-6 * np.cos(kx * x * H)
+    st = mcs.Stepper(shp, [H, H, H], 0.3, 0.5, dtype=np.float64)
+    st.set_psi(psi)
+    st.step(10)
+    k2 = kx ** 2
+    g = 1.0 / (1 + 0.5 * k2 * (-0.3 + (1 - k2) ** 2))
+    assert np.abs(st.get_psi() - psi * g ** 10).max() < 1e-17 + 1e-9 * np.abs(psi).max()
+
+
+def test_unsupported_sizes(mcs):
+    for shp in [(7, 9), (30, 30, 30), (32, 14)]:
+        with pytest.raises(mcs.McsError) as e:
+            mcs.Stepper(shp, [H] * 3, 0.3, 0.5)
+        assert e.value.code == mcs.ERR_UNSUPPORTED
