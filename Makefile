@@ -13,7 +13,19 @@ LIB := $(PKG)/lib/libmcstar_hip.so
 HIPFLAGS := -O3 -std=c++17 -fPIC --offload-arch=$(ARCH) -Xclang -target-feature -Xclang -packed-fp32-ops \
             -Wall -Wno-unused-function -Wno-unused-result -I include
 
-all: lib oracle
+all: lib oracle driver
+
+# C++ drop-in driver of the kept host API (tests/cpp), one build per precision
+DRV := tests/cpp/bin/kept_api_driver_f32 tests/cpp/bin/kept_api_driver_f64
+driver: $(DRV)
+
+tests/cpp/bin/kept_api_driver_f32: tests/cpp/kept_api_driver.cpp $(LIB) $(wildcard include/*/*.hpp include/*/*/*.hpp)
+	@mkdir -p $(dir $@)
+	$(HIPCC) -O2 -std=c++17 -DFLOAT=float -I include $< -o $@ -L$(PKG)/lib -lmcstar_hip -Wl,-rpath,'$$ORIGIN/../../../$(PKG)/lib'
+
+tests/cpp/bin/kept_api_driver_f64: tests/cpp/kept_api_driver.cpp $(LIB) $(wildcard include/*/*.hpp include/*/*/*.hpp)
+	@mkdir -p $(dir $@)
+	$(HIPCC) -O2 -std=c++17 -DFLOAT=double -I include $< -o $@ -L$(PKG)/lib -lmcstar_hip -Wl,-rpath,'$$ORIGIN/../../../$(PKG)/lib'
 
 lib: $(LIB)
 
@@ -30,6 +42,6 @@ oracle:
 	$(MAKE) -s -C oracle liboracle.so
 
 clean:
-	rm -rf $(OBJDIR) $(PKG)/lib
+	rm -rf $(OBJDIR) $(PKG)/lib tests/cpp/bin
 
-.PHONY: all lib oracle clean
+.PHONY: all lib oracle driver clean

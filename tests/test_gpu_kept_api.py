@@ -1,0 +1,103 @@
+"""Drop-in check: the C++ driver written against the kept host API (tests/cpp/kept_api_driver.cpp)
+reproduces the reference drivers' flows on the GPU, checked against numpy and the oracle."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def drv(prec):
+    p = os.path.join(REPO, "tests", "cpp", "bin", f"kept_api_driver_{prec}")
+    if not os.path.exists(p):
+        pytest.fail(f"{p} not built (make driver)")
+    return p
+
+
+def run(prec, *args):
+    r = subprocess.run([drv(prec)] + [str(a) for a in args], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    return r
+
+
+@pytest.mark.parametrize("prec,dt,tol", [("f32", np.float32, 1e-5), ("f64", np.float64, 2e-14)])
+@pytest.mark.parametrize("shp", [(36, 75, 60), (625, 10, 9), (27, 36, 27), (1, 64, 48)])
+def test_fft_flow(tmp_path, prec, dt, tol, shp):
+    x = np.random.default_rng(3).standard_normal(shp).astype(dt)
+    fin, fout = str(tmp_path / "in.npy"), str(tmp_path / "out.npy")
+    np.save(fin, x)
+    dims = [s for s in shp if True]
+    n0, n1, n2 = shp[2], shp[1], shp[0]  # my_unit_test.py:49 passes the axes reversed
+    if shp[0] == 1:
+        np.save(fin, x[0])
+    run(prec, "fft", n0, n1, n2, fin, fout, 0)
+    y = np.load(fout)
+    want = np.fft.rfftn(x.astype(np.float64)).reshape(y.shape)
+    assert np.abs(want - y).max() / np.sqrt(x.size) < tol
+    run(prec, "fft", n0, n1, n2, fin, fout, 1)
+    z = np.load(fout).reshape(x.shape)
+    assert np.abs(x - z / x.size).max() < tol
+    del dims
+
+
+def test_noise_flow(tmp_path, orc):
+    out = str(tmp_path / "xi.npy")
+    run("f64", "noise", 10, 8, 6, 99, out)
+    assert np.abs(np.load(out) - orc.kspace_noise((6, 8, 10), 99, 0)).max() < 1e-13
+
+
+@pytest.mark.parametrize("prec,tol", [("f64", 1e-11), ("f32", 2e-5)])
+@pytest.mark.parametrize("shp", [(16, 16, 16), (32, 48)])
+def test_pfc_simulator_loop(tmp_path, orc, prec, tol, shp):
+    """c2r -> psi3_split(norm) -> r2c -> fill_kspace -> update_eom with the reference split and noise."""
+    psi = orc.init_field(shp, -0.2, 0.1, 7)
+    dt = np.float64 if prec == "f64" else np.float32
+    fin, fout = str(tmp_path / "in.npy"), str(tmp_path / "out.npy")
+    np.save(fin, psi.astype(dt))
+    steps, eps, dtt, namp, seed, h = 10, 0.3, 0.25, 0.05, 5, np.pi / 4
+    ext = list(shp[::-1]) + [1] * (3 - len(shp))
+    run(prec, "pfc", *ext, fin, fout, steps, eps, dtt, namp, seed, h)
+    got = np.load(fout)
+    # the same loop on the oracle (eom.hpp:60-72 noise scale dt*namp*sqrtf(N)*|k|)
+    N = psi.size
+    lens = [e * h for e in ext]
+    lens = [l if e > 1 else 1.0 for l, e in zip(lens, ext)]
+    kp = orc.r2c(psi.astype(dt).astype(np.float64))
+    for s in range(steps):
+        r = orc.c2r(kp, shp) / N
+        kn = orc.r2c(r ** 3 - r)
+        xi = orc.kspace_noise(shp, seed, s)
+        kp = orc.update(kp, kn, xi, shp, lens, dtt, eps, dtt * namp * float(np.sqrt(np.float32(N))), 1)
+    want = orc.c2r(kp, shp) / N
+    assert np.linalg.norm(got - want) / np.linalg.norm(want) < tol
+
+
+def test_peaks_and_bond_order_flow(tmp_path, orc):
+    n = 48
+    x = np.arange(n) * 2 * np.pi / 12
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+    rng = np.random.default_rng(2)
+    f = np.cos(X) * np.cos(Y) + np.cos(Y) * np.cos(Z) + np.cos(Z) * np.cos(X) + 0.01 * rng.standard_normal(X.shape)
+    fin = str(tmp_path / "f.npy")
+    np.save(fin, f)
+    pk, bo = str(tmp_path / "peaks.txt"), str(tmp_path / "bo.txt")
+    h = 0.5
+    r = run("f64", "peaks", n, n, n, fin, pk, bo, h, 0.5, 5.0)
+    idx = orc.findpeaks(f, 0.5)
+    assert f"found {idx.size} maxima" in r.stderr
+    rows = np.loadtxt(pk)
+    i0, i1, i2 = idx % n, (idx // n) % n, idx // (n * n)
+    assert np.allclose(rows[:, 0], i0 * h, atol=1e-6) and np.allclose(rows[:, 2], i2 * h, atol=1e-6)
+    assert np.allclose(rows[:, 3], f.reshape(-1)[idx], atol=1e-6)
+    table = np.loadtxt(bo, comments="#")
+    keep = f.reshape(-1)[idx] > 0.5
+    xyz = np.column_stack([i0, i1, i2])[keep] * h
+    nb, out = orc.bond_order_cutoff(xyz, n * h, 5.0)
+    m = nb > 0
+    want = np.column_stack([xyz[m], nb[m], out[m][:, 8], out[m][:, :8]])
+    assert table.shape == want.shape
+    assert np.array_equal(table[:, 3:5], want[:, 3:5])
+    assert np.abs(table - want).max() <= 6e-7

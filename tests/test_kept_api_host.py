@@ -1,0 +1,44 @@
+"""The kept C++ host headers compile and their host-only helpers behave as the
+reference's (cuda/math.hpp:125-149 size policy, technicalities.hpp shape_tr, npy I/O).  CPU only."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HIPCC = "/opt/rocm/bin/hipcc"
+
+
+@pytest.fixture(scope="module")
+def helper(tmp_path_factory):
+    if not os.path.exists(HIPCC):
+        pytest.skip("hipcc not available")
+    out = str(tmp_path_factory.mktemp("hh") / "host_helpers")
+    subprocess.run([HIPCC, "-O1", "-std=c++17", "-I", os.path.join(REPO, "include"),
+                    os.path.join(REPO, "tests", "cpp", "host_helpers.cpp"), "-o", out], check=True)
+    return out
+
+
+def test_size_policy_and_layout(helper, tmp_path):
+    npy = str(tmp_path / "a.npy")
+    txt = subprocess.run([helper, npy], check=True, capture_output=True, text=True).stdout.splitlines()
+    table = {int(l.split()[0]): (int(l.split()[1]), int(l.split()[2])) for l in txt[:13]}
+    # factors235 / roundup235: prime factors above 5 are replaced by the next 2,3,5-smooth number
+    assert table[1] == (1, 1) and table[5] == (1, 5) and table[7] == (0, 8) and table[11] == (0, 12)
+    assert table[13] == (0, 15) and table[14] == (0, 16) and table[26] == (0, 30) and table[49] == (0, 64)
+    assert table[97] == (0, 100) and table[120] == (1, 120) and table[625] == (1, 625) and table[1000] == (1, 1000)
+    assert table[1023] == (0, 3 * 12 * 32)  # 1023 = 3 * 11 * 31 -> 3 * 12 * 32
+    assert txt[13] == "shape_tr 6 5 4 | 5 4 1 | 4 1 1"
+    # (1,0,0),(0,1,1): frame matrix {1, 0; 0, sqrt(2)} -> largest singular value sqrt(2)
+    assert abs(float(txt[14].split()[1]) - np.sqrt(2)) < 1e-12
+    assert txt[15].startswith("closest 1 0 ")
+    a = np.load(npy)
+    assert a.shape == (2, 3, 4) and a.dtype == np.float64 and a[0, 0, 0] == -3 and a[-1, -1, -1] == 8.5
+    assert txt[16] == "npy 24 2 3 4 -3 8.5" and txt[17] == "npy dtype mismatch rejected"
+
+
+def test_driver_builds_against_kept_headers():
+    subprocess.run(["make", "-s", "driver"], cwd=REPO, check=True)
+    for p in ("f32", "f64"):
+        assert os.path.exists(os.path.join(REPO, "tests", "cpp", "bin", f"kept_api_driver_{p}"))
