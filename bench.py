@@ -71,6 +71,26 @@ def cpu_baseline(n, cfg, steps):
                       f"minus a 0-step run for the transforms in and out; {per_step:.2f} s/step"}
 
 
+def ensemble_bench(mcs, kind, steps):
+    """replica-steps/s of the batched, graph-replayed ensemble against looping single-replica launches"""
+    reps = mcs.ensemble_config(kind)
+    e = mcs.Ensemble(reps, dtype=np.float32)
+    e.init()
+    e.step(8, mcs.ENS_GRAPH)
+    cells = sum(int(np.prod(r["shape"])) for r in reps)
+    out = {"workload": ("64 replicas of {128,192,320,384,640,768,1024}^2, T=5e-4, dt=0.5" if kind == "2d" else
+                        "32 replicas of {64,96,128,160,192}^3, T=1e-3, dt=0.25") + ", FP32, seeded sizes",
+           "replicas": len(reps), "cells": cells, "steps": steps}
+    for name, mode in (("loop", mcs.ENS_LOOP), ("batched", mcs.ENS_BATCHED), ("graph", mcs.ENS_GRAPH)):
+        mcs.sync()
+        with mcs.Timer() as t:
+            e.step_async(steps, mode)
+        out[name] = {"ms_per_step": t.ms / steps, "replica_steps_per_s": len(reps) * steps / (t.ms / 1e3),
+                     "cell_updates_per_s": cells * steps / (t.ms / 1e3)}
+    out["speedup_graph_over_loop"] = out["graph"]["replica_steps_per_s"] / out["loop"]["replica_steps_per_s"]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -80,6 +100,7 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-ensemble", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (the PFC step runs on a single "
@@ -140,6 +161,10 @@ def main():
                        "rel_l2_fp32_vs_fp64_gpu": rel, "max_abs_fp32_vs_fp64_gpu": float(np.abs(p32 - p64).max()),
                        "steps_compared": steps32}
         del st64, p32, p64
+    if not a.no_ensemble:
+        log("[bench] ensembles")
+        out["ensemble_2d"] = ensemble_bench(mcs, "2d", 40)
+        out["ensemble_3d"] = ensemble_bench(mcs, "3d", 16)
     if not a.no_cpu_baseline:
         log("[bench] cpu baseline")
         out["cpu_baseline"] = cpu_baseline(n, cfg, a.cpu_steps)

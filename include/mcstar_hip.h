@@ -157,6 +157,39 @@ int mcs_stepper_pass_times(mcs_stepper* st, int nsteps, double* ms, int cap, int
 int mcs_timer_start(void);
 int mcs_timer_stop(double* ms);
 
+/* ---------------------------------------------------------------- ensembles
+ * Independent replicas (own grid size, eps, dt, temperature, initial field and
+ * noise seed) advanced together.  Replicas of equal size share plans and are
+ * stepped by one launch per pass (persistent workgroups over a (replica, work
+ * unit) table); all replicas take the same number of steps and share the
+ * Philox draw number.  Same size rules as the stepper.  No counterpart in the
+ * reference (one replica per process there). */
+typedef struct {
+    int ext[3];
+    double h[3];
+    double eps, dt, temp;
+    double psi0, amp;     /* initial field psi0 + amp (2u - 1) */
+    uint64_t init_seed;   /* Philox key of the initial field */
+    uint64_t noise_seed;  /* Philox key of the noise */
+} mcs_replica;
+typedef struct mcs_ensemble mcs_ensemble;
+#define MCS_ENS_LOOP 0    /* replica by replica, single-grid launches (the looping baseline) */
+#define MCS_ENS_BATCHED 1 /* one launch per pass and size group */
+#define MCS_ENS_GRAPH 2   /* batched, replayed from a captured HIP graph */
+int mcs_ensemble_create(mcs_ensemble** e, const mcs_replica* reps, int nrep, int split, int prec);
+/* seeded initial fields for every replica */
+int mcs_ensemble_init(mcs_ensemble* e);
+int mcs_ensemble_set_psi(mcs_ensemble* e, int r, const void* psi_dev);
+int mcs_ensemble_get_psi(mcs_ensemble* e, int r, void* psi_dev);
+int mcs_ensemble_step(mcs_ensemble* e, int nsteps, int mode);
+/* enqueue without waiting (mcs_sync() to finish) */
+int mcs_ensemble_step_async(mcs_ensemble* e, int nsteps, int mode);
+/* per replica (3D replicas): peaks >= ths, k-nearest-neighbour q4/q6 histograms over [0,1),
+ * all on the device; counts[nrep], hist_q4/hist_q6[nrep * nbins] on the host */
+int mcs_ensemble_analyze(mcs_ensemble* e, double ths, int k, int nbins, int64_t* counts, int64_t* hist_q4,
+                         int64_t* hist_q6);
+int mcs_ensemble_destroy(mcs_ensemble* e);
+
 /* ---------------------------------------------------------------- peaks
  * call_kernel_findpeaks / _2d (cuda/kernels/findpeaks.hpp:7-63,
  * findpeaks_2d.hpp:7-56): strict periodic local maxima (26 neighbours in 3D,

@@ -42,6 +42,14 @@ class PfcParams(ctypes.Structure):
                 ("split", ctypes.c_int), ("prec", ctypes.c_int)]
 
 
+class ReplicaParams(ctypes.Structure):
+    _fields_ = [("ext", ctypes.c_int * 3), ("h", ctypes.c_double * 3), ("eps", ctypes.c_double),
+                ("dt", ctypes.c_double), ("temp", ctypes.c_double), ("psi0", ctypes.c_double),
+                ("amp", ctypes.c_double), ("init_seed", ctypes.c_uint64), ("noise_seed", ctypes.c_uint64)]
+
+
+ENS_LOOP, ENS_BATCHED, ENS_GRAPH = 0, 1, 2
+
 _SIGS = {
     "mcs_error_string": ([ctypes.c_int], ctypes.c_char_p),
     "mcs_last_error": ([], ctypes.c_char_p),
@@ -84,6 +92,15 @@ _SIGS = {
     "mcs_stepper_pass_times": ([_vp, ctypes.c_int, _dp, ctypes.c_int, _ip], ctypes.c_int),
     "mcs_timer_start": ([], ctypes.c_int),
     "mcs_timer_stop": ([_dp], ctypes.c_int),
+    "mcs_ensemble_create": ([ctypes.POINTER(_vp), ctypes.POINTER(ReplicaParams), ctypes.c_int, ctypes.c_int,
+                             ctypes.c_int], ctypes.c_int),
+    "mcs_ensemble_init": ([_vp], ctypes.c_int),
+    "mcs_ensemble_set_psi": ([_vp, ctypes.c_int, _vp], ctypes.c_int),
+    "mcs_ensemble_get_psi": ([_vp, ctypes.c_int, _vp], ctypes.c_int),
+    "mcs_ensemble_step": ([_vp, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+    "mcs_ensemble_step_async": ([_vp, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+    "mcs_ensemble_analyze": ([_vp, ctypes.c_double, ctypes.c_int, ctypes.c_int, _i64p, _i64p, _i64p], ctypes.c_int),
+    "mcs_ensemble_destroy": ([_vp], ctypes.c_int),
     "mcs_findpeaks": ([_vp, _ip, ctypes.c_double, _i64p, _dp, ctypes.c_int64, _i64p, ctypes.c_int], ctypes.c_int),
     "mcs_bond_order_cutoff": ([_dp, ctypes.c_int, ctypes.c_double, ctypes.c_double, _ip, _dp], ctypes.c_int),
     "mcs_bond_order_knn": ([_ip, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, _ip, _dp, _i64p,
@@ -426,3 +443,78 @@ class Timer:
         check(lib().mcs_timer_stop(ctypes.byref(ms)), "mcs_timer_stop")
         self.ms = ms.value
         return False
+
+
+class Ensemble:
+    """Batched replicas (mcs_ensemble_*).  Each replica is a dict with keys
+    shape (numpy order), h, eps, dt, temp, psi0, amp, init_seed, noise_seed."""
+
+    def __init__(self, replicas, split=SPLIT_NORTHSTAR, dtype=np.float32):
+        self.dtype = np.dtype(dtype)
+        self.shapes = [tuple(r["shape"]) for r in replicas]
+        arr = (ReplicaParams * len(replicas))()
+        for a, r in zip(arr, replicas):
+            a.ext = _ext(r["shape"])
+            h = r.get("h", [np.pi / 4] * 3)
+            a.h = (ctypes.c_double * 3)(*(list(h) + [h[-1]] * (3 - len(h))))
+            a.eps, a.dt, a.temp = r["eps"], r["dt"], r.get("temp", 0.0)
+            a.psi0, a.amp = r["psi0"], r["amp"]
+            a.init_seed, a.noise_seed = r["init_seed"], r.get("noise_seed", 0)
+        p = _vp()
+        check(lib().mcs_ensemble_create(ctypes.byref(p), arr, len(replicas), split, _prec(dtype)),
+              "mcs_ensemble_create")
+        self.ptr = p
+        self.buf = DeviceBuffer(max(int(np.prod(s)) for s in self.shapes) * self.dtype.itemsize)
+
+    def init(self):
+        check(lib().mcs_ensemble_init(self.ptr), "mcs_ensemble_init")
+
+    def set_psi(self, r, psi):
+        self.buf.upload(np.ascontiguousarray(psi, dtype=self.dtype))
+        check(lib().mcs_ensemble_set_psi(self.ptr, r, self.buf.ptr), "mcs_ensemble_set_psi")
+
+    def get_psi(self, r):
+        check(lib().mcs_ensemble_get_psi(self.ptr, r, self.buf.ptr), "mcs_ensemble_get_psi")
+        return self.buf.download(self.shapes[r], self.dtype)
+
+    def step(self, n, mode=ENS_GRAPH):
+        check(lib().mcs_ensemble_step(self.ptr, n, mode), "mcs_ensemble_step")
+
+    def step_async(self, n, mode=ENS_GRAPH):
+        check(lib().mcs_ensemble_step_async(self.ptr, n, mode), "mcs_ensemble_step_async")
+
+    def analyze(self, ths, k=12, nbins=64):
+        nr = len(self.shapes)
+        counts = np.zeros(nr, np.int64)
+        h4 = np.zeros((nr, nbins), np.int64)
+        h6 = np.zeros((nr, nbins), np.int64)
+        check(lib().mcs_ensemble_analyze(self.ptr, ths, k, nbins, counts.ctypes.data_as(_i64p),
+                                         h4.ctypes.data_as(_i64p), h6.ctypes.data_as(_i64p)), "mcs_ensemble_analyze")
+        return counts, h4, h6
+
+    def __del__(self):
+        try:
+            lib().mcs_ensemble_destroy(self.ptr)
+        except Exception:
+            pass
+
+
+def ensemble_config(kind, nrep=None, seed=2024):
+    """Replica lists of the two ensemble workloads, drawn from a seeded generator:
+    '2d': sizes {128,192,320,384,640,768,1024}^2, eps U[0.1,0.4], psi0 U[-0.35,-0.15], T=5e-4, dt=0.5
+    '3d': sizes {64,96,128,160,192}^3, eps U[0.2,0.35], psi0 U[-0.3,-0.18], T=1e-3, dt=0.25"""
+    rng = np.random.default_rng(seed)
+    out = []
+    if kind == "2d":
+        sizes, n = [128, 192, 320, 384, 640, 768, 1024], nrep or 64
+        for i in range(n):
+            s = int(rng.choice(sizes))
+            out.append(dict(shape=(s, s), eps=float(rng.uniform(0.1, 0.4)), psi0=float(rng.uniform(-0.35, -0.15)),
+                            amp=0.05, temp=5e-4, dt=0.5, init_seed=1000 + i, noise_seed=5000 + i))
+    else:
+        sizes, n = [64, 96, 128, 160, 192], nrep or 32
+        for i in range(n):
+            s = int(rng.choice(sizes))
+            out.append(dict(shape=(s, s, s), eps=float(rng.uniform(0.2, 0.35)), psi0=float(rng.uniform(-0.3, -0.18)),
+                            amp=0.05, temp=1e-3, dt=0.25, init_seed=2000 + i, noise_seed=6000 + i))
+    return out

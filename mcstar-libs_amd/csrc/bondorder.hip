@@ -15,6 +15,7 @@
 // contraction off so neighbour sets match the CPU bit for bit).
 #include "mcs_common.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -233,6 +234,60 @@ __global__ void __launch_bounds__(128) k_bo_knn(const int* ijk, int P, int n, do
     }
 }
 
+// device-resident peaks (linear cell indices, count on the device) -> ijk
+__global__ void __launch_bounds__(256) k_idx_to_ijk(const int64_t* idx, const int64_t* d_count, int64_t cap, int nx,
+                                                    int ny, int* ijk) {
+    const int64_t P = *d_count < cap ? *d_count : cap;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < P; i += (int64_t)gridDim.x * 256) {
+        const int64_t c = idx[i];
+        ijk[3 * i] = (int)(c % nx);
+        ijk[3 * i + 1] = (int)((c / nx) % ny);
+        ijk[3 * i + 2] = (int)(c / ((int64_t)nx * ny));
+    }
+}
+
+// k nearest neighbours on a periodic nx*ny*nz grid (spacing h) and q4/q6 histograms only
+__global__ void __launch_bounds__(128) k_bo_knn_hist(const int* ijk, const int64_t* d_count, int64_t cap, int nx,
+                                                     int ny, int nz, double h, int k, Wig4 W,
+                                                     unsigned long long* h4, unsigned long long* h6, int nbins) {
+    const int P = (int)(*d_count < cap ? *d_count : cap);
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P) return;
+    long bd[KMAX];
+    int bj[KMAX];
+    int cnt = 0;
+    const int xi = ijk[3 * i], yi = ijk[3 * i + 1], zi = ijk[3 * i + 2];
+    for (int j = 0; j < P; ++j) {
+        if (j == i) continue;
+        const long dx = min_image(ijk[3 * j] - xi, nx), dy = min_image(ijk[3 * j + 1] - yi, ny),
+                   dz = min_image(ijk[3 * j + 2] - zi, nz);
+        const long d2 = dx * dx + dy * dy + dz * dz;
+        if (cnt == k && !(d2 < bd[k - 1] || (d2 == bd[k - 1] && j < bj[k - 1]))) continue;
+        int pos = cnt < k ? cnt : k - 1;
+        while (pos > 0 && (bd[pos - 1] > d2 || (bd[pos - 1] == d2 && bj[pos - 1] > j))) {
+            bd[pos] = bd[pos - 1]; bj[pos] = bj[pos - 1]; --pos;
+        }
+        bd[pos] = d2; bj[pos] = j;
+        if (cnt < k) ++cnt;
+    }
+    if (cnt == 0) return;
+    double acc[78];
+    for (int t = 0; t < 78; ++t) acc[t] = 0;
+    for (int t = 0; t < cnt; ++t) {
+        const int j = bj[t];
+        const double dx = h * min_image(ijk[3 * j] - xi, nx), dy = h * min_image(ijk[3 * j + 1] - yi, ny),
+                     dz = h * min_image(ijk[3 * j + 2] - zi, nz);
+        const double r = sqrt(dx * dx + dy * dy + dz * dz);
+        add_bond(acc, dz / r, atan2(dy, dx), false);
+    }
+    for (int t = 0; t < 78; ++t) acc[t] /= (double)cnt;
+    int b4 = (int)(ql_dev(acc, 4) * nbins), b6 = (int)(ql_dev(acc + 18, 6) * nbins);
+    b4 = b4 < 0 ? 0 : (b4 >= nbins ? nbins - 1 : b4);
+    b6 = b6 < 0 ? 0 : (b6 >= nbins ? nbins - 1 : b6);
+    atomicAdd(&h4[b4], 1ull);
+    atomicAdd(&h6[b6], 1ull);
+}
+
 // Wigner 3j (4 4 4; m1 m2 m3) by the Racah formula, on the host
 static double hfact(int k) { double r = 1; for (int i = 2; i <= k; ++i) r *= i; return r; }
 static double w3j444(int m1, int m2, int m3) {
@@ -256,6 +311,19 @@ static Wig4 wig4() {
     for (int m1 = -4; m1 <= 4; ++m1)
         for (int m2 = -4; m2 <= 4; ++m2) W.w[(m1 + 4) * 9 + (m2 + 4)] = w3j444(m1, m2, -m1 - m2);
     return W;
+}
+
+int knn_hist_launch(const int64_t* d_idx, const int64_t* d_count, int64_t cap, const int ext[3], double h, int k,
+                    int* d_ijk, unsigned long long* d_h4, unsigned long long* d_h6, int nbins, hipStream_t s) {
+    MCS_CHECK(k >= 1 && k <= KMAX, MCS_ERR_INVALID, "k must be in [1, 32]");
+    const unsigned g1 = (unsigned)std::min<int64_t>(4096, (cap + 255) / 256);
+    hipLaunchKernelGGL(k_idx_to_ijk, dim3(g1 ? g1 : 1), dim3(256), 0, s, d_idx, d_count, cap, ext[0], ext[1], d_ijk);
+    const Wig4 W = wig4();
+    const unsigned g2 = (unsigned)((cap + 127) / 128);
+    hipLaunchKernelGGL(k_bo_knn_hist, dim3(g2 ? g2 : 1), dim3(128), 0, s, d_ijk, d_count, cap, ext[0], ext[1], ext[2],
+                       h, k, W, d_h4, d_h6, nbins);
+    MCS_HIP_TRY(hipGetLastError());
+    return MCS_OK;
 }
 
 extern "C" int mcs_bond_order_cutoff(const double* xyz, int npts, double box, double rlim, int* nb, double* out9) {

@@ -1,0 +1,464 @@
+// Batched ensembles of independent PFC replicas.
+//
+// Replicas are grouped by grid size; a group shares the FFT plans, twiddles
+// and row-pair table, and each replica has a descriptor (state pointers and
+// its own eps, dt, noise amplitude, seed and FP64 mass).  One step of the
+// ensemble is one launch per pass and group: persistent workgroups walk the
+// (replica, work unit) table with the same pass bodies as the single-grid
+// stepper.  All x passes run before the first column pass, and the shared
+// Philox draw counter is advanced by the last nonlinear pass of the step.
+// MCS_ENS_LOOP issues the same passes replica by replica (the looping
+// baseline); MCS_ENS_GRAPH replays a captured graph of several steps.
+#include "peaks.hpp"
+#include "stepper.hpp"
+
+#include <algorithm>
+#include <map>
+#include <vector>
+
+template <typename F> struct EnsRep {
+    C2<F>* U;
+    C2<F>* W;
+    XArgs<F> xa;
+};
+
+// ------------------------------------------------------------------ kernels
+template <typename F, int P, int SPLIT>
+__global__ void __launch_bounds__(256) k_ens_xupd(const EnsRep<F>* __restrict__ reps, int nr, int units,
+                                                  const int2* __restrict__ pairs, int npairs, int G, FftPlanDev pl,
+                                                  const C2<F>* __restrict__ tw, const C2<F>* __restrict__ wph,
+                                                  const int64_t* __restrict__ d_step) {
+    extern __shared__ unsigned char smem_raw[];
+    C2<F>* lds = reinterpret_cast<C2<F>*>(smem_raw);
+    const uint32_t draw = (uint32_t)*d_step;
+    const long total = (long)nr * units;
+    for (long it = blockIdx.x; it < total; it += gridDim.x) {
+        const int r = (int)(it / units), u = (int)(it % units);
+        const EnsRep<F>& R = reps[r];
+        xupd_body<F, P, SPLIT>(R.W, R.U, pairs, npairs, G, R.xa, pl, tw, wph, draw, lds, u);
+        __syncthreads();
+    }
+}
+
+template <typename F, int P, int DIR>
+__global__ void __launch_bounds__(1024) k_ens_col(const EnsRep<F>* __restrict__ reps, int nr, ColGeom g, int G,
+                                                  FftPlanDev pl, const C2<F>* __restrict__ tw) {
+    extern __shared__ unsigned char smem_raw[];
+    C2<F>* lds = reinterpret_cast<C2<F>*>(smem_raw);
+    const int tilesx = (g.nxc + G - 1) / G;
+    const long units = (long)tilesx * g.nouter;
+    const long total = (long)nr * units;
+    for (long it = blockIdx.x; it < total; it += gridDim.x) {
+        const int r = (int)(it / units);
+        const long u = it % units;
+        C2<F>* W = reps[r].W;
+        col_body<F, P, DIR>(W, W, g, G, pl, tw, lds, (int)(u % tilesx), (int)(u / tilesx));
+        __syncthreads();
+    }
+}
+
+template <typename F, int P, int SPLIT>
+__global__ void __launch_bounds__(1024) k_ens_col_nl(const EnsRep<F>* __restrict__ reps, int nr, ColGeom g, int G,
+                                                     FftPlanDev pl, const C2<F>* __restrict__ tw, F scale,
+                                                     int64_t* d_step) {
+    extern __shared__ unsigned char smem_raw[];
+    C2<F>* lds = reinterpret_cast<C2<F>*>(smem_raw);
+    const int tilesx = (g.nxc + G - 1) / G;
+    const long units = (long)tilesx * g.nouter;
+    const long total = (long)nr * units;
+    for (long it = blockIdx.x; it < total; it += gridDim.x) {
+        const int r = (int)(it / units);
+        const long u = it % units;
+        col_nl_body<F, P, SPLIT>(reps[r].W, g, G, pl, tw, scale, lds, (int)(u % tilesx), (int)(u / tilesx));
+        __syncthreads();
+    }
+    if (d_step && blockIdx.x == 0 && threadIdx.x == 0) *d_step += 1;
+}
+
+// ------------------------------------------------------------------ host side
+struct EnsGroup {
+    StepGeom sg;
+    std::vector<int> members;
+    void* d_reps = nullptr;  // EnsRep<F>[members]
+};
+
+struct mcs_ensemble {
+    std::vector<mcs_replica> reps;
+    int split = 0, prec = 32;
+    std::vector<void*> U, W;
+    std::vector<double> mass;
+    std::vector<EnsGroup*> groups;
+    std::vector<int> group_of, slot_of;
+    int64_t* d_step = nullptr;
+    int64_t step = 0;
+    bool loaded = false;
+    hipGraphExec_t gexec = nullptr;
+    hipGraph_t graph = nullptr;
+    int gsteps = 0;
+    int cus = 0;
+};
+
+static size_t elem_size(int prec) { return prec == 64 ? 16 : 8; }
+
+static StepPhys phys_of(const mcs_replica& r, int split, double mass) {
+    StepPhys ph;
+    ph.h[0] = r.h[0]; ph.h[1] = r.h[1]; ph.h[2] = r.h[2];
+    ph.eps = r.eps; ph.dt = r.dt; ph.temp = r.temp; ph.seed = r.noise_seed; ph.split = split; ph.mass = mass;
+    return ph;
+}
+
+template <typename F> static int upload_reps_t(mcs_ensemble* e, EnsGroup* g) {
+    std::vector<EnsRep<F>> h(g->members.size());
+    for (size_t i = 0; i < g->members.size(); ++i) {
+        const int r = g->members[i];
+        h[i].U = (C2<F>*)e->U[r];
+        h[i].W = (C2<F>*)e->W[r];
+        h[i].xa = make_xargs<F>(g->sg, phys_of(e->reps[r], e->split, e->mass[r]));
+    }
+    if (!g->d_reps) MCS_HIP_TRY(hipMalloc(&g->d_reps, h.size() * sizeof(EnsRep<F>)));
+    MCS_HIP_TRY(hipMemcpy(g->d_reps, h.data(), h.size() * sizeof(EnsRep<F>), hipMemcpyHostToDevice));
+    return MCS_OK;
+}
+
+static int upload_reps(mcs_ensemble* e) {
+    for (EnsGroup* g : e->groups) {
+        int st = e->prec == 64 ? upload_reps_t<double>(e, g) : upload_reps_t<float>(e, g);
+        if (st) return st;
+    }
+    return MCS_OK;
+}
+
+template <typename Fn> static unsigned persistent_grid(mcs_ensemble* e, Fn fn, int threads, size_t lds, long total) {
+    int per = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, threads, lds) != hipSuccess || per < 1) per = 1;
+    const long cap = (long)per * e->cus;
+    return (unsigned)std::max<long>(1, std::min<long>(total, cap));
+}
+
+// one x pass over members [first, first + nr) of group g
+template <typename F, int P, int SPLIT>
+static int ens_xupd_t(mcs_ensemble* e, EnsGroup* g, int first, int nr, hipStream_t s) {
+    const LinePlan& lp = g->sg.px;
+    const int G = xupd_pairs_per_wg(lp);
+    const int S = lp.L / P;
+    const size_t lds = (size_t)G * 2 * lp.L * sizeof(C2<F>);
+    const int units = (g->sg.npairs + G - 1) / G;
+    auto fn = k_ens_xupd<F, P, SPLIT>;
+    const unsigned grid = persistent_grid(e, fn, G * S, lds, (long)nr * units);
+    hipLaunchKernelGGL(fn, dim3(grid), dim3(G * S), lds, s, (const EnsRep<F>*)g->d_reps + first, nr, units,
+                       g->sg.pairs, g->sg.npairs, G, lp.dev, (const C2<F>*)lp.tw, (const C2<F>*)g->sg.wph,
+                       (const int64_t*)e->d_step);
+    MCS_HIP_TRY(hipGetLastError());
+    return MCS_OK;
+}
+
+template <typename F, int P, int DIR>
+static int ens_col_t(mcs_ensemble* e, EnsGroup* g, const LinePlan& lp, const ColGeom& cg, int first, int nr,
+                     hipStream_t s) {
+    const int G = col_lines_per_wg(lp);
+    const int S = lp.L / P;
+    const size_t lds = (size_t)G * lp.L * sizeof(C2<F>);
+    const long total = (long)nr * ((cg.nxc + G - 1) / G) * cg.nouter;
+    auto fn = k_ens_col<F, P, DIR>;
+    const unsigned grid = persistent_grid(e, fn, G * S, lds, total);
+    hipLaunchKernelGGL(fn, dim3(grid), dim3(G * S), lds, s, (const EnsRep<F>*)g->d_reps + first, nr, cg, G, lp.dev,
+                       (const C2<F>*)lp.tw);
+    MCS_HIP_TRY(hipGetLastError());
+    return MCS_OK;
+}
+
+template <typename F, int P, int SPLIT>
+static int ens_col_nl_t(mcs_ensemble* e, EnsGroup* g, const LinePlan& lp, const ColGeom& cg, int first, int nr,
+                        bool inc, hipStream_t s) {
+    const int G = col_lines_per_wg(lp);
+    const int S = lp.L / P;
+    const size_t lds = (size_t)G * lp.L * sizeof(C2<F>);
+    const long total = (long)nr * ((cg.nxc + G - 1) / G) * cg.nouter;
+    auto fn = k_ens_col_nl<F, P, SPLIT>;
+    const unsigned grid = persistent_grid(e, fn, G * S, lds, total);
+    const double scale = 2.0 / ((double)g->sg.nx * g->sg.ny * g->sg.nz);
+    hipLaunchKernelGGL(fn, dim3(grid), dim3(G * S), lds, s, (const EnsRep<F>*)g->d_reps + first, nr, cg, G, lp.dev,
+                       (const C2<F>*)lp.tw, (F)scale, inc ? e->d_step : nullptr);
+    MCS_HIP_TRY(hipGetLastError());
+    return MCS_OK;
+}
+
+#define MCS_P_DISPATCH(Pv, CALL)                                                            \
+    switch (Pv) {                                                                           \
+        case 4: { constexpr int PP = 4; return CALL; }                                      \
+        case 8: { constexpr int PP = 8; return CALL; }                                      \
+        case 12: { constexpr int PP = 12; return CALL; }                                    \
+        case 20: { constexpr int PP = 20; return CALL; }                                    \
+        default: mcs_set_last_error(__FILE__, __LINE__, "no register plan"); return MCS_ERR_UNSUPPORTED; \
+    }
+
+template <typename F>
+static int ens_xupd(mcs_ensemble* e, EnsGroup* g, int first, int nr, hipStream_t s) {
+    if (e->split == MCS_SPLIT_REFERENCE) { MCS_P_DISPATCH(g->sg.px.P, (ens_xupd_t<F, PP, MCS_SPLIT_REFERENCE>(e, g, first, nr, s))) }
+    MCS_P_DISPATCH(g->sg.px.P, (ens_xupd_t<F, PP, MCS_SPLIT_NORTHSTAR>(e, g, first, nr, s)))
+}
+
+template <typename F>
+static int ens_col(mcs_ensemble* e, EnsGroup* g, const LinePlan& lp, const ColGeom& cg, int dir, int first, int nr,
+                   hipStream_t s) {
+    if (dir < 0) { MCS_P_DISPATCH(lp.P, (ens_col_t<F, PP, -1>(e, g, lp, cg, first, nr, s))) }
+    MCS_P_DISPATCH(lp.P, (ens_col_t<F, PP, +1>(e, g, lp, cg, first, nr, s)))
+}
+
+template <typename F>
+static int ens_col_nl(mcs_ensemble* e, EnsGroup* g, const LinePlan& lp, const ColGeom& cg, int first, int nr,
+                      bool inc, hipStream_t s) {
+    if (e->split == MCS_SPLIT_REFERENCE) { MCS_P_DISPATCH(lp.P, (ens_col_nl_t<F, PP, MCS_SPLIT_REFERENCE>(e, g, lp, cg, first, nr, inc, s))) }
+    MCS_P_DISPATCH(lp.P, (ens_col_nl_t<F, PP, MCS_SPLIT_NORTHSTAR>(e, g, lp, cg, first, nr, inc, s)))
+}
+
+// the passes after the x pass for members [first, first+nr) of g
+template <typename F>
+static int ens_rest(mcs_ensemble* e, EnsGroup* g, int first, int nr, bool inc, hipStream_t s) {
+    const StepGeom& sg = g->sg;
+    int st;
+    if (sg.is3d) {
+        st = ens_col<F>(e, g, sg.py, step_geom_y(sg), +1, first, nr, s);
+        if (!st) st = ens_col_nl<F>(e, g, sg.pz, step_geom_z(sg), first, nr, inc, s);
+        if (!st) st = ens_col<F>(e, g, sg.py, step_geom_y(sg), -1, first, nr, s);
+    } else {
+        st = ens_col_nl<F>(e, g, sg.py, step_geom_y(sg), first, nr, inc, s);
+    }
+    return st;
+}
+
+template <typename F> static int ens_steps_t(mcs_ensemble* e, int nsteps, int mode, hipStream_t s) {
+    const int ng = (int)e->groups.size();
+    for (int it = 0; it < nsteps; ++it) {
+        int st = MCS_OK;
+        if (mode == MCS_ENS_LOOP) {
+            const int nrep = (int)e->reps.size();
+            for (int r = 0; r < nrep && !st; ++r) {
+                EnsGroup* g = e->groups[e->group_of[r]];
+                st = ens_xupd<F>(e, g, e->slot_of[r], 1, s);
+                if (!st) st = ens_rest<F>(e, g, e->slot_of[r], 1, r == nrep - 1, s);
+            }
+        } else {
+            for (int gi = 0; gi < ng && !st; ++gi)
+                st = ens_xupd<F>(e, e->groups[gi], 0, (int)e->groups[gi]->members.size(), s);
+            for (int gi = 0; gi < ng && !st; ++gi)
+                st = ens_rest<F>(e, e->groups[gi], 0, (int)e->groups[gi]->members.size(), gi == ng - 1, s);
+        }
+        if (st) return st;
+    }
+    return MCS_OK;
+}
+
+static int ens_steps(mcs_ensemble* e, int nsteps, int mode, hipStream_t s) {
+    return e->prec == 64 ? ens_steps_t<double>(e, nsteps, mode, s) : ens_steps_t<float>(e, nsteps, mode, s);
+}
+
+static constexpr int GRAPH_STEPS = 8;
+
+static int ens_graph_build(mcs_ensemble* e, hipStream_t s) {
+    if (e->gexec) return MCS_OK;
+    MCS_HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    int st = ens_steps(e, GRAPH_STEPS, MCS_ENS_BATCHED, s);
+    hipGraph_t g = nullptr;
+    const hipError_t ce = hipStreamEndCapture(s, &g);
+    if (st) { if (g) (void)hipGraphDestroy(g); return st; }
+    MCS_HIP_TRY(ce);
+    MCS_HIP_TRY(hipGraphInstantiate(&e->gexec, g, nullptr, nullptr, 0));
+    e->graph = g;
+    e->gsteps = GRAPH_STEPS;
+    return MCS_OK;
+}
+
+// ------------------------------------------------------------------ C ABI
+extern "C" int mcs_ensemble_destroy(mcs_ensemble* e) {
+    if (!e) return MCS_OK;
+    if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
+    if (e->graph) (void)hipGraphDestroy(e->graph);
+    for (EnsGroup* g : e->groups) {
+        step_geom_free(&g->sg);
+        if (g->d_reps) (void)hipFree(g->d_reps);
+        delete g;
+    }
+    for (void* p : e->U) if (p) (void)hipFree(p);
+    for (void* p : e->W) if (p) (void)hipFree(p);
+    if (e->d_step) (void)hipFree(e->d_step);
+    delete e;
+    return MCS_OK;
+}
+
+extern "C" int mcs_ensemble_create(mcs_ensemble** out, const mcs_replica* reps, int nrep, int split, int prec) {
+    MCS_CHECK(out && reps && nrep > 0, MCS_ERR_INVALID, "bad argument");
+    MCS_CHECK(prec == 32 || prec == 64, MCS_ERR_INVALID, "precision must be 32 or 64");
+    MCS_CHECK(split == MCS_SPLIT_NORTHSTAR || split == MCS_SPLIT_REFERENCE, MCS_ERR_INVALID, "bad split");
+    for (int r = 0; r < nrep; ++r) {
+        const mcs_replica& p = reps[r];
+        MCS_CHECK(p.ext[0] >= 1 && p.ext[1] >= 1 && p.ext[2] >= 1, MCS_ERR_INVALID, "extent must be >=1");
+        MCS_CHECK(p.dt > 0 && p.h[0] > 0 && p.h[1] > 0 && p.h[2] > 0, MCS_ERR_INVALID, "dt and h must be positive");
+    }
+    (void)mcs_stream();
+    auto* e = new mcs_ensemble();
+    e->reps.assign(reps, reps + nrep);
+    e->split = split;
+    e->prec = prec;
+    e->U.assign(nrep, nullptr);
+    e->W.assign(nrep, nullptr);
+    e->mass.assign(nrep, 0.0);
+    e->group_of.assign(nrep, -1);
+    e->slot_of.assign(nrep, -1);
+    int rc = MCS_OK;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&e->cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        e->cus = 256;
+    std::map<std::vector<int>, int> by_size;
+    for (int r = 0; r < nrep && !rc; ++r) {
+        const std::vector<int> key = {reps[r].ext[0], reps[r].ext[1], reps[r].ext[2]};
+        auto itg = by_size.find(key);
+        if (itg == by_size.end()) {
+            auto* g = new EnsGroup();
+            e->groups.push_back(g);
+            rc = step_geom_build(&g->sg, reps[r].ext, prec);
+            itg = by_size.emplace(key, (int)e->groups.size() - 1).first;
+        }
+        EnsGroup* g = e->groups[itg->second];
+        e->group_of[r] = itg->second;
+        e->slot_of[r] = (int)g->members.size();
+        g->members.push_back(r);
+        const size_t bytes = (size_t)reps[r].ext[0] * reps[r].ext[1] * reps[r].ext[2] * (prec == 64 ? 8 : 4);
+        if (!rc && (hipMalloc(&e->U[r], bytes) != hipSuccess || hipMalloc(&e->W[r], bytes) != hipSuccess)) {
+            mcs_set_last_error(__FILE__, __LINE__, "ensemble state allocation failed");
+            rc = MCS_ERR_NOMEM;
+        }
+    }
+    if (!rc && hipMalloc((void**)&e->d_step, sizeof(int64_t)) != hipSuccess) rc = MCS_ERR_NOMEM;
+    if (!rc && hipMemset(e->d_step, 0, sizeof(int64_t)) != hipSuccess) rc = MCS_ERR_HIP;
+    if (!rc) rc = upload_reps(e);
+    if (rc) { mcs_ensemble_destroy(e); return rc; }
+    *out = e;
+    return MCS_OK;
+}
+
+static int ens_load(mcs_ensemble* e, int r, const void* psi) {
+    EnsGroup* g = e->groups[e->group_of[r]];
+    return step_import(g->sg, phys_of(e->reps[r], e->split, 0.0), psi, e->W[r], e->U[r], &e->mass[r], mcs_stream());
+}
+
+extern "C" int mcs_ensemble_init(mcs_ensemble* e) {
+    MCS_CHECK(e, MCS_ERR_INVALID, "null argument");
+    hipStream_t s = mcs_stream();
+    size_t maxb = 0;
+    for (const mcs_replica& p : e->reps)
+        maxb = std::max(maxb, (size_t)p.ext[0] * p.ext[1] * p.ext[2] * (e->prec == 64 ? 8 : 4));
+    void* tmp = nullptr;
+    MCS_HIP_TRY(hipMalloc(&tmp, maxb));
+    int rc = MCS_OK;
+    for (size_t r = 0; r < e->reps.size() && !rc; ++r) {
+        const mcs_replica& p = e->reps[r];
+        rc = mcs_init_field(tmp, p.ext, p.psi0, p.amp, p.init_seed, e->prec);
+        if (!rc) rc = ens_load(e, (int)r, tmp);
+    }
+    if (!rc && hipStreamSynchronize(s) != hipSuccess) rc = MCS_ERR_HIP;
+    (void)hipFree(tmp);
+    if (!rc) rc = upload_reps(e);
+    if (!rc) e->loaded = true;
+    return rc;
+}
+
+extern "C" int mcs_ensemble_set_psi(mcs_ensemble* e, int r, const void* psi) {
+    MCS_CHECK(e && psi && r >= 0 && r < (int)e->reps.size(), MCS_ERR_INVALID, "bad argument");
+    int rc = ens_load(e, r, psi);
+    if (!rc) MCS_HIP_TRY(hipStreamSynchronize(mcs_stream()));
+    if (!rc) rc = upload_reps(e);
+    if (!rc) e->loaded = true;
+    return rc;
+}
+
+extern "C" int mcs_ensemble_get_psi(mcs_ensemble* e, int r, void* psi) {
+    MCS_CHECK(e && psi && r >= 0 && r < (int)e->reps.size(), MCS_ERR_INVALID, "bad argument");
+    MCS_CHECK(e->loaded, MCS_ERR_INVALID, "ensemble has no fields (call mcs_ensemble_init)");
+    const mcs_replica& p = e->reps[r];
+    void* tmp = nullptr;
+    MCS_HIP_TRY(hipMalloc(&tmp, (size_t)p.ext[0] * p.ext[1] * p.ext[2] * (e->prec == 64 ? 8 : 4)));
+    int rc = step_export(e->groups[e->group_of[r]]->sg, e->U[r], psi, tmp, mcs_stream());
+    if (!rc && hipStreamSynchronize(mcs_stream()) != hipSuccess) rc = MCS_ERR_HIP;
+    (void)hipFree(tmp);
+    return rc;
+}
+
+extern "C" int mcs_ensemble_step_async(mcs_ensemble* e, int nsteps, int mode) {
+    MCS_CHECK(e && nsteps >= 0, MCS_ERR_INVALID, "bad argument");
+    MCS_CHECK(mode == MCS_ENS_LOOP || mode == MCS_ENS_BATCHED || mode == MCS_ENS_GRAPH, MCS_ERR_INVALID, "bad mode");
+    MCS_CHECK(e->loaded, MCS_ERR_INVALID, "ensemble has no fields (call mcs_ensemble_init)");
+    hipStream_t s = mcs_stream();
+    int done = 0;
+    if (mode == MCS_ENS_GRAPH && nsteps >= GRAPH_STEPS) {
+        int rc = ens_graph_build(e, s);
+        if (rc) return rc;
+        for (; done + e->gsteps <= nsteps; done += e->gsteps) MCS_HIP_TRY(hipGraphLaunch(e->gexec, s));
+    }
+    int rc = ens_steps(e, nsteps - done, mode == MCS_ENS_LOOP ? MCS_ENS_LOOP : MCS_ENS_BATCHED, s);
+    if (rc) return rc;
+    e->step += nsteps;
+    return MCS_OK;
+}
+
+extern "C" int mcs_ensemble_step(mcs_ensemble* e, int nsteps, int mode) {
+    int rc = mcs_ensemble_step_async(e, nsteps, mode);
+    if (rc) return rc;
+    MCS_HIP_TRY(hipStreamSynchronize(mcs_stream()));
+    return MCS_OK;
+}
+
+int knn_hist_launch(const int64_t* d_idx, const int64_t* d_count, int64_t cap, const int ext[3], double h, int k,
+                    int* d_ijk, unsigned long long* d_h4, unsigned long long* d_h6, int nbins, hipStream_t s);
+
+extern "C" int mcs_ensemble_analyze(mcs_ensemble* e, double ths, int k, int nbins, int64_t* counts, int64_t* hist_q4,
+                                    int64_t* hist_q6) {
+    MCS_CHECK(e && counts && hist_q4 && hist_q6 && nbins > 0 && k >= 1 && k <= 32, MCS_ERR_INVALID, "bad argument");
+    MCS_CHECK(e->loaded, MCS_ERR_INVALID, "ensemble has no fields (call mcs_ensemble_init)");
+    hipStream_t s = mcs_stream();
+    const int nrep = (int)e->reps.size();
+    std::vector<void*> bufs;
+    auto dalloc = [&](size_t b) -> void* {
+        void* p = nullptr;
+        if (hipMalloc(&p, b ? b : 1) != hipSuccess) return nullptr;
+        bufs.push_back(p);
+        return p;
+    };
+    int rc = MCS_OK;
+    unsigned long long* dh = (unsigned long long*)dalloc(sizeof(unsigned long long) * 2 * nbins * nrep);
+    int64_t* dcnt = (int64_t*)dalloc(sizeof(int64_t) * nrep);
+    if (!dh || !dcnt) rc = MCS_ERR_NOMEM;
+    if (!rc) MCS_HIP_TRY(hipMemsetAsync(dh, 0, sizeof(unsigned long long) * 2 * nbins * nrep, s));
+    for (int r = 0; r < nrep && !rc; ++r) {
+        const mcs_replica& p = e->reps[r];
+        MCS_CHECK(p.ext[2] > 1, MCS_ERR_UNSUPPORTED, "bond-order analysis needs 3D replicas");
+        const long n = (long)p.ext[0] * p.ext[1] * p.ext[2];
+        const int64_t cap = n / 8 + 8;
+        void* psi = dalloc(n * (e->prec == 64 ? 8 : 4));
+        void* tmp = dalloc(n * (e->prec == 64 ? 8 : 4));
+        int64_t* idx = (int64_t*)dalloc(cap * sizeof(int64_t));
+        int* ijk = (int*)dalloc(3 * cap * sizeof(int));
+        void* ws = dalloc(peaks_workspace_bytes(p.ext));
+        if (!psi || !tmp || !idx || !ijk || !ws) { rc = MCS_ERR_NOMEM; break; }
+        rc = step_export(e->groups[e->group_of[r]]->sg, e->U[r], psi, tmp, s);
+        if (!rc) rc = findpeaks_launch(psi, p.ext, ths, idx, nullptr, cap, dcnt + r, ws, e->prec, s);
+        if (!rc) rc = knn_hist_launch(idx, dcnt + r, cap, p.ext, p.h[0], k, ijk, dh + 2L * nbins * r,
+                                      dh + 2L * nbins * r + nbins, nbins, s);
+    }
+    std::vector<unsigned long long> hh(2 * (size_t)nbins * nrep);
+    if (!rc && hipMemcpyAsync(hh.data(), dh, hh.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost, s) != hipSuccess)
+        rc = MCS_ERR_HIP;
+    if (!rc && hipMemcpyAsync(counts, dcnt, nrep * sizeof(int64_t), hipMemcpyDeviceToHost, s) != hipSuccess)
+        rc = MCS_ERR_HIP;
+    if (!rc && hipStreamSynchronize(s) != hipSuccess) rc = MCS_ERR_HIP;
+    for (void* p : bufs) (void)hipFree(p);
+    if (rc == MCS_ERR_NOMEM) mcs_set_last_error(__FILE__, __LINE__, "analysis allocation failed");
+    if (rc) return rc;
+    for (int r = 0; r < nrep; ++r)
+        for (int b = 0; b < nbins; ++b) {
+            hist_q4[(long)r * nbins + b] = (int64_t)hh[2L * nbins * r + b];
+            hist_q6[(long)r * nbins + b] = (int64_t)hh[2L * nbins * r + nbins + b];
+        }
+    return MCS_OK;
+}

@@ -16,7 +16,7 @@
 // transform, a pointwise op and a forward transform chain without LDS.
 //
 // Generic engine: ping-pong LDS buffers and a work-item loop, for the
-// 2,3,5-smooth lengths the register engine does not take.
+// lengths the register engine does not take (factors 2, 3, 5, 7, 11, 13).
 #pragma once
 
 #include "mcs_common.hpp"
@@ -99,8 +99,30 @@ template <typename F, int DIR> MCS_DEV void dft8(C2<F>* v) {
     v[7] = csub(e[3], o3);
 }
 
+// direct DFT for the odd primes 7, 11, 13 (generic engine only); roots in FP64
+template <typename F, int R, int DIR> MCS_DEV void dftp(C2<F>* v) {
+    C2<F> w[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        double sn, cs;
+        sincospi(2.0 * k / R, &sn, &cs);
+        w[k] = mk<F>((F)cs, (F)(DIR * sn));
+    }
+    C2<F> out[R];
+#pragma unroll
+    for (int s = 0; s < R; ++s) {
+        C2<F> acc = v[0];
+#pragma unroll
+        for (int q = 1; q < R; ++q) acc = cadd(acc, cmul(v[q], w[(q * s) % R]));
+        out[s] = acc;
+    }
+#pragma unroll
+    for (int s = 0; s < R; ++s) v[s] = out[s];
+}
+
 template <typename F, int R, int DIR> MCS_DEV void dft(C2<F>* v) {
-    if constexpr (R == 2) dft2<F, DIR>(v);
+    if constexpr (R == 7 || R == 11 || R == 13) dftp<F, R, DIR>(v);
+    else if constexpr (R == 2) dft2<F, DIR>(v);
     else if constexpr (R == 3) dft3<F, DIR>(v);
     else if constexpr (R == 4) dft4<F, DIR>(v);
     else if constexpr (R == 5) dft5<F, DIR>(v);
@@ -204,6 +226,9 @@ MCS_DEV C2<F>* lds_fft(C2<F>* b0, C2<F>* b1, const FftPlanDev& pl, const C2<F>* 
             case 4: gpass<F, 4, DIR>(in, out, pl.L, Lp, G, Ns, twp, tid, nthr); break;
             case 5: gpass<F, 5, DIR>(in, out, pl.L, Lp, G, Ns, twp, tid, nthr); break;
             case 8: gpass<F, 8, DIR>(in, out, pl.L, Lp, G, Ns, twp, tid, nthr); break;
+            case 7: gpass<F, 7, DIR>(in, out, pl.L, Lp, G, Ns, twp, tid, nthr); break;
+            case 11: gpass<F, 11, DIR>(in, out, pl.L, Lp, G, Ns, twp, tid, nthr); break;
+            case 13: gpass<F, 13, DIR>(in, out, pl.L, Lp, G, Ns, twp, tid, nthr); break;
             default: break;
         }
         __syncthreads();

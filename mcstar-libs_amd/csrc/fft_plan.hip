@@ -4,18 +4,28 @@
 #include <cmath>
 #include <vector>
 
-static bool factor235(int L, int& a, int& b, int& c) {
+// L = 2^a 3^b 5^c * rest; returns rest
+static int factor235(int L, int& a, int& b, int& c) {
     a = b = c = 0;
     int m = L;
     while (m % 2 == 0) { m /= 2; ++a; }
     while (m % 3 == 0) { m /= 3; ++b; }
     while (m % 5 == 0) { m /= 5; ++c; }
+    return m;
+}
+
+// the generic engine also takes the primes 7, 11 and 13
+static bool generic_ok(int L, std::vector<int>* extra) {
+    int a, b, c;
+    int m = factor235(L, a, b, c);
+    for (int p : {7, 11, 13})
+        while (m % p == 0) { m /= p; if (extra) extra->push_back(p); }
     return m == 1;
 }
 
 int line_plan_reg_P(int L) {
     int a, b, c;
-    if (L < 4 || !factor235(L, a, b, c)) return 0;
+    if (L < 4 || factor235(L, a, b, c) != 1) return 0;
     int P = 0;
     if (b == 0 && c == 0) P = (a >= 3) ? 8 : (a == 2 ? 4 : 0);
     else if (b == 1 && c == 0 && a >= 2) P = 12;
@@ -49,6 +59,7 @@ static void gen_radices(int L, std::vector<int>& r) {
     for (int i = 0; i < a / 3; ++i) r.push_back(8);
     for (int i = 0; i < b; ++i) r.push_back(3);
     for (int i = 0; i < c; ++i) r.push_back(5);
+    generic_ok(L, &r);
 }
 
 int line_plan_gen_Lp(const LinePlan& lp) { return lp.L + 1; }
@@ -86,8 +97,7 @@ static void phase(long j, long n, double& c, double& s) {
 int line_plan_build(LinePlan* lp, int L, int prec, bool allow_reg) {
     MCS_CHECK(L >= 1, MCS_ERR_INVALID, "FFT length must be >= 1");
     MCS_CHECK(prec == 32 || prec == 64, MCS_ERR_INVALID, "precision must be 32 or 64");
-    int a, b, c;
-    MCS_CHECK(factor235(L, a, b, c), MCS_ERR_UNSUPPORTED, "FFT length must be 2,3,5-smooth");
+    MCS_CHECK(generic_ok(L, nullptr), MCS_ERR_UNSUPPORTED, "FFT length must have no prime factor above 13");
     lp->L = L;
     lp->prec = prec;
     std::vector<int> rad;

@@ -1,0 +1,205 @@
+// Device bodies of the stepper's passes, shared by the single-grid kernels and
+// the ensemble's work-table kernels.  Each body processes one work unit (a
+// tile of columns, a block of rows or a block of row pairs) with the LDS the
+// caller provides; a persistent caller loops over units and must
+// __syncthreads() between them.
+#pragma once
+
+#include "fft_kernels.hpp"
+#include "pfc_math.hpp"
+#include "philox.hpp"
+
+// ------------------------------------------------------------------ column pass
+// tile of G adjacent x columns (thread c = tid % G, t = tid / G), L = pl.L points each
+template <typename F, int P, int DIR>
+MCS_DEV void col_body(const C2<F>* src, C2<F>* dst, const ColGeom& g, int G, const FftPlanDev& pl,
+                      const C2<F>* __restrict__ tw, C2<F>* lds, int tile, int outer) {
+    const int S = pl.L / P;
+    const int c = threadIdx.x % G, t = threadIdx.x / G;
+    const int x = tile * G + c;
+    const bool valid = x < g.nxc && t < S;
+    const long base = (long)outer * g.os + x;
+    C2<F> a[P];
+#pragma unroll
+    for (int m = 0; m < P; ++m) a[m] = valid ? src[base + (long)(t + m * S) * g.ls] : mk<F>(0, 0);
+    reg_fft<F, P, DIR>(a, pl, tw, t < S ? t : 0, lds, c, G);
+    if (valid) {
+#pragma unroll
+        for (int m = 0; m < P; ++m) dst[base + (long)(t + m * S) * g.ls] = a[m];
+    }
+}
+
+// column pass with the nonlinearity between the inverse and forward transforms:
+// u = scale * value holds psi at (2m, 2m+1); N = psi^3 (or psi^3 - psi)
+template <typename F, int P, int SPLIT>
+MCS_DEV void col_nl_body(C2<F>* W, const ColGeom& g, int G, const FftPlanDev& pl, const C2<F>* __restrict__ tw,
+                         F scale, C2<F>* lds, int tile, int outer) {
+    const int S = pl.L / P;
+    const int c = threadIdx.x % G, t = threadIdx.x / G;
+    const int x = tile * G + c;
+    const bool valid = x < g.nxc && t < S;
+    const int tt = t < S ? t : 0;
+    const long base = (long)outer * g.os + x;
+    C2<F> a[P];
+#pragma unroll
+    for (int m = 0; m < P; ++m) a[m] = valid ? W[base + (long)(t + m * S) * g.ls] : mk<F>(0, 0);
+    reg_fft<F, P, +1>(a, pl, tw, tt, lds, c, G);
+#pragma unroll
+    for (int m = 0; m < P; ++m) {
+        const F u = a[m].x * scale, v = a[m].y * scale;
+        if (SPLIT == MCS_SPLIT_REFERENCE) a[m] = mk<F>(u * u * u - u, v * v * v - v);
+        else a[m] = mk<F>(u * u * u, v * v * v);
+    }
+    reg_fft<F, P, -1>(a, pl, tw, tt, lds, c, G);
+    if (valid) {
+#pragma unroll
+        for (int m = 0; m < P; ++m) W[base + (long)(t + m * S) * g.ls] = a[m];
+    }
+}
+
+// ------------------------------------------------------------------ row pass on packed rows
+template <typename F, int P, int DIR>
+MCS_DEV void row_body(const C2<F>* src, C2<F>* dst, long nrows, int G, const FftPlanDev& pl,
+                      const C2<F>* __restrict__ tw, C2<F>* lds, long blk) {
+    const int L = pl.L, S = L / P;
+    const int t = threadIdx.x % S, g = threadIdx.x / S;
+    const long row = blk * G + g;
+    const bool valid = row < nrows && g < G;
+    C2<F> a[P];
+#pragma unroll
+    for (int m = 0; m < P; ++m) a[m] = valid ? src[row * L + t + m * S] : mk<F>(0, 0);
+    reg_fft<F, P, DIR>(a, pl, tw, t, lds, (g < G ? g : 0) * L, 1);
+    if (valid) {
+#pragma unroll
+        for (int m = 0; m < P; ++m) dst[row * L + t + m * S] = a[m];
+    }
+}
+
+// ------------------------------------------------------------------ x pass with the update
+template <typename F>
+struct XArgs {
+    int nx, ny, nz, L;
+    F dkx, dky, dkz;  // 2 pi / (n h) per axis
+    F dt, lin0, amp;
+    uint32_t key0, key1;
+    int noise;
+    double mass;  // FP64 k=0 mode (sum of psi), re-imposed exactly
+};
+
+MCS_DEV int ksigned(int i, int n) { return i < n / 2 + 1 ? i : i - n; }
+
+// One packed mode p of row c = (iy, iz) together with mode -p of the mirror row m = (my, mz):
+//   E = (A + conj B)/2, O = -i (A - conj B)/2 with A = U(c, p), B = U(m, -p)
+//   Psi1 = E + w^p O (full mode p), Psi2 = E - w^p O (full mode p + nx/2)
+// both modes are updated, then E' = (Psi1' + Psi2')/2, O' = (Psi1' - Psi2') conj(w^p) / 2 and
+//   U'(c, p) = E' + i O',  U'(m, -p) = conj(E') + i conj(O').
+template <typename F>
+MCS_DEV void xupd_pair(C2<F> nA, C2<F> nBr, C2<F> pA, C2<F> pBr, int p, int iy, int iz, int my, int mz, C2<F> w,
+                       const XArgs<F>& ar, uint32_t draw, C2<F>* uc, C2<F>* um) {
+    const C2<F> nB = cconj(nBr), pB = cconj(pBr);
+    const F h = F(0.5);
+    const C2<F> eN = cscale(cadd(nA, nB), h), oN = cscale(cmuli<-1>(csub(nA, nB)), h);
+    const C2<F> eP = cscale(cadd(pA, pB), h), oP = cscale(cmuli<-1>(csub(pA, pB)), h);
+    const C2<F> woN = cmul(w, oN), woP = cmul(w, oP);
+    const C2<F> n1 = cadd(eN, woN), n2 = csub(eN, woN);
+    const C2<F> p1 = cadd(eP, woP), p2 = csub(eP, woP);
+    const F ky = F(ksigned(iy, ar.ny)) * ar.dky, kz = F(ksigned(iz, ar.nz)) * ar.dkz;
+    const F kyz = ky * ky + kz * kz;
+    const F kx1 = F(p) * ar.dkx;
+    const F kx2 = F(ar.L - p) * ar.dkx;  // |k_x| of index p + nx/2 (the Nyquist index for p = 0)
+    C2<F> xi1 = mk<F>(0, 0), xi2 = mk<F>(0, 0);
+    if (ar.noise) {
+        xi1 = noise_xi<F>(p, iy, iz, ar.nx, ar.ny, ar.nz, ar.key0, ar.key1, draw);
+        xi2 = p == 0 ? noise_xi<F>(ar.L, iy, iz, ar.nx, ar.ny, ar.nz, ar.key0, ar.key1, draw)
+                     : cconj(noise_xi<F>(ar.L - p, my, mz, ar.nx, ar.ny, ar.nz, ar.key0, ar.key1, draw));
+    }
+    const C2<F> q1 = pfc_mode_update<F>(p1, n1, xi1, kx1 * kx1 + kyz, ar.dt, ar.lin0, ar.amp);
+    const C2<F> q2 = pfc_mode_update<F>(p2, n2, xi2, kx2 * kx2 + kyz, ar.dt, ar.lin0, ar.amp);
+    if (p == 0 && iy == 0 && iz == 0) {
+        // k = 0: the mean density is carried in FP64 and re-imposed exactly
+        const double m = ar.mass;
+        const double er = 0.5 * (m + (double)q2.x), ei = 0.5 * (double)q2.y;
+        const double or_ = 0.5 * (m - (double)q2.x), oi = -0.5 * (double)q2.y;
+        *uc = mk<F>((F)(er - oi), (F)(ei + or_));
+        *um = *uc;
+        return;
+    }
+    const C2<F> e = cscale(cadd(q1, q2), h);
+    const C2<F> o = cscale(cmulc(csub(q1, q2), w), h);
+    *uc = cadd(e, cmuli<+1>(o));
+    *um = cadd(cconj(e), cmuli<+1>(cconj(o)));
+}
+
+// G row pairs (row, mirror row) per unit, S = L/P threads per pair; LDS 2 rows per pair.
+// Each thread updates the mode pairs {(c, p), (m, -p)} of its own positions p once; the
+// mirror row's results move to their owners through LDS.  A self-mirrored row (c == m)
+// is processed as row c alone.
+template <typename F, int P, int SPLIT>
+MCS_DEV void xupd_body(C2<F>* __restrict__ W, C2<F>* __restrict__ U, const int2* __restrict__ pairs, int npairs,
+                       int G, const XArgs<F>& ar, const FftPlanDev& pl, const C2<F>* __restrict__ tw,
+                       const C2<F>* __restrict__ wph, uint32_t draw, C2<F>* lds, int blk) {
+    const int L = pl.L, S = L / P;
+    const int t = threadIdx.x % S, g = threadIdx.x / S;
+    const int q = blk * G + g;
+    const bool valid = q < npairs && g < G;
+    const int2 pr = pairs[valid ? q : 0];
+    const long rc = pr.x, rm = pr.y;
+    const bool self = rc == rm;
+    const int base = (g < G ? g : 0) * 2 * L;
+    C2<F>* Nc = lds + base;
+    C2<F>* Nm = Nc + L;
+
+    C2<F> a[P], b[P], pa[P], pb[P];
+#pragma unroll
+    for (int m = 0; m < P; ++m) {
+        const int p = t + m * S;
+        const int pp = p == 0 ? 0 : L - p;
+        a[m] = valid ? W[rc * L + p] : mk<F>(0, 0);
+        b[m] = valid ? W[rm * L + p] : mk<F>(0, 0);
+        pa[m] = valid ? U[rc * L + p] : mk<F>(0, 0);
+        pb[m] = valid ? U[rm * L + pp] : mk<F>(0, 0);
+    }
+    reg_fft<F, P, -1>(a, pl, tw, t, lds, base, 1);
+    reg_fft<F, P, -1>(b, pl, tw, t, lds, base + L, 1);
+#pragma unroll
+    for (int m = 0; m < P; ++m) {
+        const int p = t + m * S;
+        Nc[p] = a[m];
+        Nm[p] = b[m];
+    }
+    __syncthreads();
+    const int iyc = (int)(rc % ar.ny), izc = (int)(rc / ar.ny);
+    const int iym = (int)(rm % ar.ny), izm = (int)(rm / ar.ny);
+#pragma unroll
+    for (int m = 0; m < P; ++m) {
+        const int p = t + m * S;
+        const int pp = p == 0 ? 0 : L - p;
+        xupd_pair<F>(Nc[p], Nm[pp], pa[m], pb[m], p, iyc, izc, iym, izm, wph[p], ar, draw, &a[m], &b[m]);
+    }
+    __syncthreads();
+    if (valid) {
+#pragma unroll
+        for (int m = 0; m < P; ++m) {
+            const int p = t + m * S;
+            const int pp = p == 0 ? 0 : L - p;
+            U[rc * L + p] = a[m];
+            if (!self) {
+                U[rm * L + pp] = b[m];
+                Nm[pp] = b[m];
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < P; ++m) b[m] = self ? a[m] : Nm[t + m * S];
+    __syncthreads();
+    reg_fft<F, P, +1>(a, pl, tw, t, lds, base, 1);
+    reg_fft<F, P, +1>(b, pl, tw, t, lds, base + L, 1);
+    if (valid) {
+#pragma unroll
+        for (int m = 0; m < P; ++m) {
+            W[rc * L + t + m * S] = a[m];
+            if (!self) W[rm * L + t + m * S] = b[m];
+        }
+    }
+}

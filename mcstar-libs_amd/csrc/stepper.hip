@@ -18,107 +18,29 @@
 //   S2  y cols:  FFT_y(W)
 // 10 sweeps of N/2 complex values per step (S3 reads W and U, writes U and W).
 // 2D: S3 on x rows, S1 on y columns (6 sweeps).
-#include "fft_kernels.hpp"
-#include "pfc_math.hpp"
-#include "philox.hpp"
+#include "pass_bodies.hpp"
 #include "stepper.hpp"
 
 #include <cmath>
 #include <vector>
 
-// ------------------------------------------------------------------ plain row pass (packed rows)
+// ------------------------------------------------------------------ kernels (one work unit per workgroup;
+// every launch uses blockDim.x == G * S, the invariant the bodies rely on)
 template <typename F, int P, int DIR>
 __global__ void __launch_bounds__(1024) k_row_reg(const C2<F>* src, C2<F>* dst, long nrows, int G, FftPlanDev pl,
                                                   const C2<F>* __restrict__ tw) {
     extern __shared__ unsigned char smem_raw[];
-    C2<F>* lds = reinterpret_cast<C2<F>*>(smem_raw);
-    const int L = pl.L, S = L / P;
-    const int t = threadIdx.x % S, g = threadIdx.x / S;
-    const long row = (long)blockIdx.x * G + g;
-    const bool valid = row < nrows;
-    C2<F> a[P];
-#pragma unroll
-    for (int m = 0; m < P; ++m) a[m] = valid ? src[row * L + t + m * S] : mk<F>(0, 0);
-    reg_fft<F, P, DIR>(a, pl, tw, t, lds, g * L, 1);
-    if (valid) {
-#pragma unroll
-        for (int m = 0; m < P; ++m) dst[row * L + t + m * S] = a[m];
-    }
+    row_body<F, P, DIR>(src, dst, nrows, G, pl, tw, reinterpret_cast<C2<F>*>(smem_raw), blockIdx.x);
 }
 
-// ------------------------------------------------------------------ S1: column pass with the nonlinearity
 template <typename F, int P, int SPLIT>
 __global__ void __launch_bounds__(1024) k_col_nl(C2<F>* W, ColGeom g, int G, FftPlanDev pl,
                                                  const C2<F>* __restrict__ tw, F scale, int64_t* d_step) {
     extern __shared__ unsigned char smem_raw[];
-    C2<F>* lds = reinterpret_cast<C2<F>*>(smem_raw);
-    const int S = pl.L / P;
     const int tilesx = (g.nxc + G - 1) / G;
-    const int tile = blockIdx.x % tilesx, outer = blockIdx.x / tilesx;
-    const int c = threadIdx.x % G, t = threadIdx.x / G;
-    const int x = tile * G + c;
-    const bool valid = x < g.nxc;
-    const long base = (long)outer * g.os + x;
-    C2<F> a[P];
-#pragma unroll
-    for (int m = 0; m < P; ++m) a[m] = valid ? W[base + (long)(t + m * S) * g.ls] : mk<F>(0, 0);
-    reg_fft<F, P, +1>(a, pl, tw, t, lds, c, G);
-#pragma unroll
-    for (int m = 0; m < P; ++m) {
-        const F u = a[m].x * scale, v = a[m].y * scale;
-        if (SPLIT == MCS_SPLIT_REFERENCE) a[m] = mk<F>(u * u * u - u, v * v * v - v);
-        else a[m] = mk<F>(u * u * u, v * v * v);
-    }
-    reg_fft<F, P, -1>(a, pl, tw, t, lds, c, G);
-    if (valid) {
-#pragma unroll
-        for (int m = 0; m < P; ++m) W[base + (long)(t + m * S) * g.ls] = a[m];
-    }
+    col_nl_body<F, P, SPLIT>(W, g, G, pl, tw, scale, reinterpret_cast<C2<F>*>(smem_raw), blockIdx.x % tilesx,
+                             blockIdx.x / tilesx);
     if (d_step && blockIdx.x == 0 && threadIdx.x == 0) *d_step += 1;
-}
-
-// ------------------------------------------------------------------ S3: x rows, update, inverse
-template <typename F>
-struct XArgs {
-    int nx, ny, nz, L;
-    F lx, ly, lz, dt, lin0, amp;
-    uint32_t key0, key1;
-    int noise;
-    double mass;  // FP64 k=0 mode (sum of psi), kept exactly
-};
-
-template <typename F, int SPLIT>
-MCS_DEV C2<F> xupd_elem(C2<F> nA, C2<F> nBr, C2<F> pA, C2<F> pBr, int p, int iy, int iz, int my, int mz, C2<F> w,
-                        const XArgs<F>& ar, uint32_t draw) {
-    const C2<F> nB = cconj(nBr), pB = cconj(pBr);
-    const F h = F(0.5);
-    const C2<F> eN = cscale(cadd(nA, nB), h), oN = cscale(cmuli<-1>(csub(nA, nB)), h);
-    const C2<F> eP = cscale(cadd(pA, pB), h), oP = cscale(cmuli<-1>(csub(pA, pB)), h);
-    const C2<F> woN = cmul(w, oN), woP = cmul(w, oP);
-    const C2<F> n1 = cadd(eN, woN), n2 = csub(eN, woN);
-    const C2<F> p1 = cadd(eP, woP), p2 = csub(eP, woP);
-    const F ky = kwave<F>(iy, ar.ny, ar.ly), kz = kwave<F>(iz, ar.nz, ar.lz);
-    const F kyz = ky * ky + kz * kz;
-    const F kx1 = kwave<F>(p, ar.nx, ar.lx);
-    const F kx2 = kwave<F>(ar.L - p, ar.nx, ar.lx);  // |k_x| of index p + nx/2
-    C2<F> xi1 = mk<F>(0, 0), xi2 = mk<F>(0, 0);
-    if (ar.noise) {
-        xi1 = noise_xi<F>(p, iy, iz, ar.nx, ar.ny, ar.nz, ar.key0, ar.key1, draw);
-        xi2 = p == 0 ? noise_xi<F>(ar.L, iy, iz, ar.nx, ar.ny, ar.nz, ar.key0, ar.key1, draw)
-                     : cconj(noise_xi<F>(ar.L - p, my, mz, ar.nx, ar.ny, ar.nz, ar.key0, ar.key1, draw));
-    }
-    const C2<F> q1 = pfc_mode_update<F>(p1, n1, xi1, kx1 * kx1 + kyz, ar.dt, ar.lin0, ar.amp);
-    const C2<F> q2 = pfc_mode_update<F>(p2, n2, xi2, kx2 * kx2 + kyz, ar.dt, ar.lin0, ar.amp);
-    if (p == 0 && iy == 0 && iz == 0) {
-        // k = 0: the mean density is carried in FP64 and re-imposed exactly
-        const double m = ar.mass;
-        const double er = 0.5 * (m + (double)q2.x), ei = 0.5 * (double)q2.y;
-        const double or_ = 0.5 * (m - (double)q2.x), oi = -0.5 * (double)q2.y;
-        return mk<F>((F)(er - oi), (F)(ei + or_));
-    }
-    const C2<F> e = cscale(cadd(q1, q2), h);
-    const C2<F> o = cscale(cmulc(csub(q1, q2), w), h);
-    return cadd(e, cmuli<+1>(o));
 }
 
 template <typename F, int P, int SPLIT>
@@ -127,64 +49,9 @@ __global__ void __launch_bounds__(256) k_xupd(C2<F>* __restrict__ W, C2<F>* __re
                                               const C2<F>* __restrict__ tw, const C2<F>* __restrict__ wph,
                                               const int64_t* __restrict__ d_step) {
     extern __shared__ unsigned char smem_raw[];
-    C2<F>* lds = reinterpret_cast<C2<F>*>(smem_raw);
-    const int L = pl.L, S = L / P;
-    const int t = threadIdx.x % S, g = threadIdx.x / S;
-    const int q = blockIdx.x * G + g;
-    const bool valid = q < npairs;
-    const int2 pr = pairs[valid ? q : 0];
-    const long rc = pr.x, rm = pr.y;
-    const int base = g * 4 * L;
-    C2<F>* Nc = lds + base;
-    C2<F>* Nm = Nc + L;
-    C2<F>* Pc = Nm + L;
-    C2<F>* Pm = Pc + L;
     const uint32_t draw = ar.noise ? (uint32_t)*d_step : 0u;
-
-    C2<F> a[P], b[P];
-#pragma unroll
-    for (int m = 0; m < P; ++m) {
-        a[m] = valid ? W[rc * L + t + m * S] : mk<F>(0, 0);
-        b[m] = valid ? W[rm * L + t + m * S] : mk<F>(0, 0);
-    }
-    reg_fft<F, P, -1>(a, pl, tw, t, lds, base, 1);
-    reg_fft<F, P, -1>(b, pl, tw, t, lds, base + L, 1);
-#pragma unroll
-    for (int m = 0; m < P; ++m) {
-        const int p = t + m * S;
-        Nc[p] = a[m];
-        Nm[p] = b[m];
-        Pc[p] = valid ? U[rc * L + p] : mk<F>(0, 0);
-        Pm[p] = valid ? U[rm * L + p] : mk<F>(0, 0);
-    }
-    __syncthreads();
-    const int iyc = (int)(rc % ar.ny), izc = (int)(rc / ar.ny);
-    const int iym = (int)(rm % ar.ny), izm = (int)(rm / ar.ny);
-#pragma unroll
-    for (int m = 0; m < P; ++m) {
-        const int p = t + m * S;
-        const int pp = p == 0 ? 0 : L - p;
-        const C2<F> w = wph[p];
-        a[m] = xupd_elem<F, SPLIT>(Nc[p], Nm[pp], Pc[p], Pm[pp], p, iyc, izc, iym, izm, w, ar, draw);
-        b[m] = xupd_elem<F, SPLIT>(Nm[p], Nc[pp], Pm[p], Pc[pp], p, iym, izm, iyc, izc, w, ar, draw);
-    }
-    __syncthreads();
-    if (valid) {
-#pragma unroll
-        for (int m = 0; m < P; ++m) {
-            U[rc * L + t + m * S] = a[m];
-            U[rm * L + t + m * S] = b[m];
-        }
-    }
-    reg_fft<F, P, +1>(a, pl, tw, t, lds, base, 1);
-    reg_fft<F, P, +1>(b, pl, tw, t, lds, base + L, 1);
-    if (valid) {
-#pragma unroll
-        for (int m = 0; m < P; ++m) {
-            W[rc * L + t + m * S] = a[m];
-            W[rm * L + t + m * S] = b[m];
-        }
-    }
+    xupd_body<F, P, SPLIT>(W, U, pairs, npairs, G, ar, pl, tw, wph, draw, reinterpret_cast<C2<F>*>(smem_raw),
+                           blockIdx.x);
 }
 
 // ------------------------------------------------------------------ helpers
@@ -250,10 +117,7 @@ static int launch_col_nl_t(const LinePlan& lp, void* W, const ColGeom& g, double
                            hipStream_t s) {
     const int S = lp.L / P;
     const size_t esz = sizeof(C2<F>);
-    int G = (int)(128 / esz);
-    if (G > 1024 / S) G = 1024 / S;
-    while (G > 1 && (size_t)G * lp.L * esz > 65536) --G;
-    if (G < 1) G = 1;
+    const int G = col_lines_per_wg(lp);
     const size_t lds = (size_t)G * lp.L * esz;
     auto fn = k_col_nl<F, P, SPLIT>;
     int st = set_lds_attr((const void*)fn, lds);
@@ -291,9 +155,8 @@ static int launch_xupd_t(const StepGeom& sg, void* W, void* U, const XArgs<F>& a
     const LinePlan& lp = sg.px;
     const int S = lp.L / P;
     const size_t esz = sizeof(C2<F>);
-    int G = S >= 256 ? 1 : 256 / S;
-    while (G > 1 && (size_t)G * 4 * lp.L * esz > 65536) --G;
-    const size_t lds = (size_t)G * 4 * lp.L * esz;
+    const int G = xupd_pairs_per_wg(lp);
+    const size_t lds = (size_t)G * 2 * lp.L * esz;
     auto fn = k_xupd<F, P, SPLIT>;
     int st = set_lds_attr((const void*)fn, lds);
     if (st) return st;
@@ -312,21 +175,6 @@ static int launch_xupd_d(const StepGeom& sg, void* W, void* U, const XArgs<F>& a
         case 20: return launch_xupd_t<F, 20, SPLIT>(sg, W, U, ar, s);
         default: mcs_set_last_error(__FILE__, __LINE__, "x plan not register-eligible"); return MCS_ERR_UNSUPPORTED;
     }
-}
-
-template <typename F>
-static XArgs<F> make_xargs(const StepGeom& sg, const StepPhys& ph) {
-    XArgs<F> ar;
-    ar.nx = sg.nx; ar.ny = sg.ny; ar.nz = sg.nz; ar.L = sg.L;
-    ar.lx = (F)(sg.nx * ph.h[0]); ar.ly = (F)(sg.ny * ph.h[1]); ar.lz = (F)(sg.nz * ph.h[2]);
-    ar.dt = (F)ph.dt;
-    ar.lin0 = (F)(ph.split == MCS_SPLIT_REFERENCE ? 1.0 - ph.eps : -ph.eps);
-    const int ext[3] = {sg.nx, sg.ny, sg.nz};
-    ar.amp = (F)pfc_noise_amp(ext, ph.h, ph.dt, ph.temp);
-    ar.key0 = (uint32_t)ph.seed; ar.key1 = (uint32_t)(ph.seed >> 32);
-    ar.noise = ph.temp > 0;
-    ar.mass = ph.mass;
-    return ar;
 }
 
 int launch_xupd(const StepGeom& sg, const StepPhys& ph, void* W, void* U, hipStream_t s) {
@@ -384,8 +232,27 @@ void step_geom_free(StepGeom* sg) {
 }
 
 // column geometries on the packed [nz][ny][L] array
-static ColGeom geom_y(const StepGeom& sg) { return ColGeom{sg.L, (long)sg.L, sg.nz, (long)sg.L * sg.ny}; }
-static ColGeom geom_z(const StepGeom& sg) { return ColGeom{sg.L, (long)sg.L * sg.ny, sg.ny, (long)sg.L}; }
+ColGeom step_geom_y(const StepGeom& sg) { return ColGeom{sg.L, (long)sg.L, sg.nz, (long)sg.L * sg.ny}; }
+ColGeom step_geom_z(const StepGeom& sg) { return ColGeom{sg.L, (long)sg.L * sg.ny, sg.ny, (long)sg.L}; }
+static ColGeom geom_y(const StepGeom& sg) { return step_geom_y(sg); }
+static ColGeom geom_z(const StepGeom& sg) { return step_geom_z(sg); }
+
+int col_lines_per_wg(const LinePlan& lp) {
+    const int S = lp.L / lp.P;
+    const size_t esz = lp.prec == 64 ? 16 : 8;
+    int G = (int)(128 / esz);
+    if (G > 1024 / S) G = 1024 / S;
+    while (G > 1 && (size_t)G * lp.L * esz > 65536) --G;
+    return G < 1 ? 1 : G;
+}
+
+int xupd_pairs_per_wg(const LinePlan& lp) {
+    const int S = lp.L / lp.P;
+    const size_t esz = lp.prec == 64 ? 16 : 8;
+    int G = S >= 256 ? 1 : 256 / S;
+    while (G > 1 && (size_t)G * 2 * lp.L * esz > 65536) --G;
+    return G;
+}
 
 // W holds IFFT_x(U) on entry; leaves FFT_yz(N(psi)) in W
 int step_nonlinear_half(const StepGeom& sg, const StepPhys& ph, void* W, int64_t* d_step, hipStream_t s) {
@@ -482,7 +349,8 @@ int step_export(const StepGeom& sg, const void* U, void* psi, void* tmp, hipStre
     if (!st) st = launch_col(sg.py, tmp, tmp, geom_y(sg), +1, s);
     if (!st && sg.is3d) st = launch_col(sg.pz, tmp, tmp, geom_z(sg), +1, s);
     if (st) return st;
-    const double sc = 1.0 / (double)n;
+    // the packed inverse over N/2 points returns (N/2) u
+    const double sc = 2.0 / (double)n;
     if (sg.prec == 64)
         hipLaunchKernelGGL(k_scale_copy<double>, dim3(2048), dim3(256), 0, s, (const double*)tmp, (double*)psi, n, sc);
     else

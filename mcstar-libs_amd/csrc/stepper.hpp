@@ -1,7 +1,9 @@
 // Fused-step building blocks shared by the single-grid stepper and the ensemble.
 #pragma once
 
-#include "fft_kernels.hpp"
+#include "pass_bodies.hpp"
+
+#include <cmath>
 
 struct StepGeom {
     int nx = 0, ny = 0, nz = 0, L = 0, prec = 32;
@@ -29,3 +31,28 @@ int step_run(const StepGeom& sg, const StepPhys& ph, void* W, void* U, int nstep
 int launch_row_packed(const LinePlan& lp, const void* src, void* dst, long nrows, int dir, hipStream_t s);
 
 double pfc_noise_amp(const int ext[3], const double h[3], double dt, double temp);
+
+template <typename F> inline XArgs<F> make_xargs(const StepGeom& sg, const StepPhys& ph) {
+    XArgs<F> ar;
+    ar.nx = sg.nx; ar.ny = sg.ny; ar.nz = sg.nz; ar.L = sg.L;
+    ar.dkx = (F)(2.0 * M_PI / (sg.nx * ph.h[0]));
+    ar.dky = (F)(2.0 * M_PI / (sg.ny * ph.h[1]));
+    ar.dkz = (F)(2.0 * M_PI / (sg.nz * ph.h[2]));
+    ar.dt = (F)ph.dt;
+    ar.lin0 = (F)(ph.split == MCS_SPLIT_REFERENCE ? 1.0 - ph.eps : -ph.eps);
+    const int ext[3] = {sg.nx, sg.ny, sg.nz};
+    ar.amp = (F)pfc_noise_amp(ext, ph.h, ph.dt, ph.temp);
+    ar.key0 = (uint32_t)ph.seed;
+    ar.key1 = (uint32_t)(ph.seed >> 32);
+    ar.noise = ph.temp > 0;
+    ar.mass = ph.mass;
+    return ar;
+}
+
+// lines per workgroup of a column pass / row pairs per workgroup of the x pass
+int col_lines_per_wg(const LinePlan& lp);
+int xupd_pairs_per_wg(const LinePlan& lp);
+int launch_col_nl(const LinePlan& lp, void* W, const ColGeom& g, double scale, int split, int64_t* d_step,
+                  hipStream_t s);
+ColGeom step_geom_y(const StepGeom& sg);
+ColGeom step_geom_z(const StepGeom& sg);

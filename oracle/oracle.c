@@ -306,6 +306,28 @@ void orc_update(double* kpsid, const double* knld, const double* xid, const int 
         }
 }
 
+/* Keeps the half spectrum Hermitian on the self-conjugate x planes (i0 = 0, nx/2).
+   c2r ignores the anti-Hermitian part there, so the nonlinearity never sees it,
+   and round-off in it would grow at the linear rate of the unstable band; the
+   projection is what a round trip through the real field does. */
+static void hermitian_planes(cplx* k, const int n[3]) {
+    const int nx = n[0], ny = n[1], nz = n[2], nxc = nx / 2 + 1;
+    const int planes[2] = {0, nx / 2};
+    const int np = (nx % 2 == 0 && nx > 1) ? 2 : 1;
+    for (int pi = 0; pi < np; pi++) {
+        const int i0 = planes[pi];
+        for (int i2 = 0; i2 < nz; i2++)
+            for (int i1 = 0; i1 < ny; i1++) {
+                const int m1 = (ny - i1) % ny, m2 = (nz - i2) % nz;
+                const long a = ((long)i2 * ny + i1) * nxc + i0, b = ((long)m2 * ny + m1) * nxc + i0;
+                if (a > b) continue;
+                const cplx v = 0.5 * (k[a] + conj(k[b]));
+                k[a] = v;
+                k[b] = conj(v);
+            }
+    }
+}
+
 int orc_pfc_run(double* psi, const int n[3], const double h[3], double eps, double dt, double temp,
                 uint64_t seed, int step0, int nsteps, int split, double* mass_hist) {
     const int nx = n[0], ny = n[1], nz = n[2], nxc = nx / 2 + 1;
@@ -329,6 +351,7 @@ int orc_pfc_run(double* psi, const int n[3], const double h[3], double eps, doub
         orc_r2c(nl, knl, n);
         if (xi) orc_kspace_noise(xi, n, seed, (uint32_t)(step0 + s));
         orc_update(kpsi, knl, xi, n, lens, dt, eps, namp, split);
+        hermitian_planes((cplx*)kpsi, n);
         if (mass_hist) mass_hist[s + 1] = kpsi[0] / (double)N;
     }
     orc_c2r(kpsi, psi, n);

@@ -78,5 +78,5 @@ def test_c2r_leaves_input_untouched(mcs):
 
 def test_unsupported_length_is_an_error(mcs):
     with pytest.raises(mcs.McsError) as e:
-        mcs.FFTPlan((7, 14), np.float32, mcs.R2C)
+        mcs.FFTPlan((17, 34), np.float32, mcs.R2C)
     assert e.value.code == mcs.ERR_UNSUPPORTED

@@ -85,7 +85,7 @@ def test_peaks_and_bond_order_flow(tmp_path, orc):
     np.save(fin, f)
     pk, bo = str(tmp_path / "peaks.txt"), str(tmp_path / "bo.txt")
     h = 0.5
-    r = run("f64", "peaks", n, n, n, fin, pk, bo, h, 0.5, 5.0)
+    r = run("f64", "peaks", n, n, n, fin, pk, bo, h, 0.5, 5.5)
     idx = orc.findpeaks(f, 0.5)
     assert f"found {idx.size} maxima" in r.stderr
     rows = np.loadtxt(pk)
@@ -95,7 +95,8 @@ def test_peaks_and_bond_order_flow(tmp_path, orc):
     table = np.loadtxt(bo, comments="#")
     keep = f.reshape(-1)[idx] > 0.5
     xyz = np.column_stack([i0, i1, i2])[keep] * h
-    nb, out = orc.bond_order_cutoff(xyz, n * h, 5.0)
+    nb, out = orc.bond_order_cutoff(xyz, n * h, 5.5)
+    assert np.all(nb == 8)  # bcc: the 8 nearest sites at sqrt(3)/2 * 12 cells
     m = nb > 0
     want = np.column_stack([xyz[m], nb[m], out[m][:, 8], out[m][:, :8]])
     assert table.shape == want.shape
