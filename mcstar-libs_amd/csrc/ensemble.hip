@@ -23,7 +23,7 @@ template <typename F> struct EnsRep {
 };
 
 // ------------------------------------------------------------------ kernels
-template <typename F, int P, int SPLIT>
+template <typename F, int P, int SPLIT, bool WAVE>
 __global__ void __launch_bounds__(256) k_ens_xupd(const EnsRep<F>* __restrict__ reps, int nr, int units,
                                                   const int2* __restrict__ pairs, int npairs, int G, FftPlanDev pl,
                                                   const C2<F>* __restrict__ tw, const C2<F>* __restrict__ wph,
@@ -35,13 +35,13 @@ __global__ void __launch_bounds__(256) k_ens_xupd(const EnsRep<F>* __restrict__ 
     for (long it = blockIdx.x; it < total; it += gridDim.x) {
         const int r = (int)(it / units), u = (int)(it % units);
         const EnsRep<F>& R = reps[r];
-        xupd_body<F, P, SPLIT>(R.W, R.U, pairs, npairs, G, R.xa, pl, tw, wph, draw, lds, u);
+        xupd_body<F, P, SPLIT, WAVE>(R.W, R.U, pairs, npairs, G, R.xa, pl, tw, wph, draw, lds, u);
         __syncthreads();
     }
 }
 
 template <typename F, int P, int DIR>
-__global__ void __launch_bounds__(1024) k_ens_col(const EnsRep<F>* __restrict__ reps, int nr, ColGeom g, int G,
+__global__ void __launch_bounds__(MCS_REG_MAX_THREADS(P, F)) k_ens_col(const EnsRep<F>* __restrict__ reps, int nr, ColGeom g, int G,
                                                   FftPlanDev pl, const C2<F>* __restrict__ tw) {
     extern __shared__ unsigned char smem_raw[];
     C2<F>* lds = reinterpret_cast<C2<F>*>(smem_raw);
@@ -58,7 +58,7 @@ __global__ void __launch_bounds__(1024) k_ens_col(const EnsRep<F>* __restrict__ 
 }
 
 template <typename F, int P, int SPLIT>
-__global__ void __launch_bounds__(1024) k_ens_col_nl(const EnsRep<F>* __restrict__ reps, int nr, ColGeom g, int G,
+__global__ void __launch_bounds__(MCS_REG_MAX_THREADS(P, F)) k_ens_col_nl(const EnsRep<F>* __restrict__ reps, int nr, ColGeom g, int G,
                                                      FftPlanDev pl, const C2<F>* __restrict__ tw, F scale,
                                                      int64_t* d_step) {
     extern __shared__ unsigned char smem_raw[];
@@ -128,11 +128,12 @@ static int upload_reps(mcs_ensemble* e) {
     return MCS_OK;
 }
 
-template <typename Fn> static unsigned persistent_grid(mcs_ensemble* e, Fn fn, int threads, size_t lds, long total) {
-    int per = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, threads, lds) != hipSuccess || per < 1) per = 1;
-    const long cap = (long)per * e->cus;
-    return (unsigned)std::max<long>(1, std::min<long>(total, cap));
+// resident workgroups per CU from the thread (2048 per CU) and LDS (160 KB) budgets, times 2 for slack
+static unsigned persistent_grid(mcs_ensemble* e, int threads, size_t lds, long total) {
+    long per = 2048 / std::max(threads, 64);
+    if (lds > 0) per = std::min<long>(per, (long)(MCS_LDS_MAX / lds));
+    per = std::max<long>(per, 1) * 2;
+    return (unsigned)std::max<long>(1, std::min<long>(total, per * e->cus));
 }
 
 // one x pass over members [first, first + nr) of group g
@@ -143,8 +144,8 @@ static int ens_xupd_t(mcs_ensemble* e, EnsGroup* g, int first, int nr, hipStream
     const int S = lp.L / P;
     const size_t lds = (size_t)G * 2 * lp.L * sizeof(C2<F>);
     const int units = (g->sg.npairs + G - 1) / G;
-    auto fn = k_ens_xupd<F, P, SPLIT>;
-    const unsigned grid = persistent_grid(e, fn, G * S, lds, (long)nr * units);
+    auto fn = (64 % S == 0) ? k_ens_xupd<F, P, SPLIT, true> : k_ens_xupd<F, P, SPLIT, false>;
+    const unsigned grid = persistent_grid(e, G * S, lds, (long)nr * units);
     hipLaunchKernelGGL(fn, dim3(grid), dim3(G * S), lds, s, (const EnsRep<F>*)g->d_reps + first, nr, units,
                        g->sg.pairs, g->sg.npairs, G, lp.dev, (const C2<F>*)lp.tw, (const C2<F>*)g->sg.wph,
                        (const int64_t*)e->d_step);
@@ -160,7 +161,7 @@ static int ens_col_t(mcs_ensemble* e, EnsGroup* g, const LinePlan& lp, const Col
     const size_t lds = (size_t)G * lp.L * sizeof(C2<F>);
     const long total = (long)nr * ((cg.nxc + G - 1) / G) * cg.nouter;
     auto fn = k_ens_col<F, P, DIR>;
-    const unsigned grid = persistent_grid(e, fn, G * S, lds, total);
+    const unsigned grid = persistent_grid(e, G * S, lds, total);
     hipLaunchKernelGGL(fn, dim3(grid), dim3(G * S), lds, s, (const EnsRep<F>*)g->d_reps + first, nr, cg, G, lp.dev,
                        (const C2<F>*)lp.tw);
     MCS_HIP_TRY(hipGetLastError());
@@ -175,7 +176,7 @@ static int ens_col_nl_t(mcs_ensemble* e, EnsGroup* g, const LinePlan& lp, const 
     const size_t lds = (size_t)G * lp.L * sizeof(C2<F>);
     const long total = (long)nr * ((cg.nxc + G - 1) / G) * cg.nouter;
     auto fn = k_ens_col_nl<F, P, SPLIT>;
-    const unsigned grid = persistent_grid(e, fn, G * S, lds, total);
+    const unsigned grid = persistent_grid(e, G * S, lds, total);
     const double scale = 2.0 / ((double)g->sg.nx * g->sg.ny * g->sg.nz);
     hipLaunchKernelGGL(fn, dim3(grid), dim3(G * S), lds, s, (const EnsRep<F>*)g->d_reps + first, nr, cg, G, lp.dev,
                        (const C2<F>*)lp.tw, (F)scale, inc ? e->d_step : nullptr);
@@ -189,6 +190,8 @@ static int ens_col_nl_t(mcs_ensemble* e, EnsGroup* g, const LinePlan& lp, const 
         case 8: { constexpr int PP = 8; return CALL; }                                      \
         case 12: { constexpr int PP = 12; return CALL; }                                    \
         case 20: { constexpr int PP = 20; return CALL; }                                    \
+        case 16: { constexpr int PP = 16; return CALL; }                                    \
+        case 32: { constexpr int PP = 32; return CALL; }                                    \
         default: mcs_set_last_error(__FILE__, __LINE__, "no register plan"); return MCS_ERR_UNSUPPORTED; \
     }
 

@@ -15,7 +15,7 @@
 // Column pass, register engine.  src/dst may alias (each workgroup reads its
 // whole tile before it writes).
 template <typename F, int P, int DIR>
-__global__ void __launch_bounds__(1024) k_col_reg(const C2<F>* src, C2<F>* dst, ColGeom g, int G, FftPlanDev pl,
+__global__ void __launch_bounds__(MCS_REG_MAX_THREADS(P, F)) k_col_reg(const C2<F>* src, C2<F>* dst, ColGeom g, int G, FftPlanDev pl,
                                                   const C2<F>* __restrict__ tw) {
     extern __shared__ unsigned char smem_raw[];
     C2<F>* lds = reinterpret_cast<C2<F>*>(smem_raw);
@@ -168,7 +168,7 @@ static int launch_col_reg_t(const LinePlan& lp, const void* src, void* dst, cons
     const int S = lp.L / P;
     const int esz = (int)sizeof(C2<F>);
     int G = (int)(128 / esz);
-    if (G > 1024 / S) G = 1024 / S;
+    if (G > mcs_reg_max_threads(P, sizeof(F) == 8 ? 64 : 32) / S) G = mcs_reg_max_threads(P, sizeof(F) == 8 ? 64 : 32) / S;
     while (G > 1 && (size_t)G * lp.L * esz > 65536) --G;
     if (G < 1) G = 1;
     const size_t lds = (size_t)G * lp.L * esz;
@@ -207,6 +207,8 @@ static int launch_col_t(const LinePlan& lp, const void* src, void* dst, const Co
             case 8: return launch_col_reg_t<F, 8, DIR>(lp, src, dst, g, s);
             case 12: return launch_col_reg_t<F, 12, DIR>(lp, src, dst, g, s);
             case 20: return launch_col_reg_t<F, 20, DIR>(lp, src, dst, g, s);
+            case 16: return launch_col_reg_t<F, 16, DIR>(lp, src, dst, g, s);
+            case 32: return launch_col_reg_t<F, 32, DIR>(lp, src, dst, g, s);
             default: break;
         }
     }

@@ -22,6 +22,12 @@
 #include "mcs_common.hpp"
 
 #define MCS_MAX_PASSES 16
+// workgroup size bound of register-engine kernels: with 16+ complex floats (8+ doubles) of data
+// per thread a 1024-thread workgroup would cap the kernel at 128 VGPRs
+#define MCS_REG_MAX_THREADS(P, F) (((P) * sizeof(F) > 64 || (sizeof(F) == 4 && (P) >= 16)) ? 256 : 1024)
+inline int mcs_reg_max_threads(int P, int prec) {
+    return (P * (prec == 64 ? 8 : 4) > 64 || (prec != 64 && P >= 16)) ? 256 : 1024;
+}
 
 struct FftPlanDev {
     int L;
@@ -99,6 +105,55 @@ template <typename F, int DIR> MCS_DEV void dft8(C2<F>* v) {
     v[7] = csub(e[3], o3);
 }
 
+// compile-time cos/sin of 2 pi j / n (Taylor series in double after reduction to [-pi, pi])
+constexpr double ce_cos2pi(long j, long n) {
+    j %= n;
+    if (j < 0) j += n;
+    if (2 * j > n) j -= n;
+    const double x = 6.283185307179586476925286766559 * (double)j / (double)n;
+    double term = 1.0, sum = 1.0;
+    for (int k = 1; k < 30; ++k) { term *= -x * x / ((2.0 * k - 1.0) * (2.0 * k)); sum += term; }
+    return sum;
+}
+// DFT of size N = N1 * N2 from N2 DFTs of size N1 (stride N2), twiddles W_N^{n2 k1}, and N1 DFTs of size N2:
+//   X[k1 + N1 k2] = sum_{n2} W_N2^{n2 k2} W_N^{n2 k1} sum_{n1} W_N1^{n1 k1} x[N2 n1 + n2]
+template <typename F, int N1, int N2, int DIR, typename D1, typename D2>
+MCS_DEV void dft_split(C2<F>* v, D1 inner, D2 outer) {
+    constexpr int N = N1 * N2;
+    C2<F> y[N];
+#pragma unroll
+    for (int n2 = 0; n2 < N2; ++n2) {
+        C2<F> t[N1];
+#pragma unroll
+        for (int n1 = 0; n1 < N1; ++n1) t[n1] = v[N2 * n1 + n2];
+        inner(t);
+#pragma unroll
+        for (int k1 = 0; k1 < N1; ++k1) {
+            const int j = n2 * k1;
+            if (j == 0) { y[n2 * N1 + k1] = t[k1]; continue; }
+            const C2<F> w = mk<F>((F)ce_cos2pi(j, N), (F)(DIR * ce_cos2pi(j - N / 4, N)));
+            y[n2 * N1 + k1] = cmul(t[k1], w);
+        }
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < N1; ++k1) {
+        C2<F> t[N2];
+#pragma unroll
+        for (int n2 = 0; n2 < N2; ++n2) t[n2] = y[n2 * N1 + k1];
+        outer(t);
+#pragma unroll
+        for (int k2 = 0; k2 < N2; ++k2) v[k1 + N1 * k2] = t[k2];
+    }
+}
+
+template <typename F, int DIR> MCS_DEV void dft16(C2<F>* v) {
+    dft_split<F, 4, 4, DIR>(v, [](C2<F>* t) { dft4<F, DIR>(t); }, [](C2<F>* t) { dft4<F, DIR>(t); });
+}
+
+template <typename F, int DIR> MCS_DEV void dft32(C2<F>* v) {
+    dft_split<F, 8, 4, DIR>(v, [](C2<F>* t) { dft8<F, DIR>(t); }, [](C2<F>* t) { dft4<F, DIR>(t); });
+}
+
 // direct DFT for the odd primes 7, 11, 13 (generic engine only); roots in FP64
 template <typename F, int R, int DIR> MCS_DEV void dftp(C2<F>* v) {
     C2<F> w[R];
@@ -127,6 +182,8 @@ template <typename F, int R, int DIR> MCS_DEV void dft(C2<F>* v) {
     else if constexpr (R == 4) dft4<F, DIR>(v);
     else if constexpr (R == 5) dft5<F, DIR>(v);
     else if constexpr (R == 8) dft8<F, DIR>(v);
+    else if constexpr (R == 16) dft16<F, DIR>(v);
+    else if constexpr (R == 32) dft32<F, DIR>(v);
 }
 
 // twiddle for pass with sub-length Ns: tw[k*(R-1) + q-1] = exp(-2 pi i q k / (Ns R)) (forward)
@@ -165,7 +222,20 @@ MCS_DEV void rpass(C2<F> (&a)[P], int t, int S, int Ns, const C2<F>* __restrict_
     }
 }
 
-template <typename F, int P, int DIR>
+// Synchronisation of an LDS exchange.  When all threads of a line sit in one
+// wavefront (S divides 64) the exchange is wave-local: LDS operations of a
+// wavefront complete in issue order, so a compiler fence at wavefront scope
+// suffices and no workgroup barrier is needed.
+template <bool WAVE> MCS_DEV void lds_sync() {
+    if constexpr (WAVE) {
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+    } else {
+        __syncthreads();
+    }
+}
+
+template <typename F, int P, int DIR, bool WAVE = false>
 MCS_DEV void reg_fft(C2<F> (&a)[P], const FftPlanDev& pl, const C2<F>* __restrict__ tw, int t, C2<F>* lds,
                      int lbase, int lps) {
     const int S = pl.L / P;
@@ -180,13 +250,15 @@ MCS_DEV void reg_fft(C2<F> (&a)[P], const FftPlanDev& pl, const C2<F>* __restric
             case 4: if constexpr (P % 4 == 0) rpass<F, P, 4, DIR>(a, t, S, Ns, twp, lds, lbase, lps, last); break;
             case 5: if constexpr (P % 5 == 0) rpass<F, P, 5, DIR>(a, t, S, Ns, twp, lds, lbase, lps, last); break;
             case 8: if constexpr (P % 8 == 0) rpass<F, P, 8, DIR>(a, t, S, Ns, twp, lds, lbase, lps, last); break;
+            case 16: if constexpr (P % 16 == 0) rpass<F, P, 16, DIR>(a, t, S, Ns, twp, lds, lbase, lps, last); break;
+            case 32: if constexpr (P % 32 == 0) rpass<F, P, 32, DIR>(a, t, S, Ns, twp, lds, lbase, lps, last); break;
             default: break;
         }
         if (!last) {
-            __syncthreads();
+            lds_sync<WAVE>();
 #pragma unroll
             for (int m = 0; m < P; ++m) a[m] = lds[lbase + (t + m * S) * lps];
-            __syncthreads();
+            lds_sync<WAVE>();
         }
     }
 }

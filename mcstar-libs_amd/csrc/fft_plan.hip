@@ -23,32 +23,51 @@ static bool generic_ok(int L, std::vector<int>* extra) {
     return m == 1;
 }
 
-int line_plan_reg_P(int L) {
+// Register-engine plan: P points per thread, every radix divides P, S = L/P <= 1024.
+// Powers of two take in-register radix-16/32 passes so that 256, 512 and 1024 need a
+// single LDS exchange per transform; 3*2^a and 5*2^a use one radix-3/5 pass and radix-4 passes.
+// FP64 keeps P = 8 for powers of two (radix-8 passes): larger register blocks of doubles
+// drop the kernels to one wavefront per SIMD.
+static int reg_plan(int L, int prec, std::vector<int>* rad, int max_p = 32) {
     int a, b, c;
     if (L < 4 || factor235(L, a, b, c) != 1) return 0;
+    std::vector<int> r;
     int P = 0;
-    if (b == 0 && c == 0) P = (a >= 3) ? 8 : (a == 2 ? 4 : 0);
-    else if (b == 1 && c == 0 && a >= 2) P = 12;
-    else if (b == 0 && c == 1 && a >= 2) P = 20;
-    if (P == 0) return 0;
-    if (L / P > 1024) return 0;
-    return P;
-}
-
-static void reg_radices(int L, int P, std::vector<int>& r) {
-    int a, b, c;
-    factor235(L, a, b, c);
-    r.clear();
-    if (P == 8 || P == 4) {
-        if (a % 3 == 1) r.push_back(2);
-        if (a % 3 == 2) r.push_back(4);
-        for (int i = 0; i < a / 3; ++i) r.push_back(8);
-    } else {
-        r.push_back(P == 12 ? 3 : 5);
+    if (b == 0 && c == 0) {
+        if (a == 2) { P = 4; r = {4}; }
+        else if (a <= 6 || prec == 64 || max_p < 16) {  // radix-8 passes
+            P = 8;
+            if (a % 3 == 1) r.push_back(2);
+            if (a % 3 == 2) r.push_back(4);
+            for (int i = 0; i < a / 3; ++i) r.push_back(8);
+        } else if (a == 7) { P = 16; r = {8, 16}; }
+        else if (a == 8) { P = 16; r = {16, 16}; }
+        else if (a == 9) { P = 32; r = {16, 32}; }
+        else if (a == 10) { P = 32; r = {32, 32}; }
+        else {  // radix-16 passes with the remainder first
+            P = 16;
+            if (a % 4 == 1) r.push_back(2);
+            if (a % 4 == 2) r.push_back(4);
+            if (a % 4 == 3) r.push_back(8);
+            for (int i = 0; i < a / 4; ++i) r.push_back(16);
+        }
+    } else if (b == 1 && c == 0 && a >= 2) {
+        P = 12;
+        r.push_back(3);
+        if (a % 2 == 1) r.push_back(2);
+        for (int i = 0; i < a / 2; ++i) r.push_back(4);
+    } else if (b == 0 && c == 1 && a >= 2) {
+        P = 20;
+        r.push_back(5);
         if (a % 2 == 1) r.push_back(2);
         for (int i = 0; i < a / 2; ++i) r.push_back(4);
     }
+    if (P == 0 || L / P > 1024) return 0;
+    if (rad) *rad = r;
+    return P;
 }
+
+int line_plan_reg_P(int L, int prec) { return reg_plan(L, prec, nullptr); }
 
 static void gen_radices(int L, std::vector<int>& r) {
     int a, b, c;
@@ -94,18 +113,18 @@ static void phase(long j, long n, double& c, double& s) {
     } else if (n % 2 == 0 && j == n / 2) { c = -1; s = 0; }
 }
 
-int line_plan_build(LinePlan* lp, int L, int prec, bool allow_reg) {
+int line_plan_build(LinePlan* lp, int L, int prec, bool allow_reg, int max_p) {
     MCS_CHECK(L >= 1, MCS_ERR_INVALID, "FFT length must be >= 1");
     MCS_CHECK(prec == 32 || prec == 64, MCS_ERR_INVALID, "precision must be 32 or 64");
     MCS_CHECK(generic_ok(L, nullptr), MCS_ERR_UNSUPPORTED, "FFT length must have no prime factor above 13");
     lp->L = L;
     lp->prec = prec;
     std::vector<int> rad;
-    const int P = allow_reg ? line_plan_reg_P(L) : 0;
+    const int P = allow_reg ? reg_plan(L, prec, nullptr, max_p) : 0;
     if (P) {
         lp->engine = ENGINE_REG;
         lp->P = P;
-        reg_radices(L, P, rad);
+        reg_plan(L, prec, &rad, max_p);
         const size_t e = prec == 64 ? 16 : 8;
         MCS_CHECK((size_t)L * e <= MCS_LDS_MAX, MCS_ERR_UNSUPPORTED, "FFT length too large for LDS");
     } else {

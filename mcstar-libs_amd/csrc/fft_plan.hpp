@@ -17,10 +17,12 @@ struct LinePlan {
 };
 
 // Builds the plan (and uploads twiddles).  Returns MCS_OK or an error code.
-int line_plan_build(LinePlan* lp, int L, int prec, bool allow_reg = true);
+// max_p caps the points per thread of power-of-two register plans (kernels holding several
+// register blocks, like the stepper's x pass, run better with fewer)
+int line_plan_build(LinePlan* lp, int L, int prec, bool allow_reg = true, int max_p = 32);
 void line_plan_free(LinePlan* lp);
 // register engine eligibility for L (without building)
-int line_plan_reg_P(int L);
+int line_plan_reg_P(int L, int prec = 32);
 // generic engine LDS bytes for G lines
 size_t line_plan_gen_lds(const LinePlan& lp, int G);
 int line_plan_gen_Lp(const LinePlan& lp);

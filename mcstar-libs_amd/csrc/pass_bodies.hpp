@@ -58,7 +58,7 @@ MCS_DEV void col_nl_body(C2<F>* W, const ColGeom& g, int G, const FftPlanDev& pl
 }
 
 // ------------------------------------------------------------------ row pass on packed rows
-template <typename F, int P, int DIR>
+template <typename F, int P, int DIR, bool WAVE = false>
 MCS_DEV void row_body(const C2<F>* src, C2<F>* dst, long nrows, int G, const FftPlanDev& pl,
                       const C2<F>* __restrict__ tw, C2<F>* lds, long blk) {
     const int L = pl.L, S = L / P;
@@ -68,7 +68,7 @@ MCS_DEV void row_body(const C2<F>* src, C2<F>* dst, long nrows, int G, const Fft
     C2<F> a[P];
 #pragma unroll
     for (int m = 0; m < P; ++m) a[m] = valid ? src[row * L + t + m * S] : mk<F>(0, 0);
-    reg_fft<F, P, DIR>(a, pl, tw, t, lds, (g < G ? g : 0) * L, 1);
+    reg_fft<F, P, DIR, WAVE>(a, pl, tw, t, lds, (g < G ? g : 0) * L, 1);
     if (valid) {
 #pragma unroll
         for (int m = 0; m < P; ++m) dst[row * L + t + m * S] = a[m];
@@ -134,7 +134,7 @@ MCS_DEV void xupd_pair(C2<F> nA, C2<F> nBr, C2<F> pA, C2<F> pBr, int p, int iy, 
 // Each thread updates the mode pairs {(c, p), (m, -p)} of its own positions p once; the
 // mirror row's results move to their owners through LDS.  A self-mirrored row (c == m)
 // is processed as row c alone.
-template <typename F, int P, int SPLIT>
+template <typename F, int P, int SPLIT, bool WAVE = false>
 MCS_DEV void xupd_body(C2<F>* __restrict__ W, C2<F>* __restrict__ U, const int2* __restrict__ pairs, int npairs,
                        int G, const XArgs<F>& ar, const FftPlanDev& pl, const C2<F>* __restrict__ tw,
                        const C2<F>* __restrict__ wph, uint32_t draw, C2<F>* lds, int blk) {
@@ -149,34 +149,42 @@ MCS_DEV void xupd_body(C2<F>* __restrict__ W, C2<F>* __restrict__ U, const int2*
     C2<F>* Nc = lds + base;
     C2<F>* Nm = Nc + L;
 
-    C2<F> a[P], b[P], pa[P], pb[P];
+    // two P-element arrays live at a time: the rows' spectra, then the state pairs updated in place
+    C2<F> a[P], b[P];
 #pragma unroll
     for (int m = 0; m < P; ++m) {
         const int p = t + m * S;
-        const int pp = p == 0 ? 0 : L - p;
         a[m] = valid ? W[rc * L + p] : mk<F>(0, 0);
         b[m] = valid ? W[rm * L + p] : mk<F>(0, 0);
-        pa[m] = valid ? U[rc * L + p] : mk<F>(0, 0);
-        pb[m] = valid ? U[rm * L + pp] : mk<F>(0, 0);
     }
-    reg_fft<F, P, -1>(a, pl, tw, t, lds, base, 1);
-    reg_fft<F, P, -1>(b, pl, tw, t, lds, base + L, 1);
+    reg_fft<F, P, -1, WAVE>(a, pl, tw, t, lds, base, 1);
+    reg_fft<F, P, -1, WAVE>(b, pl, tw, t, lds, base + L, 1);
 #pragma unroll
     for (int m = 0; m < P; ++m) {
         const int p = t + m * S;
         Nc[p] = a[m];
         Nm[p] = b[m];
     }
-    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < P; ++m) {
+        const int p = t + m * S;
+        const int pp = p == 0 ? 0 : L - p;
+        a[m] = valid ? U[rc * L + p] : mk<F>(0, 0);
+        b[m] = valid ? U[rm * L + pp] : mk<F>(0, 0);
+    }
+    lds_sync<WAVE>();
     const int iyc = (int)(rc % ar.ny), izc = (int)(rc / ar.ny);
     const int iym = (int)(rm % ar.ny), izm = (int)(rm / ar.ny);
 #pragma unroll
     for (int m = 0; m < P; ++m) {
         const int p = t + m * S;
         const int pp = p == 0 ? 0 : L - p;
-        xupd_pair<F>(Nc[p], Nm[pp], pa[m], pb[m], p, iyc, izc, iym, izm, wph[p], ar, draw, &a[m], &b[m]);
+        C2<F> uc, um;
+        xupd_pair<F>(Nc[p], Nm[pp], a[m], b[m], p, iyc, izc, iym, izm, wph[p], ar, draw, &uc, &um);
+        a[m] = uc;
+        b[m] = um;
     }
-    __syncthreads();
+    lds_sync<WAVE>();
     if (valid) {
 #pragma unroll
         for (int m = 0; m < P; ++m) {
@@ -189,12 +197,12 @@ MCS_DEV void xupd_body(C2<F>* __restrict__ W, C2<F>* __restrict__ U, const int2*
             }
         }
     }
-    __syncthreads();
+    lds_sync<WAVE>();
 #pragma unroll
     for (int m = 0; m < P; ++m) b[m] = self ? a[m] : Nm[t + m * S];
-    __syncthreads();
-    reg_fft<F, P, +1>(a, pl, tw, t, lds, base, 1);
-    reg_fft<F, P, +1>(b, pl, tw, t, lds, base + L, 1);
+    lds_sync<WAVE>();
+    reg_fft<F, P, +1, WAVE>(a, pl, tw, t, lds, base, 1);
+    reg_fft<F, P, +1, WAVE>(b, pl, tw, t, lds, base + L, 1);
     if (valid) {
 #pragma unroll
         for (int m = 0; m < P; ++m) {

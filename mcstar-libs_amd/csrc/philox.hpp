@@ -55,8 +55,10 @@ MCS_DEV C2<F> raw_xi(uint64_t h, uint32_t k0, uint32_t k1, uint32_t draw, F* z0)
 // (i1,i2) ~ (-i1,-i2) with the smaller i2*ny+i1 draws and its mirror takes the
 // conjugate; self-conjugate modes are real with unit variance (the same
 // statistics as cuda/noise.hpp:36-44, 82-89).
+// kept out of line: inlined into the fully unrolled update loops it would outgrow the unroller's budget
 template <typename F>
-MCS_DEV C2<F> noise_xi(int i0, int i1, int i2, int nx, int ny, int nz, uint32_t k0, uint32_t k1, uint32_t draw) {
+__device__ __noinline__ C2<F> noise_xi(int i0, int i1, int i2, int nx, int ny, int nz, uint32_t k0, uint32_t k1,
+                                       uint32_t draw) {
     const int nxc = nx / 2 + 1;
     F z0;
     if (i0 == 0 || ((nx & 1) == 0 && i0 == nx / 2)) {

@@ -27,14 +27,14 @@
 // ------------------------------------------------------------------ kernels (one work unit per workgroup;
 // every launch uses blockDim.x == G * S, the invariant the bodies rely on)
 template <typename F, int P, int DIR>
-__global__ void __launch_bounds__(1024) k_row_reg(const C2<F>* src, C2<F>* dst, long nrows, int G, FftPlanDev pl,
+__global__ void __launch_bounds__(MCS_REG_MAX_THREADS(P, F)) k_row_reg(const C2<F>* src, C2<F>* dst, long nrows, int G, FftPlanDev pl,
                                                   const C2<F>* __restrict__ tw) {
     extern __shared__ unsigned char smem_raw[];
     row_body<F, P, DIR>(src, dst, nrows, G, pl, tw, reinterpret_cast<C2<F>*>(smem_raw), blockIdx.x);
 }
 
 template <typename F, int P, int SPLIT>
-__global__ void __launch_bounds__(1024) k_col_nl(C2<F>* W, ColGeom g, int G, FftPlanDev pl,
+__global__ void __launch_bounds__(MCS_REG_MAX_THREADS(P, F)) k_col_nl(C2<F>* W, ColGeom g, int G, FftPlanDev pl,
                                                  const C2<F>* __restrict__ tw, F scale, int64_t* d_step) {
     extern __shared__ unsigned char smem_raw[];
     const int tilesx = (g.nxc + G - 1) / G;
@@ -43,15 +43,15 @@ __global__ void __launch_bounds__(1024) k_col_nl(C2<F>* W, ColGeom g, int G, Fft
     if (d_step && blockIdx.x == 0 && threadIdx.x == 0) *d_step += 1;
 }
 
-template <typename F, int P, int SPLIT>
+template <typename F, int P, int SPLIT, bool WAVE>
 __global__ void __launch_bounds__(256) k_xupd(C2<F>* __restrict__ W, C2<F>* __restrict__ U, const int2* __restrict__ pairs,
                                               int npairs, int G, XArgs<F> ar, FftPlanDev pl,
                                               const C2<F>* __restrict__ tw, const C2<F>* __restrict__ wph,
                                               const int64_t* __restrict__ d_step) {
     extern __shared__ unsigned char smem_raw[];
     const uint32_t draw = ar.noise ? (uint32_t)*d_step : 0u;
-    xupd_body<F, P, SPLIT>(W, U, pairs, npairs, G, ar, pl, tw, wph, draw, reinterpret_cast<C2<F>*>(smem_raw),
-                           blockIdx.x);
+    xupd_body<F, P, SPLIT, WAVE>(W, U, pairs, npairs, G, ar, pl, tw, wph, draw,
+                                 reinterpret_cast<C2<F>*>(smem_raw), blockIdx.x);
 }
 
 // ------------------------------------------------------------------ helpers
@@ -102,6 +102,8 @@ static int launch_row_reg_d(const LinePlan& lp, const void* src, void* dst, long
         case 8: return launch_row_reg_t<F, 8, DIR>(lp, src, dst, nrows, s);
         case 12: return launch_row_reg_t<F, 12, DIR>(lp, src, dst, nrows, s);
         case 20: return launch_row_reg_t<F, 20, DIR>(lp, src, dst, nrows, s);
+        case 16: return launch_row_reg_t<F, 16, DIR>(lp, src, dst, nrows, s);
+        case 32: return launch_row_reg_t<F, 32, DIR>(lp, src, dst, nrows, s);
         default: mcs_set_last_error(__FILE__, __LINE__, "row plan not register-eligible"); return MCS_ERR_UNSUPPORTED;
     }
 }
@@ -137,8 +139,17 @@ static int launch_col_nl_d(const LinePlan& lp, void* W, const ColGeom& g, double
         case 8: return launch_col_nl_t<F, 8, SPLIT>(lp, W, g, scale, d_step, s);
         case 12: return launch_col_nl_t<F, 12, SPLIT>(lp, W, g, scale, d_step, s);
         case 20: return launch_col_nl_t<F, 20, SPLIT>(lp, W, g, scale, d_step, s);
+        case 16: return launch_col_nl_t<F, 16, SPLIT>(lp, W, g, scale, d_step, s);
+        case 32: return launch_col_nl_t<F, 32, SPLIT>(lp, W, g, scale, d_step, s);
         default: mcs_set_last_error(__FILE__, __LINE__, "column plan not register-eligible"); return MCS_ERR_UNSUPPORTED;
     }
+}
+
+// a column pass of the stepper on W in place: kind -1 forward, +1 inverse, 0 nonlinear
+int launch_step_col(const LinePlan& lp, void* W, const ColGeom& g, int kind, double scale, int split,
+                    int64_t* d_step, hipStream_t s) {
+    if (kind == 0) return launch_col_nl(lp, W, g, scale, split, d_step, s);
+    return launch_col(lp, W, W, g, kind, s);
 }
 
 int launch_col_nl(const LinePlan& lp, void* W, const ColGeom& g, double scale, int split, int64_t* d_step,
@@ -157,7 +168,7 @@ static int launch_xupd_t(const StepGeom& sg, void* W, void* U, const XArgs<F>& a
     const size_t esz = sizeof(C2<F>);
     const int G = xupd_pairs_per_wg(lp);
     const size_t lds = (size_t)G * 2 * lp.L * esz;
-    auto fn = k_xupd<F, P, SPLIT>;
+    auto fn = (64 % S == 0) ? k_xupd<F, P, SPLIT, true> : k_xupd<F, P, SPLIT, false>;
     int st = set_lds_attr((const void*)fn, lds);
     if (st) return st;
     hipLaunchKernelGGL(fn, dim3((unsigned)((sg.npairs + G - 1) / G)), dim3(G * S), lds, s, (C2<F>*)W, (C2<F>*)U,
@@ -173,6 +184,8 @@ static int launch_xupd_d(const StepGeom& sg, void* W, void* U, const XArgs<F>& a
         case 8: return launch_xupd_t<F, 8, SPLIT>(sg, W, U, ar, s);
         case 12: return launch_xupd_t<F, 12, SPLIT>(sg, W, U, ar, s);
         case 20: return launch_xupd_t<F, 20, SPLIT>(sg, W, U, ar, s);
+        case 16: return launch_xupd_t<F, 16, SPLIT>(sg, W, U, ar, s);
+        case 32: return launch_xupd_t<F, 32, SPLIT>(sg, W, U, ar, s);
         default: mcs_set_last_error(__FILE__, __LINE__, "x plan not register-eligible"); return MCS_ERR_UNSUPPORTED;
     }
 }
@@ -197,10 +210,11 @@ int step_geom_build(StepGeom* sg, const int ext[3], int prec) {
     sg->L = ext[0] / 2;
     sg->prec = prec;
     sg->is3d = ext[2] > 1;
-    MCS_CHECK(line_plan_reg_P(sg->L) && line_plan_reg_P(sg->ny) && (!sg->is3d || line_plan_reg_P(sg->nz)),
+    MCS_CHECK(line_plan_reg_P(sg->L, prec) && line_plan_reg_P(sg->ny, prec) &&
+                  (!sg->is3d || line_plan_reg_P(sg->nz, prec)),
               MCS_ERR_UNSUPPORTED, "stepper axis lengths must be 2^a, 3*2^a or 5*2^a (nx/2, ny, nz >= 4)");
-    MCS_CHECK(sg->L / line_plan_reg_P(sg->L) <= 256, MCS_ERR_UNSUPPORTED, "nx/2 too long for the x pass");
-    int st = line_plan_build(&sg->px, sg->L, prec);
+    MCS_CHECK(sg->L <= 2048, MCS_ERR_UNSUPPORTED, "nx/2 too long for the x pass");
+    int st = line_plan_build(&sg->px, sg->L, prec, true, 8);
     if (!st) st = line_plan_build(&sg->py, sg->ny, prec);
     if (!st && sg->is3d) st = line_plan_build(&sg->pz, sg->nz, prec);
     if (!st) st = make_phase_table(&sg->wph, sg->nx, sg->L, prec);
@@ -241,7 +255,7 @@ int col_lines_per_wg(const LinePlan& lp) {
     const int S = lp.L / lp.P;
     const size_t esz = lp.prec == 64 ? 16 : 8;
     int G = (int)(128 / esz);
-    if (G > 1024 / S) G = 1024 / S;
+    if (G > mcs_reg_max_threads(lp.P, lp.prec) / S) G = mcs_reg_max_threads(lp.P, lp.prec) / S;
     while (G > 1 && (size_t)G * lp.L * esz > 65536) --G;
     return G < 1 ? 1 : G;
 }
@@ -259,11 +273,11 @@ int step_nonlinear_half(const StepGeom& sg, const StepPhys& ph, void* W, int64_t
     const double scale = 2.0 / ((double)sg.nx * sg.ny * sg.nz);
     int st;
     if (sg.is3d) {
-        st = launch_col(sg.py, W, W, geom_y(sg), +1, s);
-        if (!st) st = launch_col_nl(sg.pz, W, geom_z(sg), scale, ph.split, d_step, s);
-        if (!st) st = launch_col(sg.py, W, W, geom_y(sg), -1, s);
+        st = launch_step_col(sg.py, W, geom_y(sg), +1, scale, ph.split, nullptr, s);
+        if (!st) st = launch_step_col(sg.pz, W, geom_z(sg), 0, scale, ph.split, d_step, s);
+        if (!st) st = launch_step_col(sg.py, W, geom_y(sg), -1, scale, ph.split, nullptr, s);
     } else {
-        st = launch_col_nl(sg.py, W, geom_y(sg), scale, ph.split, d_step, s);
+        st = launch_step_col(sg.py, W, geom_y(sg), 0, scale, ph.split, d_step, s);
     }
     return st;
 }
@@ -283,14 +297,14 @@ int step_run_timed(const StepGeom& sg, const StepPhys& ph, void* W, void* U, int
         st = launch_xupd(sg, ph, W, U, s);
         (void)hipEventRecord(ev[1], s);
         if (sg.is3d) {
-            if (!st) st = launch_col(sg.py, W, W, geom_y(sg), +1, s);
+            if (!st) st = launch_step_col(sg.py, W, geom_y(sg), +1, scale, ph.split, nullptr, s);
             (void)hipEventRecord(ev[2], s);
-            if (!st) st = launch_col_nl(sg.pz, W, geom_z(sg), scale, ph.split, sg.d_step, s);
+            if (!st) st = launch_step_col(sg.pz, W, geom_z(sg), 0, scale, ph.split, sg.d_step, s);
             (void)hipEventRecord(ev[3], s);
-            if (!st) st = launch_col(sg.py, W, W, geom_y(sg), -1, s);
+            if (!st) st = launch_step_col(sg.py, W, geom_y(sg), -1, scale, ph.split, nullptr, s);
             (void)hipEventRecord(ev[4], s);
         } else {
-            if (!st) st = launch_col_nl(sg.py, W, geom_y(sg), scale, ph.split, sg.d_step, s);
+            if (!st) st = launch_step_col(sg.py, W, geom_y(sg), 0, scale, ph.split, sg.d_step, s);
             (void)hipEventRecord(ev[2], s);
         }
         if (!st && hipEventSynchronize(ev[np]) != hipSuccess) st = MCS_ERR_HIP;
