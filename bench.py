@@ -71,6 +71,22 @@ def cpu_baseline(n, cfg, steps):
                       f"minus a 0-step run for the transforms in and out; {per_step:.2f} s/step"}
 
 
+def measured_traffic(pass_name, n):
+    """HBM bytes per launch of a pass from the committed rocprofv3 --pmc summary (512^3 FP32 only)"""
+    if n != 512:
+        return None
+    kern = {"x_update": "`k_xupd<", "z_nonlinear": "`k_col_nl<", "y_inverse": "`k_col_reg<float, 32, 1>",
+            "y_forward": "`k_col_reg<float, 32, -1>"}[pass_name]
+    path = os.path.join(REPO, "profiles", "pfc_step_512cubed_fp32_hbm_traffic.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        cols = [c.strip() for c in line.split("|")]
+        if len(cols) > 6 and cols[1].startswith(kern):
+            return float(cols[5].split()[0]) * 1e9
+    return None
+
+
 def ensemble_bench(mcs, kind, steps):
     """replica-steps/s of the batched, graph-replayed ensemble against looping single-replica launches"""
     reps = mcs.ensemble_config(kind)
@@ -101,6 +117,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--only-ensemble", action="store_true", help="run only the ensemble lines (profiling)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a 1 GiB device-to-device copy (reference dispatch for HBM counters)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (the PFC step runs on a single "
@@ -108,6 +127,14 @@ def main():
         sys.exit(2)
 
     mcs = load_pkg()
+    if a.only_ensemble:
+        print(json.dumps({"ensemble_2d": ensemble_bench(mcs, "2d", 40), "ensemble_3d": ensemble_bench(mcs, "3d", 16)}))
+        return
+    if a.copy_calibration:
+        nb = 1 << 30
+        src, dst = mcs.DeviceBuffer(nb).zero(), mcs.DeviceBuffer(nb)
+        mcs.check(mcs.lib().mcs_memcpy(dst.ptr, src.ptr, nb, mcs.D2D), "mcs_memcpy")
+        del src, dst
     n = a.n
     cfg = dict(eps=0.3, psi0=-0.25, amp=0.05, seed=11, dx=np.pi / 4, dt=0.5)
     N = n ** 3
@@ -128,8 +155,11 @@ def main():
     pass_bytes = {"x_update": 4 * sweep, "y_inverse": 2 * sweep, "z_nonlinear": 2 * sweep, "y_forward": 2 * sweep}
     dom = max(passes, key=passes.get)
     achieved = pass_bytes[dom] / (passes[dom] / 1e3) / 1e9
+    traffic = measured_traffic(dom, n)
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "traffic_source": "profiles/pfc_step_512cubed_fp32_hbm_traffic.md (2 x FETCH_SIZE + WRITE_SIZE)"
+                if traffic else None,
                 "bytes_per_launch": pass_bytes[dom], "launch_ms": round(passes[dom], 5)}
     # step-level model: 11 Cb + 2 Rb bytes per 3D step (half-spectrum FFT passes with fused pointwise work)
     cb = (n // 2 + 1) * n * n * 8
@@ -151,15 +181,19 @@ def main():
         st64.step(a.warmup)
         k64 = min(a.steps, 50)
         ms64 = timed_steps(mcs, st64, k64)
-        st64.step(steps32 - a.warmup - k64)
+        # precision study at 500 steps (or more, if the timed run went further)
+        n_cmp = max(steps32, 500)
+        st.step(n_cmp - steps32)
+        st64.step(n_cmp - a.warmup - k64)
         mcs.sync()
+        assert st.steps_taken() == st64.steps_taken() == n_cmp, (st.steps_taken(), st64.steps_taken(), n_cmp)
         v64 = N * k64 / (ms64 / 1e3)
         p32 = st.get_psi().astype(np.float64)
         p64 = st64.get_psi()
         rel = float(np.linalg.norm(p32 - p64) / np.linalg.norm(p64))
         out["fp64"] = {"value": v64, "ms_per_step": ms64 / k64, "speedup_fp32_over_fp64": value / v64,
                        "rel_l2_fp32_vs_fp64_gpu": rel, "max_abs_fp32_vs_fp64_gpu": float(np.abs(p32 - p64).max()),
-                       "steps_compared": steps32}
+                       "steps_compared": n_cmp}
         del st64, p32, p64
     if not a.no_ensemble:
         log("[bench] ensembles")

@@ -65,8 +65,12 @@ int mcs_memset(void* dst, int value, size_t bytes);
  * FFT::CufftPlan<F, R2C|C2R> (cuda/cufft_wrapper.hpp:61-86).  Rank 1/2/3
  * from the extents (72-74), unnormalised both ways, out-of-place.  The C2R
  * input is left unmodified (the plan owns a scratch buffer).  Axis lengths
- * must be 2,3,5-smooth (cuda/math.hpp:125-149), at most 16384 (f32) / 8192
- * (f64).  C2R of a non-Hermitian spectrum follows numpy.fft.irfftn: the
+ * may have prime factors 2, 3, 5, 7, 11 and 13 (cuda/math.hpp:125-149 rounds
+ * sizes to 2,3,5-smooth ones).  Lengths 2^a, 3*2^a and 5*2^a run in
+ * registers up to 16384 points (f32) / 8192 (f64, powers of two); other
+ * lengths need two copies of the line in 160 KB of LDS (at most 10239
+ * points in f32, 5119 in f64).  Larger lengths return MCS_ERR_UNSUPPORTED.
+ * C2R of a non-Hermitian spectrum follows numpy.fft.irfftn: the
  * imaginary parts of the x = 0 and x = nx/2 entries are ignored. */
 #define MCS_R2C 0
 #define MCS_C2R 1
