@@ -179,7 +179,8 @@ typedef struct {
 typedef struct mcs_ensemble mcs_ensemble;
 #define MCS_ENS_LOOP 0    /* replica by replica, single-grid launches (the looping baseline) */
 #define MCS_ENS_BATCHED 1 /* one launch per pass and size group */
-#define MCS_ENS_GRAPH 2   /* batched, replayed from a captured HIP graph */
+#define MCS_ENS_GRAPH 2   /* replayed from a captured HIP graph of 8 steps in which every size group is an
+                             independent branch (the groups' kernels overlap); remainder steps run batched */
 int mcs_ensemble_create(mcs_ensemble** e, const mcs_replica* reps, int nrep, int split, int prec);
 /* seeded initial fields for every replica */
 int mcs_ensemble_init(mcs_ensemble* e);

@@ -5,10 +5,13 @@
 // its own eps, dt, noise amplitude, seed and FP64 mass).  One step of the
 // ensemble is one launch per pass and group: persistent workgroups walk the
 // (replica, work unit) table with the same pass bodies as the single-grid
-// stepper.  All x passes run before the first column pass, and the shared
-// Philox draw counter is advanced by the last nonlinear pass of the step.
-// MCS_ENS_LOOP issues the same passes replica by replica (the looping
-// baseline); MCS_ENS_GRAPH replays a captured graph of several steps.
+// stepper.  The x pass of step j of a call draws noise number d_step + j,
+// and one single-thread kernel advances the device counter d_step at the end
+// of the call, so the passes of different groups need no ordering among
+// themselves.  MCS_ENS_LOOP issues the same passes replica by replica (the
+// looping baseline); MCS_ENS_GRAPH replays a captured graph of several steps
+// in which every size group is an independent branch, so that the groups'
+// kernels overlap on the GPU.
 #include "peaks.hpp"
 #include "stepper.hpp"
 
@@ -27,15 +30,15 @@ template <typename F, int P, int SPLIT, bool WAVE>
 __global__ void __launch_bounds__(256) k_ens_xupd(const EnsRep<F>* __restrict__ reps, int nr, int units,
                                                   const int2* __restrict__ pairs, int npairs, int G, FftPlanDev pl,
                                                   const C2<F>* __restrict__ tw, const C2<F>* __restrict__ wph,
-                                                  const int64_t* __restrict__ d_step) {
+                                                  const int64_t* __restrict__ d_step, int draw_off) {
     extern __shared__ unsigned char smem_raw[];
     C2<F>* lds = reinterpret_cast<C2<F>*>(smem_raw);
-    const uint32_t draw = (uint32_t)*d_step;
+    const uint32_t draw = (uint32_t)(*d_step + draw_off);
     const long total = (long)nr * units;
     for (long it = blockIdx.x; it < total; it += gridDim.x) {
         const int r = (int)(it / units), u = (int)(it % units);
         const EnsRep<F>& R = reps[r];
-        xupd_body<F, P, SPLIT, WAVE>(R.W, R.U, pairs, npairs, G, R.xa, pl, tw, wph, draw, lds, u);
+        xupd_body<F, P, SPLIT, WAVE>(R.W, R.U, pairs, npairs, G, R.xa, pl, tw, wph, draw, lds, u, opaque_tid());
         __syncthreads();
     }
 }
@@ -52,15 +55,14 @@ __global__ void __launch_bounds__(MCS_REG_MAX_THREADS(P, F)) k_ens_col(const Ens
         const int r = (int)(it / units);
         const long u = it % units;
         C2<F>* W = reps[r].W;
-        col_body<F, P, DIR>(W, W, g, G, pl, tw, lds, (int)(u % tilesx), (int)(u / tilesx));
+        col_body<F, P, DIR>(W, W, g, G, pl, tw, lds, (int)(u % tilesx), (int)(u / tilesx), opaque_tid());
         __syncthreads();
     }
 }
 
 template <typename F, int P, int SPLIT>
 __global__ void __launch_bounds__(MCS_REG_MAX_THREADS(P, F)) k_ens_col_nl(const EnsRep<F>* __restrict__ reps, int nr, ColGeom g, int G,
-                                                     FftPlanDev pl, const C2<F>* __restrict__ tw, F scale,
-                                                     int64_t* d_step) {
+                                                     FftPlanDev pl, const C2<F>* __restrict__ tw, F scale) {
     extern __shared__ unsigned char smem_raw[];
     C2<F>* lds = reinterpret_cast<C2<F>*>(smem_raw);
     const int tilesx = (g.nxc + G - 1) / G;
@@ -69,10 +71,14 @@ __global__ void __launch_bounds__(MCS_REG_MAX_THREADS(P, F)) k_ens_col_nl(const 
     for (long it = blockIdx.x; it < total; it += gridDim.x) {
         const int r = (int)(it / units);
         const long u = it % units;
-        col_nl_body<F, P, SPLIT>(reps[r].W, g, G, pl, tw, scale, lds, (int)(u % tilesx), (int)(u / tilesx));
+        col_nl_body<F, P, SPLIT>(reps[r].W, g, G, pl, tw, scale, lds, (int)(u % tilesx), (int)(u / tilesx),
+                                 opaque_tid());
         __syncthreads();
     }
-    if (d_step && blockIdx.x == 0 && threadIdx.x == 0) *d_step += 1;
+}
+
+__global__ void k_ens_advance(int64_t* d_step, int n) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *d_step += n;
 }
 
 // ------------------------------------------------------------------ host side
@@ -95,6 +101,8 @@ struct mcs_ensemble {
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
     int gsteps = 0;
+    std::vector<hipStream_t> side;     // one capture branch per size group
+    std::vector<hipEvent_t> side_ev;   // [0] fork, [1 + gi] join of group gi
     int cus = 0;
 };
 
@@ -138,7 +146,7 @@ static unsigned persistent_grid(mcs_ensemble* e, int threads, size_t lds, long t
 
 // one x pass over members [first, first + nr) of group g
 template <typename F, int P, int SPLIT>
-static int ens_xupd_t(mcs_ensemble* e, EnsGroup* g, int first, int nr, hipStream_t s) {
+static int ens_xupd_t(mcs_ensemble* e, EnsGroup* g, int first, int nr, int draw_off, hipStream_t s) {
     const LinePlan& lp = g->sg.px;
     const int G = xupd_pairs_per_wg(lp);
     const int S = lp.L / P;
@@ -148,7 +156,7 @@ static int ens_xupd_t(mcs_ensemble* e, EnsGroup* g, int first, int nr, hipStream
     const unsigned grid = persistent_grid(e, G * S, lds, (long)nr * units);
     hipLaunchKernelGGL(fn, dim3(grid), dim3(G * S), lds, s, (const EnsRep<F>*)g->d_reps + first, nr, units,
                        g->sg.pairs, g->sg.npairs, G, lp.dev, (const C2<F>*)lp.tw, (const C2<F>*)g->sg.wph,
-                       (const int64_t*)e->d_step);
+                       (const int64_t*)e->d_step, draw_off);
     MCS_HIP_TRY(hipGetLastError());
     return MCS_OK;
 }
@@ -170,7 +178,7 @@ static int ens_col_t(mcs_ensemble* e, EnsGroup* g, const LinePlan& lp, const Col
 
 template <typename F, int P, int SPLIT>
 static int ens_col_nl_t(mcs_ensemble* e, EnsGroup* g, const LinePlan& lp, const ColGeom& cg, int first, int nr,
-                        bool inc, hipStream_t s) {
+                        hipStream_t s) {
     const int G = col_lines_per_wg(lp);
     const int S = lp.L / P;
     const size_t lds = (size_t)G * lp.L * sizeof(C2<F>);
@@ -179,7 +187,7 @@ static int ens_col_nl_t(mcs_ensemble* e, EnsGroup* g, const LinePlan& lp, const 
     const unsigned grid = persistent_grid(e, G * S, lds, total);
     const double scale = 2.0 / ((double)g->sg.nx * g->sg.ny * g->sg.nz);
     hipLaunchKernelGGL(fn, dim3(grid), dim3(G * S), lds, s, (const EnsRep<F>*)g->d_reps + first, nr, cg, G, lp.dev,
-                       (const C2<F>*)lp.tw, (F)scale, inc ? e->d_step : nullptr);
+                       (const C2<F>*)lp.tw, (F)scale);
     MCS_HIP_TRY(hipGetLastError());
     return MCS_OK;
 }
@@ -196,9 +204,9 @@ static int ens_col_nl_t(mcs_ensemble* e, EnsGroup* g, const LinePlan& lp, const 
     }
 
 template <typename F>
-static int ens_xupd(mcs_ensemble* e, EnsGroup* g, int first, int nr, hipStream_t s) {
-    if (e->split == MCS_SPLIT_REFERENCE) { MCS_P_DISPATCH(g->sg.px.P, (ens_xupd_t<F, PP, MCS_SPLIT_REFERENCE>(e, g, first, nr, s))) }
-    MCS_P_DISPATCH(g->sg.px.P, (ens_xupd_t<F, PP, MCS_SPLIT_NORTHSTAR>(e, g, first, nr, s)))
+static int ens_xupd(mcs_ensemble* e, EnsGroup* g, int first, int nr, int draw_off, hipStream_t s) {
+    if (e->split == MCS_SPLIT_REFERENCE) { MCS_P_DISPATCH(g->sg.px.P, (ens_xupd_t<F, PP, MCS_SPLIT_REFERENCE>(e, g, first, nr, draw_off, s))) }
+    MCS_P_DISPATCH(g->sg.px.P, (ens_xupd_t<F, PP, MCS_SPLIT_NORTHSTAR>(e, g, first, nr, draw_off, s)))
 }
 
 template <typename F>
@@ -210,58 +218,89 @@ static int ens_col(mcs_ensemble* e, EnsGroup* g, const LinePlan& lp, const ColGe
 
 template <typename F>
 static int ens_col_nl(mcs_ensemble* e, EnsGroup* g, const LinePlan& lp, const ColGeom& cg, int first, int nr,
-                      bool inc, hipStream_t s) {
-    if (e->split == MCS_SPLIT_REFERENCE) { MCS_P_DISPATCH(lp.P, (ens_col_nl_t<F, PP, MCS_SPLIT_REFERENCE>(e, g, lp, cg, first, nr, inc, s))) }
-    MCS_P_DISPATCH(lp.P, (ens_col_nl_t<F, PP, MCS_SPLIT_NORTHSTAR>(e, g, lp, cg, first, nr, inc, s)))
+                      hipStream_t s) {
+    if (e->split == MCS_SPLIT_REFERENCE) { MCS_P_DISPATCH(lp.P, (ens_col_nl_t<F, PP, MCS_SPLIT_REFERENCE>(e, g, lp, cg, first, nr, s))) }
+    MCS_P_DISPATCH(lp.P, (ens_col_nl_t<F, PP, MCS_SPLIT_NORTHSTAR>(e, g, lp, cg, first, nr, s)))
 }
 
-// the passes after the x pass for members [first, first+nr) of g
+// one step of members [first, first+nr) of g, drawing noise number d_step + draw_off
 template <typename F>
-static int ens_rest(mcs_ensemble* e, EnsGroup* g, int first, int nr, bool inc, hipStream_t s) {
+static int ens_step_group(mcs_ensemble* e, EnsGroup* g, int first, int nr, int draw_off, hipStream_t s) {
     const StepGeom& sg = g->sg;
-    int st;
+    int st = ens_xupd<F>(e, g, first, nr, draw_off, s);
+    if (st) return st;
     if (sg.is3d) {
         st = ens_col<F>(e, g, sg.py, step_geom_y(sg), +1, first, nr, s);
-        if (!st) st = ens_col_nl<F>(e, g, sg.pz, step_geom_z(sg), first, nr, inc, s);
+        if (!st) st = ens_col_nl<F>(e, g, sg.pz, step_geom_z(sg), first, nr, s);
         if (!st) st = ens_col<F>(e, g, sg.py, step_geom_y(sg), -1, first, nr, s);
     } else {
-        st = ens_col_nl<F>(e, g, sg.py, step_geom_y(sg), first, nr, inc, s);
+        st = ens_col_nl<F>(e, g, sg.py, step_geom_y(sg), first, nr, s);
     }
     return st;
 }
 
+static int ens_advance(mcs_ensemble* e, int nsteps, hipStream_t s) {
+    hipLaunchKernelGGL(k_ens_advance, dim3(1), dim3(64), 0, s, e->d_step, nsteps);
+    MCS_HIP_TRY(hipGetLastError());
+    return MCS_OK;
+}
+
+// nsteps steps in one stream: replica by replica (LOOP) or one launch per pass and group (BATCHED)
 template <typename F> static int ens_steps_t(mcs_ensemble* e, int nsteps, int mode, hipStream_t s) {
     const int ng = (int)e->groups.size();
-    for (int it = 0; it < nsteps; ++it) {
-        int st = MCS_OK;
+    int st = MCS_OK;
+    for (int it = 0; it < nsteps && !st; ++it) {
         if (mode == MCS_ENS_LOOP) {
-            const int nrep = (int)e->reps.size();
-            for (int r = 0; r < nrep && !st; ++r) {
-                EnsGroup* g = e->groups[e->group_of[r]];
-                st = ens_xupd<F>(e, g, e->slot_of[r], 1, s);
-                if (!st) st = ens_rest<F>(e, g, e->slot_of[r], 1, r == nrep - 1, s);
-            }
+            for (int r = 0; r < (int)e->reps.size() && !st; ++r)
+                st = ens_step_group<F>(e, e->groups[e->group_of[r]], e->slot_of[r], 1, it, s);
         } else {
             for (int gi = 0; gi < ng && !st; ++gi)
-                st = ens_xupd<F>(e, e->groups[gi], 0, (int)e->groups[gi]->members.size(), s);
-            for (int gi = 0; gi < ng && !st; ++gi)
-                st = ens_rest<F>(e, e->groups[gi], 0, (int)e->groups[gi]->members.size(), gi == ng - 1, s);
+                st = ens_step_group<F>(e, e->groups[gi], 0, (int)e->groups[gi]->members.size(), it, s);
         }
-        if (st) return st;
     }
-    return MCS_OK;
+    if (!st && nsteps > 0) st = ens_advance(e, nsteps, s);
+    return st;
 }
 
 static int ens_steps(mcs_ensemble* e, int nsteps, int mode, hipStream_t s) {
     return e->prec == 64 ? ens_steps_t<double>(e, nsteps, mode, s) : ens_steps_t<float>(e, nsteps, mode, s);
 }
 
+// GRAPH_STEPS steps with every size group on its own branch: fork from s, each
+// group runs its steps in order, join, then advance the draw counter
+template <typename F> static int ens_branches_t(mcs_ensemble* e, int nsteps, hipStream_t s) {
+    const int ng = (int)e->groups.size();
+    int st = MCS_OK;
+    if (hipEventRecord(e->side_ev[0], s) != hipSuccess) return MCS_ERR_HIP;
+    for (int gi = 0; gi < ng && !st; ++gi) {
+        hipStream_t b = e->side[gi];
+        if (hipStreamWaitEvent(b, e->side_ev[0], 0) != hipSuccess) return MCS_ERR_HIP;
+        EnsGroup* g = e->groups[gi];
+        for (int it = 0; it < nsteps && !st; ++it) st = ens_step_group<F>(e, g, 0, (int)g->members.size(), it, b);
+        if (!st && hipEventRecord(e->side_ev[1 + gi], b) != hipSuccess) st = MCS_ERR_HIP;
+        if (!st && hipStreamWaitEvent(s, e->side_ev[1 + gi], 0) != hipSuccess) st = MCS_ERR_HIP;
+    }
+    if (!st) st = ens_advance(e, nsteps, s);
+    return st;
+}
+
 static constexpr int GRAPH_STEPS = 8;
 
 static int ens_graph_build(mcs_ensemble* e, hipStream_t s) {
     if (e->gexec) return MCS_OK;
+    const int ng = (int)e->groups.size();
+    while ((int)e->side.size() < ng) {
+        hipStream_t b = nullptr;
+        MCS_HIP_TRY(hipStreamCreateWithFlags(&b, hipStreamNonBlocking));
+        e->side.push_back(b);
+    }
+    while ((int)e->side_ev.size() < ng + 1) {
+        hipEvent_t ev = nullptr;
+        MCS_HIP_TRY(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        e->side_ev.push_back(ev);
+    }
     MCS_HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    int st = ens_steps(e, GRAPH_STEPS, MCS_ENS_BATCHED, s);
+    int st = e->prec == 64 ? ens_branches_t<double>(e, GRAPH_STEPS, s) : ens_branches_t<float>(e, GRAPH_STEPS, s);
     hipGraph_t g = nullptr;
     const hipError_t ce = hipStreamEndCapture(s, &g);
     if (st) { if (g) (void)hipGraphDestroy(g); return st; }
@@ -277,6 +316,8 @@ extern "C" int mcs_ensemble_destroy(mcs_ensemble* e) {
     if (!e) return MCS_OK;
     if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
     if (e->graph) (void)hipGraphDestroy(e->graph);
+    for (hipStream_t b : e->side) (void)hipStreamDestroy(b);
+    for (hipEvent_t ev : e->side_ev) (void)hipEventDestroy(ev);
     for (EnsGroup* g : e->groups) {
         step_geom_free(&g->sg);
         if (g->d_reps) (void)hipFree(g->d_reps);

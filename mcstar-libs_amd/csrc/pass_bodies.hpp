@@ -9,13 +9,24 @@
 #include "pfc_math.hpp"
 #include "philox.hpp"
 
+// The bodies take the thread index as an argument.  Persistent callers pass
+// opaque_tid() on every iteration: with threadIdx.x itself, the compiler hoists
+// all address arithmetic derived from it (P offsets per access, LDS and twiddle
+// indices) out of the unit loop and keeps it live across the whole body, which
+// doubled the register use of the loop kernels and spilled the P = 12 ones.
+MCS_DEV int opaque_tid() {
+    int t = (int)threadIdx.x;
+    asm volatile("" : "+v"(t));
+    return t;
+}
+
 // ------------------------------------------------------------------ column pass
 // tile of G adjacent x columns (thread c = tid % G, t = tid / G), L = pl.L points each
 template <typename F, int P, int DIR>
 MCS_DEV void col_body(const C2<F>* src, C2<F>* dst, const ColGeom& g, int G, const FftPlanDev& pl,
-                      const C2<F>* __restrict__ tw, C2<F>* lds, int tile, int outer) {
+                      const C2<F>* __restrict__ tw, C2<F>* lds, int tile, int outer, int tid) {
     const int S = pl.L / P;
-    const int c = threadIdx.x % G, t = threadIdx.x / G;
+    const int c = tid % G, t = tid / G;
     const int x = tile * G + c;
     const bool valid = x < g.nxc && t < S;
     const long base = (long)outer * g.os + x;
@@ -33,9 +44,9 @@ MCS_DEV void col_body(const C2<F>* src, C2<F>* dst, const ColGeom& g, int G, con
 // u = scale * value holds psi at (2m, 2m+1); N = psi^3 (or psi^3 - psi)
 template <typename F, int P, int SPLIT>
 MCS_DEV void col_nl_body(C2<F>* W, const ColGeom& g, int G, const FftPlanDev& pl, const C2<F>* __restrict__ tw,
-                         F scale, C2<F>* lds, int tile, int outer) {
+                         F scale, C2<F>* lds, int tile, int outer, int tid) {
     const int S = pl.L / P;
-    const int c = threadIdx.x % G, t = threadIdx.x / G;
+    const int c = tid % G, t = tid / G;
     const int x = tile * G + c;
     const bool valid = x < g.nxc && t < S;
     const int tt = t < S ? t : 0;
@@ -60,9 +71,9 @@ MCS_DEV void col_nl_body(C2<F>* W, const ColGeom& g, int G, const FftPlanDev& pl
 // ------------------------------------------------------------------ row pass on packed rows
 template <typename F, int P, int DIR, bool WAVE = false>
 MCS_DEV void row_body(const C2<F>* src, C2<F>* dst, long nrows, int G, const FftPlanDev& pl,
-                      const C2<F>* __restrict__ tw, C2<F>* lds, long blk) {
+                      const C2<F>* __restrict__ tw, C2<F>* lds, long blk, int tid) {
     const int L = pl.L, S = L / P;
-    const int t = threadIdx.x % S, g = threadIdx.x / S;
+    const int t = tid % S, g = tid / S;
     const long row = blk * G + g;
     const bool valid = row < nrows && g < G;
     C2<F> a[P];
@@ -137,9 +148,9 @@ MCS_DEV void xupd_pair(C2<F> nA, C2<F> nBr, C2<F> pA, C2<F> pBr, int p, int iy, 
 template <typename F, int P, int SPLIT, bool WAVE = false>
 MCS_DEV void xupd_body(C2<F>* __restrict__ W, C2<F>* __restrict__ U, const int2* __restrict__ pairs, int npairs,
                        int G, const XArgs<F>& ar, const FftPlanDev& pl, const C2<F>* __restrict__ tw,
-                       const C2<F>* __restrict__ wph, uint32_t draw, C2<F>* lds, int blk) {
+                       const C2<F>* __restrict__ wph, uint32_t draw, C2<F>* lds, int blk, int tid) {
     const int L = pl.L, S = L / P;
-    const int t = threadIdx.x % S, g = threadIdx.x / S;
+    const int t = tid % S, g = tid / S;
     const int q = blk * G + g;
     const bool valid = q < npairs && g < G;
     const int2 pr = pairs[valid ? q : 0];

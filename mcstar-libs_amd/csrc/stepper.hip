@@ -30,7 +30,7 @@ template <typename F, int P, int DIR>
 __global__ void __launch_bounds__(MCS_REG_MAX_THREADS(P, F)) k_row_reg(const C2<F>* src, C2<F>* dst, long nrows, int G, FftPlanDev pl,
                                                   const C2<F>* __restrict__ tw) {
     extern __shared__ unsigned char smem_raw[];
-    row_body<F, P, DIR>(src, dst, nrows, G, pl, tw, reinterpret_cast<C2<F>*>(smem_raw), blockIdx.x);
+    row_body<F, P, DIR>(src, dst, nrows, G, pl, tw, reinterpret_cast<C2<F>*>(smem_raw), blockIdx.x, threadIdx.x);
 }
 
 template <typename F, int P, int SPLIT>
@@ -39,7 +39,7 @@ __global__ void __launch_bounds__(MCS_REG_MAX_THREADS(P, F)) k_col_nl(C2<F>* W, 
     extern __shared__ unsigned char smem_raw[];
     const int tilesx = (g.nxc + G - 1) / G;
     col_nl_body<F, P, SPLIT>(W, g, G, pl, tw, scale, reinterpret_cast<C2<F>*>(smem_raw), blockIdx.x % tilesx,
-                             blockIdx.x / tilesx);
+                             blockIdx.x / tilesx, threadIdx.x);
     if (d_step && blockIdx.x == 0 && threadIdx.x == 0) *d_step += 1;
 }
 
@@ -51,7 +51,7 @@ __global__ void __launch_bounds__(256) k_xupd(C2<F>* __restrict__ W, C2<F>* __re
     extern __shared__ unsigned char smem_raw[];
     const uint32_t draw = ar.noise ? (uint32_t)*d_step : 0u;
     xupd_body<F, P, SPLIT, WAVE>(W, U, pairs, npairs, G, ar, pl, tw, wph, draw,
-                                 reinterpret_cast<C2<F>*>(smem_raw), blockIdx.x);
+                                 reinterpret_cast<C2<F>*>(smem_raw), blockIdx.x, threadIdx.x);
 }
 
 // ------------------------------------------------------------------ helpers

@@ -36,6 +36,23 @@ def test_modes_agree_bit_for_bit(mcs, make):
         assert np.array_equal(outs[0][r], outs[1][r]) and np.array_equal(outs[0][r], outs[2][r])
 
 
+def test_graph_replays_between_eager_steps(mcs):
+    # the captured graph holds one branch per size group; replays interleaved with eager steps and host
+    # synchronisations must continue the same noise sequence as the looping launches
+    reps = small_3d() + [dict(shape=(16, 64), eps=0.3, psi0=-0.25, amp=0.05, temp=5e-4, dt=0.5, init_seed=5,
+                              noise_seed=6)]
+    a = mcs.Ensemble(reps, dtype=np.float32)
+    a.init()
+    a.step(27, mcs.ENS_LOOP)
+    b = mcs.Ensemble(reps, dtype=np.float32)
+    b.init()
+    b.step(8, mcs.ENS_GRAPH)
+    b.step(3, mcs.ENS_BATCHED)
+    b.step(16, mcs.ENS_GRAPH)
+    for r in range(len(reps)):
+        assert np.array_equal(a.get_psi(r), b.get_psi(r)), r
+
+
 def test_replicas_match_single_stepper(mcs):
     reps = small_2d()
     e = mcs.Ensemble(reps, dtype=np.float32)
