@@ -34,6 +34,7 @@ struct FftPlanDev {
     int np;
     int radix[MCS_MAX_PASSES];
     int ns[MCS_MAX_PASSES];
+    unsigned nsm[MCS_MAX_PASSES];  // ceil(2^32 / ns): j % ns = j - ns * umulhi(j, nsm) for the j of a line
     int twoff[MCS_MAX_PASSES];
 };
 
@@ -199,8 +200,8 @@ MCS_DEV void apply_tw(C2<F>* v, const C2<F>* __restrict__ tw, int k) {
 
 // ------------------------------------------------------------ register engine
 template <typename F, int P, int R, int DIR>
-MCS_DEV void rpass(C2<F> (&a)[P], int t, int S, int Ns, const C2<F>* __restrict__ tw, C2<F>* lds, int lbase,
-                   int lps, bool last) {
+MCS_DEV void rpass(C2<F> (&a)[P], int t, int S, int Ns, unsigned nsm, const C2<F>* __restrict__ tw, C2<F>* lds,
+                   int lbase, int lps, bool last) {
     constexpr int B = P / R;
 #pragma unroll
     for (int b = 0; b < B; ++b) {
@@ -208,7 +209,7 @@ MCS_DEV void rpass(C2<F> (&a)[P], int t, int S, int Ns, const C2<F>* __restrict_
 #pragma unroll
         for (int q = 0; q < R; ++q) v[q] = a[b + q * B];
         const int j = t + b * S;
-        const int k = j % Ns;
+        const int k = Ns > 1 ? j - Ns * (int)__umulhi((unsigned)j, nsm) : 0;
         if (Ns > 1) apply_tw<F, R, DIR>(v, tw, k);
         dft<F, R, DIR>(v);
         if (last) {
@@ -243,15 +244,16 @@ MCS_DEV void reg_fft(C2<F> (&a)[P], const FftPlanDev& pl, const C2<F>* __restric
         const bool last = (p == pl.np - 1);
         const int R = pl.radix[p];
         const int Ns = pl.ns[p];
+        const unsigned M = pl.nsm[p];
         const C2<F>* twp = tw + pl.twoff[p];
         switch (R) {
-            case 2: if constexpr (P % 2 == 0) rpass<F, P, 2, DIR>(a, t, S, Ns, twp, lds, lbase, lps, last); break;
-            case 3: if constexpr (P % 3 == 0) rpass<F, P, 3, DIR>(a, t, S, Ns, twp, lds, lbase, lps, last); break;
-            case 4: if constexpr (P % 4 == 0) rpass<F, P, 4, DIR>(a, t, S, Ns, twp, lds, lbase, lps, last); break;
-            case 5: if constexpr (P % 5 == 0) rpass<F, P, 5, DIR>(a, t, S, Ns, twp, lds, lbase, lps, last); break;
-            case 8: if constexpr (P % 8 == 0) rpass<F, P, 8, DIR>(a, t, S, Ns, twp, lds, lbase, lps, last); break;
-            case 16: if constexpr (P % 16 == 0) rpass<F, P, 16, DIR>(a, t, S, Ns, twp, lds, lbase, lps, last); break;
-            case 32: if constexpr (P % 32 == 0) rpass<F, P, 32, DIR>(a, t, S, Ns, twp, lds, lbase, lps, last); break;
+            case 2: if constexpr (P % 2 == 0) rpass<F, P, 2, DIR>(a, t, S, Ns, M, twp, lds, lbase, lps, last); break;
+            case 3: if constexpr (P % 3 == 0) rpass<F, P, 3, DIR>(a, t, S, Ns, M, twp, lds, lbase, lps, last); break;
+            case 4: if constexpr (P % 4 == 0) rpass<F, P, 4, DIR>(a, t, S, Ns, M, twp, lds, lbase, lps, last); break;
+            case 5: if constexpr (P % 5 == 0) rpass<F, P, 5, DIR>(a, t, S, Ns, M, twp, lds, lbase, lps, last); break;
+            case 8: if constexpr (P % 8 == 0) rpass<F, P, 8, DIR>(a, t, S, Ns, M, twp, lds, lbase, lps, last); break;
+            case 16: if constexpr (P % 16 == 0) rpass<F, P, 16, DIR>(a, t, S, Ns, M, twp, lds, lbase, lps, last); break;
+            case 32: if constexpr (P % 32 == 0) rpass<F, P, 32, DIR>(a, t, S, Ns, M, twp, lds, lbase, lps, last); break;
             default: break;
         }
         if (!last) {

@@ -144,6 +144,13 @@ int line_plan_build(LinePlan* lp, int L, int prec, bool allow_reg, int max_p) {
         const int R = rad[p];
         d.radix[p] = R;
         d.ns[p] = Ns;
+        d.nsm[p] = 0;
+        if (Ns > 1) {
+            const uint64_t m = ((uint64_t)1 << 32) / (uint64_t)Ns + (((uint64_t)1 << 32) % (uint64_t)Ns ? 1 : 0);
+            d.nsm[p] = (unsigned)m;
+            for (int j = 0; j < L; ++j)  // the identity is exact for every index of a line; checked, not assumed
+                MCS_CHECK((int)(((uint64_t)j * m) >> 32) == j / Ns, MCS_ERR_UNSUPPORTED, "index reciprocal inexact");
+        }
         d.twoff[p] = (int)re.size();
         if (Ns > 1) {
             for (int k = 0; k < Ns; ++k)
