@@ -173,6 +173,7 @@ def main():
                                   f"dt=0.5, noise off", "n": n, "precision": "fp32", "parallelism": "1 GPU"},
            "steps_per_s": 1e3 / ms_step, "hbm_roofline_frac_step": round(step_frac, 4),
            "model_bytes_per_step": b_step, "pass_ms": {k: round(v, 5) for k, v in passes.items()},
+           "plans": st.plan_info(), "autotune": mcs.get_autotune(),
            "roofline": roofline}
 
     if not a.no_fp64:

@@ -157,6 +157,28 @@ int mcs_stepper_destroy(mcs_stepper* st);
  * 3D; x_update, y_nonlinear in 2D) in ms[0..*npass). */
 int mcs_stepper_pass_times(mcs_stepper* st, int nsteps, double* ms, int cap, int* npass);
 
+/* Autotuning of the fused step (no reference counterpart).  When on (the
+ * default), mcs_stepper_create and mcs_ensemble_create time every candidate
+ * register plan (points per thread, hence radix sequence) and tile shape
+ * (lines or row pairs per workgroup) of each pass on the new buffers and keep
+ * the fastest; choices are cached per (extents, precision) for the process, so
+ * steppers and ensemble groups of one shape run the same plans.  Off: fixed
+ * default plans (reproducible bit for bit across processes). */
+int mcs_set_autotune(int on);
+int mcs_get_autotune(int* on);
+/* the stepper's plans: {P_x, row pairs per workgroup, P_y, lines per workgroup,
+ * P_z, lines per workgroup} (P_z and its tile are 0 in 2D) */
+int mcs_stepper_plan_info(mcs_stepper* st, int info[6]);
+/* a stepper with explicit plans in the layout above instead of tuned ones (for
+ * pinned, reproducible runs and plan studies); each (P, tile) must come from
+ * mcs_plan_options / mcs_tile_options, else MCS_ERR_UNSUPPORTED */
+int mcs_stepper_create_planned(mcs_stepper** st, const mcs_pfc_params* p, const int plan[6]);
+/* register-plan points per thread for an axis of length L (default first; *n is the count) */
+int mcs_plan_options(int L, int prec, int* P, int cap, int* n);
+/* tile shapes for that plan: row pairs per workgroup (xpass = 1, the packed x axis of
+ * length L = nx/2) or lines per workgroup (xpass = 0, the y and z axes) */
+int mcs_tile_options(int L, int prec, int P, int xpass, int* G, int cap, int* n);
+
 /* HIP-event timer on the library stream (instrumentation, no reference counterpart) */
 int mcs_timer_start(void);
 int mcs_timer_stop(double* ms);

@@ -90,6 +90,13 @@ _SIGS = {
     "mcs_stepper_mass": ([_vp, _dp], ctypes.c_int),
     "mcs_stepper_destroy": ([_vp], ctypes.c_int),
     "mcs_stepper_pass_times": ([_vp, ctypes.c_int, _dp, ctypes.c_int, _ip], ctypes.c_int),
+    "mcs_stepper_plan_info": ([_vp, _ip], ctypes.c_int),
+    "mcs_set_autotune": ([ctypes.c_int], ctypes.c_int),
+    "mcs_get_autotune": ([_ip], ctypes.c_int),
+    "mcs_stepper_create_planned": ([ctypes.POINTER(_vp), ctypes.c_void_p, _ip], ctypes.c_int),
+    "mcs_plan_options": ([ctypes.c_int, ctypes.c_int, _ip, ctypes.c_int, _ip], ctypes.c_int),
+    "mcs_tile_options": ([ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _ip, ctypes.c_int, _ip],
+                         ctypes.c_int),
     "mcs_timer_start": ([], ctypes.c_int),
     "mcs_timer_stop": ([_dp], ctypes.c_int),
     "mcs_ensemble_create": ([ctypes.POINTER(_vp), ctypes.POINTER(ReplicaParams), ctypes.c_int, ctypes.c_int,
@@ -371,7 +378,9 @@ def bond_order_knn(ijk, n, h, k=12, nbins=64):
 class Stepper:
     """The fused PFC stepper (mcs_stepper_*)."""
 
-    def __init__(self, shape_zyx, h, eps, dt, temp=0.0, seed=0, split=SPLIT_NORTHSTAR, dtype=np.float32):
+    def __init__(self, shape_zyx, h, eps, dt, temp=0.0, seed=0, split=SPLIT_NORTHSTAR, dtype=np.float32,
+                 plan=None):
+        """plan: optional explicit [P_x, pairs/wg, P_y, lines/wg, P_z, lines/wg] instead of the tuned plans"""
         self.shape = tuple(shape_zyx)
         self.dtype = np.dtype(dtype)
         p = PfcParams()
@@ -380,7 +389,12 @@ class Stepper:
         p.eps, p.dt, p.temp, p.seed, p.split, p.prec = eps, dt, temp, seed, split, _prec(dtype)
         self.params = p
         ptr = _vp()
-        check(lib().mcs_stepper_create(ctypes.byref(ptr), ctypes.byref(p)), "mcs_stepper_create")
+        if plan is None:
+            check(lib().mcs_stepper_create(ctypes.byref(ptr), ctypes.byref(p)), "mcs_stepper_create")
+        else:
+            pl = (ctypes.c_int * 6)(*(list(plan) + [0] * (6 - len(plan))))
+            check(lib().mcs_stepper_create_planned(ctypes.byref(ptr), ctypes.addressof(p), pl),
+                  "mcs_stepper_create_planned")
         self.ptr = ptr
         self.buf = DeviceBuffer(int(np.prod(shape_zyx)) * self.dtype.itemsize)
 
@@ -420,11 +434,45 @@ class Stepper:
             ["x_update", "y_nonlinear"]
         return dict(zip(names, [ms[i] for i in range(npass.value)]))
 
+    def plan_info(self):
+        """points per thread and lines (row pairs) per workgroup of each pass"""
+        info = (ctypes.c_int * 6)()
+        check(lib().mcs_stepper_plan_info(self.ptr, info), "mcs_stepper_plan_info")
+        names = ["x", "y", "z"] if len(self.shape) == 3 else ["x", "y"]
+        return {n: {"P": info[2 * i], "tile": info[2 * i + 1]} for i, n in enumerate(names)}
+
     def __del__(self):
         try:
             lib().mcs_stepper_destroy(self.ptr)
         except Exception:
             pass
+
+
+def set_autotune(on):
+    """switch the per-shape plan and tile autotuner on (default) or off (fixed default plans)"""
+    check(lib().mcs_set_autotune(1 if on else 0), "mcs_set_autotune")
+
+
+def plan_options(L, dtype=np.float32):
+    """register-plan points per thread available for an axis of length L (default first)"""
+    buf = (ctypes.c_int * 8)()
+    n = ctypes.c_int()
+    check(lib().mcs_plan_options(L, _prec(dtype), buf, 8, ctypes.byref(n)), "mcs_plan_options")
+    return [buf[i] for i in range(min(n.value, 8))]
+
+
+def tile_options(L, dtype, P, xpass):
+    buf = (ctypes.c_int * 16)()
+    n = ctypes.c_int()
+    check(lib().mcs_tile_options(L, _prec(dtype), P, 1 if xpass else 0, buf, 16, ctypes.byref(n)),
+          "mcs_tile_options")
+    return [buf[i] for i in range(min(n.value, 16))]
+
+
+def get_autotune():
+    v = ctypes.c_int()
+    check(lib().mcs_get_autotune(ctypes.byref(v)), "mcs_get_autotune")
+    return bool(v.value)
 
 
 def sync():

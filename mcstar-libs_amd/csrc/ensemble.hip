@@ -374,6 +374,12 @@ extern "C" int mcs_ensemble_create(mcs_ensemble** out, const mcs_replica* reps, 
             rc = MCS_ERR_NOMEM;
         }
     }
+    // plans and tile shapes per size group, timed on the group's first state buffers
+    for (size_t gi = 0; gi < e->groups.size() && !rc; ++gi) {
+        EnsGroup* g = e->groups[gi];
+        const int r0 = g->members[0];
+        rc = step_geom_tune(&g->sg, phys_of(e->reps[r0], split, 0.0), e->W[r0], e->U[r0], mcs_stream());
+    }
     if (!rc && hipMalloc((void**)&e->d_step, sizeof(int64_t)) != hipSuccess) rc = MCS_ERR_NOMEM;
     if (!rc && hipMemset(e->d_step, 0, sizeof(int64_t)) != hipSuccess) rc = MCS_ERR_HIP;
     if (!rc) rc = upload_reps(e);

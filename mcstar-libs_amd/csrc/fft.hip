@@ -167,10 +167,7 @@ template <typename F, int P, int DIR>
 static int launch_col_reg_t(const LinePlan& lp, const void* src, void* dst, const ColGeom& g, hipStream_t s) {
     const int S = lp.L / P;
     const int esz = (int)sizeof(C2<F>);
-    int G = (int)(128 / esz);
-    if (G > mcs_reg_max_threads(P, sizeof(F) == 8 ? 64 : 32) / S) G = mcs_reg_max_threads(P, sizeof(F) == 8 ? 64 : 32) / S;
-    while (G > 1 && (size_t)G * lp.L * esz > 65536) --G;
-    if (G < 1) G = 1;
+    const int G = col_lines_per_wg(lp);
     const size_t lds = (size_t)G * lp.L * esz;
     auto fn = k_col_reg<F, P, DIR>;
     int st = set_lds((const void*)fn, lds);

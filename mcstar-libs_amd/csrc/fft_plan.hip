@@ -69,6 +69,52 @@ static int reg_plan(int L, int prec, std::vector<int>* rad, int max_p = 32) {
 
 int line_plan_reg_P(int L, int prec) { return reg_plan(L, prec, nullptr); }
 
+// power-of-two L with P points per thread: radix-P passes after one remainder pass
+static bool reg_plan_pow2(int L, int P, std::vector<int>* rad) {
+    int a, b, c;
+    if (L < 4 || factor235(L, a, b, c) != 1 || b || c) return false;
+    if (P < 4 || (P & (P - 1)) || P > L || L / P > 1024) return false;
+    int lp = 0;
+    while ((1 << lp) < P) ++lp;
+    std::vector<int> r;
+    if (a % lp) r.push_back(1 << (a % lp));
+    for (int i = 0; i < a / lp; ++i) r.push_back(P);
+    if (rad) *rad = r;
+    return true;
+}
+
+std::vector<int> line_plan_reg_options(int L, int prec) {
+    std::vector<int> out;
+    const int def = reg_plan(L, prec, nullptr);
+    if (!def) return out;
+    out.push_back(def);
+    const size_t e = prec == 64 ? 16 : 8;
+    for (int P : {4, 8, 16, 32}) {
+        if (P == def || (prec == 64 && P > 16) || (size_t)L * e > MCS_LDS_MAX) continue;
+        if (reg_plan_pow2(L, P, nullptr)) out.push_back(P);
+    }
+    return out;
+}
+
+int col_lines_per_wg(const LinePlan& lp) {
+    if (lp.G > 0) return lp.G;
+    const int S = lp.L / lp.P;
+    const size_t esz = lp.prec == 64 ? 16 : 8;
+    int G = (int)(128 / esz);
+    if (G > mcs_reg_max_threads(lp.P, lp.prec) / S) G = mcs_reg_max_threads(lp.P, lp.prec) / S;
+    while (G > 1 && (size_t)G * lp.L * esz > 65536) --G;
+    return G < 1 ? 1 : G;
+}
+
+int xupd_pairs_per_wg(const LinePlan& lp) {
+    if (lp.G > 0) return lp.G;
+    const int S = lp.L / lp.P;
+    const size_t esz = lp.prec == 64 ? 16 : 8;
+    int G = S >= 256 ? 1 : 256 / S;
+    while (G > 1 && (size_t)G * 2 * lp.L * esz > 65536) --G;
+    return G;
+}
+
 static void gen_radices(int L, std::vector<int>& r) {
     int a, b, c;
     factor235(L, a, b, c);
@@ -113,12 +159,15 @@ static void phase(long j, long n, double& c, double& s) {
     } else if (n % 2 == 0 && j == n / 2) { c = -1; s = 0; }
 }
 
+static int plan_finish(LinePlan* lp, const std::vector<int>& rad);
+
 int line_plan_build(LinePlan* lp, int L, int prec, bool allow_reg, int max_p) {
     MCS_CHECK(L >= 1, MCS_ERR_INVALID, "FFT length must be >= 1");
     MCS_CHECK(prec == 32 || prec == 64, MCS_ERR_INVALID, "precision must be 32 or 64");
     MCS_CHECK(generic_ok(L, nullptr), MCS_ERR_UNSUPPORTED, "FFT length must have no prime factor above 13");
     lp->L = L;
     lp->prec = prec;
+    lp->G = 0;
     std::vector<int> rad;
     const int P = allow_reg ? reg_plan(L, prec, nullptr, max_p) : 0;
     if (P) {
@@ -133,6 +182,29 @@ int line_plan_build(LinePlan* lp, int L, int prec, bool allow_reg, int max_p) {
         gen_radices(L, rad);
         MCS_CHECK(line_plan_gen_lds(*lp, 1) <= MCS_LDS_MAX, MCS_ERR_UNSUPPORTED, "FFT length too large for LDS");
     }
+    return plan_finish(lp, rad);
+}
+
+int line_plan_build_reg(LinePlan* lp, int L, int prec, int P) {
+    MCS_CHECK(prec == 32 || prec == 64, MCS_ERR_INVALID, "precision must be 32 or 64");
+    std::vector<int> rad;
+    if (P == reg_plan(L, prec, nullptr)) {
+        reg_plan(L, prec, &rad);
+    } else {
+        MCS_CHECK(reg_plan_pow2(L, P, &rad), MCS_ERR_UNSUPPORTED, "no register plan with this many points per thread");
+    }
+    const size_t e = prec == 64 ? 16 : 8;
+    MCS_CHECK((size_t)L * e <= MCS_LDS_MAX, MCS_ERR_UNSUPPORTED, "FFT length too large for LDS");
+    lp->L = L;
+    lp->prec = prec;
+    lp->G = 0;
+    lp->engine = ENGINE_REG;
+    lp->P = P;
+    return plan_finish(lp, rad);
+}
+
+static int plan_finish(LinePlan* lp, const std::vector<int>& rad) {
+    const int L = lp->L, prec = lp->prec;
     MCS_CHECK((int)rad.size() <= MCS_MAX_PASSES, MCS_ERR_UNSUPPORTED, "too many FFT passes");
     FftPlanDev& d = lp->dev;
     d = FftPlanDev{};

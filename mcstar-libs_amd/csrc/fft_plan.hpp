@@ -4,6 +4,8 @@
 
 #include "fft_core.hpp"
 
+#include <vector>
+
 enum { ENGINE_REG = 0, ENGINE_GEN = 1 };
 
 struct LinePlan {
@@ -11,6 +13,7 @@ struct LinePlan {
     int prec = 32;
     int engine = ENGINE_GEN;
     int P = 0;             // register engine: points per thread
+    int G = 0;             // lines (or row pairs) per workgroup chosen by the tuner; 0 = default rule
     FftPlanDev dev{};      // passed by value to kernels
     void* tw = nullptr;    // device twiddle table (C2<F>)
     size_t twcount = 0;
@@ -20,7 +23,16 @@ struct LinePlan {
 // max_p caps the points per thread of power-of-two register plans (kernels holding several
 // register blocks, like the stepper's x pass, run better with fewer)
 int line_plan_build(LinePlan* lp, int L, int prec, bool allow_reg = true, int max_p = 32);
+// register plan with exactly P points per thread (P a power of two; the radix-P passes
+// follow a remainder pass); MCS_ERR_UNSUPPORTED if no such plan exists
+int line_plan_build_reg(LinePlan* lp, int L, int prec, int P);
+// the P values line_plan_build_reg accepts for L, the default first
+std::vector<int> line_plan_reg_options(int L, int prec);
 void line_plan_free(LinePlan* lp);
+// lines per workgroup of a column pass / row pairs per workgroup of the stepper's x pass
+// (lp.G when the tuner set it, else the default rule)
+int col_lines_per_wg(const LinePlan& lp);
+int xupd_pairs_per_wg(const LinePlan& lp);
 // register engine eligibility for L (without building)
 int line_plan_reg_P(int L, int prec = 32);
 // generic engine LDS bytes for G lines

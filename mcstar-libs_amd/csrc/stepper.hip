@@ -251,23 +251,6 @@ ColGeom step_geom_z(const StepGeom& sg) { return ColGeom{sg.L, (long)sg.L * sg.n
 static ColGeom geom_y(const StepGeom& sg) { return step_geom_y(sg); }
 static ColGeom geom_z(const StepGeom& sg) { return step_geom_z(sg); }
 
-int col_lines_per_wg(const LinePlan& lp) {
-    const int S = lp.L / lp.P;
-    const size_t esz = lp.prec == 64 ? 16 : 8;
-    int G = (int)(128 / esz);
-    if (G > mcs_reg_max_threads(lp.P, lp.prec) / S) G = mcs_reg_max_threads(lp.P, lp.prec) / S;
-    while (G > 1 && (size_t)G * lp.L * esz > 65536) --G;
-    return G < 1 ? 1 : G;
-}
-
-int xupd_pairs_per_wg(const LinePlan& lp) {
-    const int S = lp.L / lp.P;
-    const size_t esz = lp.prec == 64 ? 16 : 8;
-    int G = S >= 256 ? 1 : 256 / S;
-    while (G > 1 && (size_t)G * 2 * lp.L * esz > 65536) --G;
-    return G;
-}
-
 // W holds IFFT_x(U) on entry; leaves FFT_yz(N(psi)) in W
 int step_nonlinear_half(const StepGeom& sg, const StepPhys& ph, void* W, int64_t* d_step, hipStream_t s) {
     const double scale = 2.0 / ((double)sg.nx * sg.ny * sg.nz);
@@ -385,7 +368,7 @@ struct mcs_stepper {
     int64_t step = 0;
 };
 
-extern "C" int mcs_stepper_create(mcs_stepper** out, const mcs_pfc_params* p) {
+static int stepper_create(mcs_stepper** out, const mcs_pfc_params* p, const int* plan) {
     MCS_CHECK(out && p, MCS_ERR_INVALID, "null argument");
     MCS_CHECK(p->ext[0] >= 1 && p->ext[1] >= 1 && p->ext[2] >= 1, MCS_ERR_INVALID, "extent must be >=1");
     MCS_CHECK(p->prec == 32 || p->prec == 64, MCS_ERR_INVALID, "precision must be 32 or 64");
@@ -404,7 +387,22 @@ extern "C" int mcs_stepper_create(mcs_stepper** out, const mcs_pfc_params* p) {
     st->ph.h[0] = p->h[0]; st->ph.h[1] = p->h[1]; st->ph.h[2] = p->h[2];
     st->ph.eps = p->eps; st->ph.dt = p->dt; st->ph.temp = p->temp; st->ph.seed = p->seed; st->ph.split = p->split;
     st->ph.mass = 0;
+    rc = plan ? step_geom_apply_plan(&st->sg, plan) : step_geom_tune(&st->sg, st->ph, st->W, st->U, mcs_stream());
+    if (rc) { mcs_stepper_destroy(st); return rc; }
     *out = st;
+    return MCS_OK;
+}
+
+extern "C" int mcs_stepper_create(mcs_stepper** out, const mcs_pfc_params* p) { return stepper_create(out, p, nullptr); }
+
+extern "C" int mcs_stepper_create_planned(mcs_stepper** out, const mcs_pfc_params* p, const int plan[6]) {
+    MCS_CHECK(plan, MCS_ERR_INVALID, "null plan");
+    return stepper_create(out, p, plan);
+}
+
+extern "C" int mcs_stepper_plan_info(mcs_stepper* st, int info[6]) {
+    MCS_CHECK(st && info, MCS_ERR_INVALID, "null argument");
+    step_geom_plan_info(st->sg, info);
     return MCS_OK;
 }
 

@@ -49,10 +49,16 @@ template <typename F> inline XArgs<F> make_xargs(const StepGeom& sg, const StepP
     return ar;
 }
 
-// lines per workgroup of a column pass / row pairs per workgroup of the x pass
-int col_lines_per_wg(const LinePlan& lp);
-int xupd_pairs_per_wg(const LinePlan& lp);
+int launch_xupd(const StepGeom& sg, const StepPhys& ph, void* W, void* U, hipStream_t s);
 int launch_col_nl(const LinePlan& lp, void* W, const ColGeom& g, double scale, int split, int64_t* d_step,
                   hipStream_t s);
 ColGeom step_geom_y(const StepGeom& sg);
 ColGeom step_geom_z(const StepGeom& sg);
+
+// autotuner: picks the register plan and tile shape of every pass for this shape and
+// precision (cached per process); a no-op when tuning is switched off.  Overwrites W and U.
+int step_geom_tune(StepGeom* sg, const StepPhys& ph, void* W, void* U, hipStream_t s);
+// {P_x, pairs per workgroup, P_y, lines per workgroup, P_z, lines per workgroup}
+void step_geom_plan_info(const StepGeom& sg, int info[6]);
+// installs an explicit plan in that layout after checking it against the candidate lists
+int step_geom_apply_plan(StepGeom* sg, const int plan[6]);

@@ -1,0 +1,225 @@
+// Autotuner of the fused step.  For one grid shape and precision it times, on
+// the caller's own buffers, every candidate register plan (points per thread P,
+// hence the radix sequence and the number of LDS exchanges) and tile shape
+// (lines or row pairs per workgroup, hence the global segment width and the
+// LDS footprint) of each pass, and keeps the fastest.  Choices are cached per
+// (nx, ny, nz, precision) for the life of the process, so that a stepper and an
+// ensemble group of the same shape run identical plans, and therefore
+// bit-identical arithmetic.  Tuning runs with the noise off; the pass kernels
+// overwrite the buffers, so it runs before a field is loaded.
+#include "stepper.hpp"
+
+#include <algorithm>
+#include <map>
+#include <mutex>
+#include <tuple>
+#include <vector>
+
+namespace {
+
+struct PassChoice {
+    int P = 0, G = 0;
+    double ms = 0;
+};
+struct StepChoice {
+    PassChoice x, y, z;
+};
+
+std::mutex g_mu;
+std::map<std::tuple<int, int, int, int>, StepChoice> g_cache;
+int g_enabled = 1;
+
+// mean milliseconds of one launch: one warm-up, then enough launches for about a millisecond
+template <typename Fn> double time_launch(hipStream_t s, Fn&& launch) {
+    hipEvent_t a = nullptr, b = nullptr;
+    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return -1;
+    double ms = -1;
+    float el = 0;
+    int reps = 1;
+    if (launch() == MCS_OK && hipEventRecord(a, s) == hipSuccess && launch() == MCS_OK &&
+        hipEventRecord(b, s) == hipSuccess && hipEventSynchronize(b) == hipSuccess &&
+        hipEventElapsedTime(&el, a, b) == hipSuccess) {
+        reps = std::max(3, std::min(50, (int)(1.0 / std::max(el, 1e-3f))));
+        bool ok = hipEventRecord(a, s) == hipSuccess;
+        for (int i = 0; i < reps && ok; ++i) ok = launch() == MCS_OK;
+        ok = ok && hipEventRecord(b, s) == hipSuccess && hipEventSynchronize(b) == hipSuccess &&
+             hipEventElapsedTime(&el, a, b) == hipSuccess;
+        if (ok) ms = el / reps;
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return ms;
+}
+
+size_t esz(int prec) { return prec == 64 ? 16 : 8; }
+
+// tile shapes of a column pass: G lines of L points, G*S threads within the launch bound,
+// G*L elements of LDS within 64 KB
+std::vector<int> col_tiles(int L, int P, int prec) {
+    const int S = L / P;
+    std::vector<int> out;
+    for (int G : {4, 8, 16, 32})
+        if (G * S <= mcs_reg_max_threads(P, prec) && (size_t)G * L * esz(prec) <= 65536) out.push_back(G);
+    if (out.empty() && S <= mcs_reg_max_threads(P, prec) && (size_t)L * esz(prec) <= 65536) out.push_back(1);
+    return out;
+}
+
+// tile shapes of the x pass: G row pairs, 64..256 threads (the kernel's launch bound),
+// two rows of LDS per pair within 64 KB
+std::vector<int> x_tiles(int L, int P, int prec) {
+    const int S = L / P;
+    std::vector<int> out;
+    for (int G = 1; G * S <= 256; G *= 2)
+        if ((G * S >= 64 || (G == 1 && S >= 64)) && (size_t)G * 2 * L * esz(prec) <= 65536) out.push_back(G);
+    if (out.empty() && S <= 256 && (size_t)2 * L * esz(prec) <= 65536) out.push_back(std::max(1, 64 / S));
+    return out;
+}
+
+int rebuild(LinePlan* lp, int L, int prec, int P) {
+    if (lp->P == P && lp->engine == ENGINE_REG) return MCS_OK;
+    LinePlan np;
+    int st = line_plan_build_reg(&np, L, prec, P);
+    if (st) return st;
+    line_plan_free(lp);
+    *lp = np;
+    return MCS_OK;
+}
+
+// best (P, G) of one pass; `run` launches the pass with the candidate plan
+template <typename Run>
+int tune_pass(int L, int prec, int maxP, bool xpass, hipStream_t s, Run&& run, PassChoice* best) {
+    best->ms = -1;
+    const std::vector<int> opts = line_plan_reg_options(L, prec);
+    for (int P : opts) {
+        if (P > maxP && P != opts[0]) continue;  // the default plan is always a candidate
+        LinePlan cand;
+        if (line_plan_build_reg(&cand, L, prec, P) != MCS_OK) continue;
+        for (int G : xpass ? x_tiles(L, P, prec) : col_tiles(L, P, prec)) {
+            cand.G = G;
+            const double ms = time_launch(s, [&]() { return run(cand); });
+            if (ms > 0 && (best->ms < 0 || ms < best->ms)) *best = PassChoice{P, G, ms};
+        }
+        line_plan_free(&cand);
+    }
+    (void)hipGetLastError();
+    MCS_CHECK(best->ms > 0, MCS_ERR_UNSUPPORTED, "no tunable plan ran");
+    return MCS_OK;
+}
+
+int apply(StepGeom* sg, const StepChoice& c) {
+    int st = rebuild(&sg->px, sg->L, sg->prec, c.x.P);
+    if (!st) st = rebuild(&sg->py, sg->ny, sg->prec, c.y.P);
+    if (!st && sg->is3d) st = rebuild(&sg->pz, sg->nz, sg->prec, c.z.P);
+    if (st) return st;
+    sg->px.G = c.x.G;
+    sg->py.G = c.y.G;
+    if (sg->is3d) sg->pz.G = c.z.G;
+    return MCS_OK;
+}
+
+}  // namespace
+
+extern "C" int mcs_set_autotune(int on) {
+    std::lock_guard<std::mutex> l(g_mu);
+    g_enabled = on ? 1 : 0;
+    return MCS_OK;
+}
+
+extern "C" int mcs_get_autotune(int* on) {
+    MCS_CHECK(on, MCS_ERR_INVALID, "null argument");
+    std::lock_guard<std::mutex> l(g_mu);
+    *on = g_enabled;
+    return MCS_OK;
+}
+
+int step_geom_tune(StepGeom* sg, const StepPhys& ph_in, void* W, void* U, hipStream_t s) {
+    const auto key = std::make_tuple(sg->nx, sg->ny, sg->nz, sg->prec);
+    {
+        std::lock_guard<std::mutex> l(g_mu);
+        if (!g_enabled) return MCS_OK;
+        auto it = g_cache.find(key);
+        if (it != g_cache.end()) return apply(sg, it->second);
+    }
+    StepPhys ph = ph_in;
+    ph.temp = 0;  // noise off: the draw counter is not read
+    ph.mass = 0;
+    const int prec = sg->prec;
+    const int maxX = prec == 64 ? 8 : 16, maxC = prec == 64 ? 16 : 32;
+    const double scale = 2.0 / ((double)sg->nx * sg->ny * sg->nz);
+    StepChoice c;
+    int st = tune_pass(sg->L, prec, maxX, true, s, [&](const LinePlan& lp) {
+        StepGeom g = *sg;
+        g.px = lp;
+        return launch_xupd(g, ph, W, U, s);
+    }, &c.x);
+    if (!st && sg->is3d) {
+        const ColGeom gy = step_geom_y(*sg), gz = step_geom_z(*sg);
+        st = tune_pass(sg->ny, prec, maxC, false, s, [&](const LinePlan& lp) {
+            return launch_col(lp, W, W, gy, -1, s);
+        }, &c.y);
+        if (!st) st = tune_pass(sg->nz, prec, maxC, false, s, [&](const LinePlan& lp) {
+            return launch_col_nl(lp, W, gz, scale, ph.split, nullptr, s);
+        }, &c.z);
+    } else if (!st) {
+        const ColGeom gy = step_geom_y(*sg);
+        st = tune_pass(sg->ny, prec, maxC, false, s, [&](const LinePlan& lp) {
+            return launch_col_nl(lp, W, gy, scale, ph.split, nullptr, s);
+        }, &c.y);
+    }
+    if (st) return st;
+    MCS_HIP_TRY(hipStreamSynchronize(s));
+    {
+        std::lock_guard<std::mutex> l(g_mu);
+        auto it = g_cache.find(key);  // another thread may have tuned the same shape meanwhile
+        if (it != g_cache.end()) c = it->second;
+        else g_cache.emplace(key, c);
+    }
+    return apply(sg, c);
+}
+
+int step_geom_apply_plan(StepGeom* sg, const int plan[6]) {
+    const int prec = sg->prec;
+    const int n = sg->is3d ? 3 : 2;
+    const int L[3] = {sg->L, sg->ny, sg->nz};
+    for (int i = 0; i < n; ++i) {
+        const int P = plan[2 * i], G = plan[2 * i + 1];
+        const std::vector<int> opts = line_plan_reg_options(L[i], prec);
+        MCS_CHECK(std::find(opts.begin(), opts.end(), P) != opts.end(), MCS_ERR_UNSUPPORTED,
+                  "points per thread not available for this axis length");
+        const std::vector<int> tiles = i == 0 ? x_tiles(L[i], P, prec) : col_tiles(L[i], P, prec);
+        MCS_CHECK(std::find(tiles.begin(), tiles.end(), G) != tiles.end(), MCS_ERR_UNSUPPORTED,
+                  "tile shape outside the launch and LDS limits of this plan");
+    }
+    StepChoice c;
+    c.x = PassChoice{plan[0], plan[1], 0};
+    c.y = PassChoice{plan[2], plan[3], 0};
+    c.z = PassChoice{plan[4], plan[5], 0};
+    return apply(sg, c);
+}
+
+extern "C" int mcs_plan_options(int L, int prec, int* P, int cap, int* n) {
+    MCS_CHECK(P && n && cap >= 0, MCS_ERR_INVALID, "bad argument");
+    MCS_CHECK(prec == 32 || prec == 64, MCS_ERR_INVALID, "precision must be 32 or 64");
+    const std::vector<int> o = line_plan_reg_options(L, prec);
+    *n = (int)o.size();
+    for (int i = 0; i < (int)o.size() && i < cap; ++i) P[i] = o[i];
+    return MCS_OK;
+}
+
+extern "C" int mcs_tile_options(int L, int prec, int P, int xpass, int* G, int cap, int* n) {
+    MCS_CHECK(G && n && cap >= 0 && P > 0 && L >= P, MCS_ERR_INVALID, "bad argument");
+    MCS_CHECK(prec == 32 || prec == 64, MCS_ERR_INVALID, "precision must be 32 or 64");
+    const std::vector<int> o = xpass ? x_tiles(L, P, prec) : col_tiles(L, P, prec);
+    *n = (int)o.size();
+    for (int i = 0; i < (int)o.size() && i < cap; ++i) G[i] = o[i];
+    return MCS_OK;
+}
+
+void step_geom_plan_info(const StepGeom& sg, int info[6]) {
+    info[0] = sg.px.P;
+    info[1] = xupd_pairs_per_wg(sg.px);
+    info[2] = sg.py.P;
+    info[3] = col_lines_per_wg(sg.py);
+    info[4] = sg.is3d ? sg.pz.P : 0;
+    info[5] = sg.is3d ? col_lines_per_wg(sg.pz) : 0;
+}
