@@ -13,7 +13,19 @@ LIB := $(PKG)/lib/libmcstar_hip.so
 HIPFLAGS := -O3 -std=c++17 -fPIC --offload-arch=$(ARCH) -Xclang -target-feature -Xclang -packed-fp32-ops \
             -Wall -Wno-unused-function -Wno-unused-result -I include
 
-all: lib oracle driver
+all: lib oracle driver sim
+
+# the simulator driver (mcstar-libs_amd/sim), one build per precision
+SIM := $(PKG)/bin/mcs_sim_f32 $(PKG)/bin/mcs_sim_f64
+sim: $(SIM)
+
+$(PKG)/bin/mcs_sim_f32: $(PKG)/sim/mcs_sim.cpp $(LIB) $(wildcard include/*/*.hpp include/*/*/*.hpp)
+	@mkdir -p $(dir $@)
+	$(HIPCC) -O2 -std=c++17 -DFLOAT=float -I include $< -o $@ -L$(PKG)/lib -lmcstar_hip -Wl,-rpath,'$$ORIGIN/../lib'
+
+$(PKG)/bin/mcs_sim_f64: $(PKG)/sim/mcs_sim.cpp $(LIB) $(wildcard include/*/*.hpp include/*/*/*.hpp)
+	@mkdir -p $(dir $@)
+	$(HIPCC) -O2 -std=c++17 -DFLOAT=double -I include $< -o $@ -L$(PKG)/lib -lmcstar_hip -Wl,-rpath,'$$ORIGIN/../lib'
 
 # C++ drop-in driver of the kept host API (tests/cpp), one build per precision
 DRV := tests/cpp/bin/kept_api_driver_f32 tests/cpp/bin/kept_api_driver_f64
@@ -42,6 +54,6 @@ oracle:
 	$(MAKE) -s -C oracle liboracle.so
 
 clean:
-	rm -rf $(OBJDIR) $(PKG)/lib tests/cpp/bin
+	rm -rf $(OBJDIR) $(PKG)/lib $(PKG)/bin tests/cpp/bin
 
 .PHONY: all lib oracle driver clean

@@ -107,6 +107,31 @@ def ensemble_bench(mcs, kind, steps):
     return out
 
 
+def fft_bench(mcs, n, reps):
+    """boundary FFT (the FFT::CufftPlan replacement) R2C and C2R on an n^3 FP32 field"""
+    shp = (n, n, n)
+    nb = mcs.buffer_bytes(shp, np.float32)
+    x = mcs.DeviceBuffer(nb)
+    mcs.check(mcs.lib().mcs_init_field(x.ptr, mcs._ext(shp), 0.0, 1.0, 3, 32), "mcs_init_field")
+    k, y = mcs.DeviceBuffer(nb), mcs.DeviceBuffer(nb)
+    fwd, inv = mcs.FFTPlan(shp, np.float32, mcs.R2C), mcs.FFTPlan(shp, np.float32, mcs.C2R)
+    fwd.execute(x, k)
+    inv.execute(k, y)
+    out = {}
+    # algorithmic bytes: x pass reads/writes the real field and the half spectrum, y and z read/write the spectrum
+    half = (n // 2 + 1) * n * n * 8
+    byts = (n ** 3 * 4 + half) + 2 * 2 * half
+    for name, plan, a, b in (("r2c", fwd, x, k), ("c2r", inv, k, y)):
+        mcs.sync()
+        with mcs.Timer() as t:
+            for _ in range(reps):
+                mcs.check(mcs.lib().mcs_fft_exec(plan.ptr, a.ptr, b.ptr), "mcs_fft_exec")
+        ms = t.ms / reps
+        out[name] = {"ms": round(ms, 4), "GB_per_s": round(byts / (ms / 1e3) / 1e9, 1)}
+    out["bytes_per_transform"] = byts
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -118,6 +143,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--only-ensemble", action="store_true", help="run only the ensemble lines (profiling)")
+    ap.add_argument("--only-fft", action="store_true", help="run only the boundary FFT lines (profiling)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a 1 GiB device-to-device copy (reference dispatch for HBM counters)")
     a = ap.parse_args()
@@ -129,6 +155,9 @@ def main():
     mcs = load_pkg()
     if a.only_ensemble:
         print(json.dumps({"ensemble_2d": ensemble_bench(mcs, "2d", 40), "ensemble_3d": ensemble_bench(mcs, "3d", 16)}))
+        return
+    if a.only_fft:
+        print(json.dumps({"boundary_fft_512": fft_bench(mcs, a.n, 5)}))
         return
     if a.copy_calibration:
         nb = 1 << 30
@@ -196,6 +225,8 @@ def main():
                        "rel_l2_fp32_vs_fp64_gpu": rel, "max_abs_fp32_vs_fp64_gpu": float(np.abs(p32 - p64).max()),
                        "steps_compared": n_cmp}
         del st64, p32, p64
+    log("[bench] boundary FFT")
+    out["boundary_fft"] = fft_bench(mcs, n, 5)
     if not a.no_ensemble:
         log("[bench] ensembles")
         out["ensemble_2d"] = ensemble_bench(mcs, "2d", 40)

@@ -166,6 +166,14 @@ int mcs_stepper_pass_times(mcs_stepper* st, int nsteps, double* ms, int cap, int
  * default plans (reproducible bit for bit across processes). */
 int mcs_set_autotune(int on);
 int mcs_get_autotune(int* on);
+/* Exact checkpoint of the stepper (no reference counterpart; the simulators saved psi
+ * with CPUArray::save, array_abstraction.hpp:155-180).  The state is the packed spectrum
+ * U = FFT(u) of u(m,y,z) = psi(2m,y,z) + i psi(2m+1,y,z) over (nx/2) x ny x nz, complex,
+ * unnormalised, nx*ny*nz reals of the stepper's precision, plus the FP64 mass (sum of
+ * psi) and the step number (the noise draw).  spec may be host or device memory.
+ * Restoring into a stepper with the same plans continues bit for bit. */
+int mcs_stepper_get_state(mcs_stepper* st, void* spec, double* mass, int64_t* step);
+int mcs_stepper_set_state(mcs_stepper* st, const void* spec, double mass, int64_t step);
 /* the stepper's plans: {P_x, row pairs per workgroup, P_y, lines per workgroup,
  * P_z, lines per workgroup} (P_z and its tile are 0 in 2D) */
 int mcs_stepper_plan_info(mcs_stepper* st, int info[6]);
@@ -215,6 +223,12 @@ int mcs_ensemble_step_async(mcs_ensemble* e, int nsteps, int mode);
  * all on the device; counts[nrep], hist_q4/hist_q6[nrep * nbins] on the host */
 int mcs_ensemble_analyze(mcs_ensemble* e, double ths, int k, int nbins, int64_t* counts, int64_t* hist_q4,
                          int64_t* hist_q6);
+/* nsteps (>= 8) steps replayed from captured graphs, the last graph also holding every
+ * (3D) replica's peak finding and k-nearest-neighbour q4/q6 histograms after its final
+ * step; outputs as mcs_ensemble_analyze.  Analysis buffers and the graph are kept for
+ * later calls with the same (ths, k, nbins). */
+int mcs_ensemble_run(mcs_ensemble* e, int nsteps, double ths, int k, int nbins, int64_t* counts, int64_t* hist_q4,
+                     int64_t* hist_q6);
 int mcs_ensemble_destroy(mcs_ensemble* e);
 
 /* ---------------------------------------------------------------- peaks

@@ -91,8 +91,12 @@ _SIGS = {
     "mcs_stepper_destroy": ([_vp], ctypes.c_int),
     "mcs_stepper_pass_times": ([_vp, ctypes.c_int, _dp, ctypes.c_int, _ip], ctypes.c_int),
     "mcs_stepper_plan_info": ([_vp, _ip], ctypes.c_int),
+    "mcs_stepper_get_state": ([_vp, ctypes.c_void_p, _dp, _i64p], ctypes.c_int),
+    "mcs_stepper_set_state": ([_vp, ctypes.c_void_p, ctypes.c_double, ctypes.c_int64], ctypes.c_int),
     "mcs_set_autotune": ([ctypes.c_int], ctypes.c_int),
     "mcs_get_autotune": ([_ip], ctypes.c_int),
+    "mcs_ensemble_run": ([_vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int, _i64p, _i64p, _i64p],
+                         ctypes.c_int),
     "mcs_stepper_create_planned": ([ctypes.POINTER(_vp), ctypes.c_void_p, _ip], ctypes.c_int),
     "mcs_plan_options": ([ctypes.c_int, ctypes.c_int, _ip, ctypes.c_int, _ip], ctypes.c_int),
     "mcs_tile_options": ([ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _ip, ctypes.c_int, _ip],
@@ -434,6 +438,21 @@ class Stepper:
             ["x_update", "y_nonlinear"]
         return dict(zip(names, [ms[i] for i in range(npass.value)]))
 
+    def get_state(self):
+        """(packed spectrum [nz][ny][nx/2] complex, FP64 mass, step): an exact checkpoint"""
+        ct = np.complex128 if self.dtype == np.float64 else np.complex64
+        shp = self.shape[:-1] + (self.shape[-1] // 2,)
+        spec = np.empty(shp, ct)
+        mass, step = ctypes.c_double(), ctypes.c_int64()
+        check(lib().mcs_stepper_get_state(self.ptr, spec.ctypes.data, ctypes.byref(mass), ctypes.byref(step)),
+              "mcs_stepper_get_state")
+        return spec, mass.value, step.value
+
+    def set_state(self, spec, mass, step):
+        ct = np.complex128 if self.dtype == np.float64 else np.complex64
+        spec = np.ascontiguousarray(spec, dtype=ct)
+        check(lib().mcs_stepper_set_state(self.ptr, spec.ctypes.data, mass, step), "mcs_stepper_set_state")
+
     def plan_info(self):
         """points per thread and lines (row pairs) per workgroup of each pass"""
         info = (ctypes.c_int * 6)()
@@ -538,6 +557,16 @@ class Ensemble:
         h6 = np.zeros((nr, nbins), np.int64)
         check(lib().mcs_ensemble_analyze(self.ptr, ths, k, nbins, counts.ctypes.data_as(_i64p),
                                          h4.ctypes.data_as(_i64p), h6.ctypes.data_as(_i64p)), "mcs_ensemble_analyze")
+        return counts, h4, h6
+
+    def run(self, nsteps, ths, k=12, nbins=64):
+        """nsteps graph-replayed steps with peaks and q4/q6 histograms captured after the last one"""
+        nr = len(self.shapes)
+        counts = np.zeros(nr, np.int64)
+        h4 = np.zeros((nr, nbins), np.int64)
+        h6 = np.zeros((nr, nbins), np.int64)
+        check(lib().mcs_ensemble_run(self.ptr, nsteps, ths, k, nbins, counts.ctypes.data_as(_i64p),
+                                     h4.ctypes.data_as(_i64p), h6.ctypes.data_as(_i64p)), "mcs_ensemble_run")
         return counts, h4, h6
 
     def __del__(self):

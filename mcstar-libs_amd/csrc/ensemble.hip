@@ -81,6 +81,11 @@ __global__ void k_ens_advance(int64_t* d_step, int n) {
     if (blockIdx.x == 0 && threadIdx.x == 0) *d_step += n;
 }
 
+// clears histogram bins inside a captured graph (a kernel node, not a memset node)
+__global__ void k_zero_u64(unsigned long long* p, long n) {
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) p[i] = 0ull;
+}
+
 // ------------------------------------------------------------------ host side
 struct EnsGroup {
     StepGeom sg;
@@ -104,6 +109,18 @@ struct mcs_ensemble {
     std::vector<hipStream_t> side;     // one capture branch per size group
     std::vector<hipEvent_t> side_ev;   // [0] fork, [1 + gi] join of group gi
     int cus = 0;
+    // in-graph analysis (mcs_ensemble_run): the last GRAPH_STEPS steps followed by peaks and
+    // k-nearest-neighbour q4/q6 histograms of every replica, on per-group device buffers
+    hipGraphExec_t aexec = nullptr;
+    hipGraph_t agraph = nullptr;
+    double a_ths = 0;
+    int a_k = 0, a_nbins = 0;
+    std::vector<void*> a_bufs;               // every allocation below, freed together
+    std::vector<void*> a_psi, a_tmp, a_ws;   // per group: one replica's field, export scratch, peak workspace
+    std::vector<int64_t*> a_idx;             // per group: peak indices (capacity n/8 + 8)
+    std::vector<int*> a_ijk;                 // per group: peak coordinates
+    int64_t* a_counts = nullptr;             // [nrep]
+    unsigned long long* a_hist = nullptr;    // [nrep][2][nbins]: q4 then q6
 };
 
 static size_t elem_size(int prec) { return prec == 64 ? 16 : 8; }
@@ -266,9 +283,34 @@ static int ens_steps(mcs_ensemble* e, int nsteps, int mode, hipStream_t s) {
     return e->prec == 64 ? ens_steps_t<double>(e, nsteps, mode, s) : ens_steps_t<float>(e, nsteps, mode, s);
 }
 
+int knn_hist_launch(const int64_t* d_idx, const int64_t* d_count, int64_t cap, const int ext[3], double h, int k,
+                    int* d_ijk, unsigned long long* d_h4, unsigned long long* d_h6, int nbins, hipStream_t s);
+
+static int64_t peak_cap(const mcs_replica& p) { return (int64_t)p.ext[0] * p.ext[1] * p.ext[2] / 8 + 8; }
+
+// analysis of the members of group gi on stream b: export, peaks >= ths, kNN q4/q6 histograms
+static int ens_analysis_group(mcs_ensemble* e, int gi, hipStream_t b) {
+    EnsGroup* g = e->groups[gi];
+    const int nb = e->a_nbins;
+    for (int r : g->members) {
+        const mcs_replica& p = e->reps[r];
+        unsigned long long* h = e->a_hist + 2L * nb * r;
+        hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(256), 0, b, h, 2L * nb);
+        MCS_HIP_TRY(hipGetLastError());
+        int st = step_export(g->sg, e->U[r], e->a_psi[gi], e->a_tmp[gi], b);
+        if (!st) st = findpeaks_launch(e->a_psi[gi], p.ext, e->a_ths, e->a_idx[gi], nullptr, peak_cap(p),
+                                       e->a_counts + r, e->a_ws[gi], e->prec, b);
+        if (!st) st = knn_hist_launch(e->a_idx[gi], e->a_counts + r, peak_cap(p), p.ext, p.h[0], e->a_k, e->a_ijk[gi],
+                                      h, h + nb, nb, b);
+        if (st) return st;
+    }
+    return MCS_OK;
+}
+
 // GRAPH_STEPS steps with every size group on its own branch: fork from s, each
-// group runs its steps in order, join, then advance the draw counter
-template <typename F> static int ens_branches_t(mcs_ensemble* e, int nsteps, hipStream_t s) {
+// group runs its steps in order (then, with `analysis`, its replicas' analysis),
+// join, then advance the draw counter
+template <typename F> static int ens_branches_t(mcs_ensemble* e, int nsteps, hipStream_t s, bool analysis) {
     const int ng = (int)e->groups.size();
     int st = MCS_OK;
     if (hipEventRecord(e->side_ev[0], s) != hipSuccess) return MCS_ERR_HIP;
@@ -277,6 +319,7 @@ template <typename F> static int ens_branches_t(mcs_ensemble* e, int nsteps, hip
         if (hipStreamWaitEvent(b, e->side_ev[0], 0) != hipSuccess) return MCS_ERR_HIP;
         EnsGroup* g = e->groups[gi];
         for (int it = 0; it < nsteps && !st; ++it) st = ens_step_group<F>(e, g, 0, (int)g->members.size(), it, b);
+        if (!st && analysis) st = ens_analysis_group(e, gi, b);
         if (!st && hipEventRecord(e->side_ev[1 + gi], b) != hipSuccess) st = MCS_ERR_HIP;
         if (!st && hipStreamWaitEvent(s, e->side_ev[1 + gi], 0) != hipSuccess) st = MCS_ERR_HIP;
     }
@@ -286,8 +329,7 @@ template <typename F> static int ens_branches_t(mcs_ensemble* e, int nsteps, hip
 
 static constexpr int GRAPH_STEPS = 8;
 
-static int ens_graph_build(mcs_ensemble* e, hipStream_t s) {
-    if (e->gexec) return MCS_OK;
+static int ens_capture(mcs_ensemble* e, hipStream_t s, bool analysis, hipGraph_t* graph, hipGraphExec_t* exec) {
     const int ng = (int)e->groups.size();
     while ((int)e->side.size() < ng) {
         hipStream_t b = nullptr;
@@ -300,20 +342,78 @@ static int ens_graph_build(mcs_ensemble* e, hipStream_t s) {
         e->side_ev.push_back(ev);
     }
     MCS_HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    int st = e->prec == 64 ? ens_branches_t<double>(e, GRAPH_STEPS, s) : ens_branches_t<float>(e, GRAPH_STEPS, s);
+    int st = e->prec == 64 ? ens_branches_t<double>(e, GRAPH_STEPS, s, analysis)
+                           : ens_branches_t<float>(e, GRAPH_STEPS, s, analysis);
     hipGraph_t g = nullptr;
     const hipError_t ce = hipStreamEndCapture(s, &g);
     if (st) { if (g) (void)hipGraphDestroy(g); return st; }
     MCS_HIP_TRY(ce);
-    MCS_HIP_TRY(hipGraphInstantiate(&e->gexec, g, nullptr, nullptr, 0));
-    e->graph = g;
-    e->gsteps = GRAPH_STEPS;
+    MCS_HIP_TRY(hipGraphInstantiate(exec, g, nullptr, nullptr, 0));
+    *graph = g;
     return MCS_OK;
+}
+
+static int ens_graph_build(mcs_ensemble* e, hipStream_t s) {
+    if (e->gexec) return MCS_OK;
+    int st = ens_capture(e, s, false, &e->graph, &e->gexec);
+    if (!st) e->gsteps = GRAPH_STEPS;
+    return st;
+}
+
+static void ens_analysis_free(mcs_ensemble* e) {
+    if (e->aexec) (void)hipGraphExecDestroy(e->aexec);
+    if (e->agraph) (void)hipGraphDestroy(e->agraph);
+    e->aexec = nullptr;
+    e->agraph = nullptr;
+    for (void* p : e->a_bufs) (void)hipFree(p);
+    e->a_bufs.clear();
+    e->a_psi.clear(); e->a_tmp.clear(); e->a_ws.clear(); e->a_idx.clear(); e->a_ijk.clear();
+    e->a_counts = nullptr;
+    e->a_hist = nullptr;
+    e->a_k = e->a_nbins = 0;
+}
+
+// buffers and the steps+analysis graph for (ths, k, nbins); rebuilt when they change
+static int ens_analysis_build(mcs_ensemble* e, double ths, int k, int nbins, hipStream_t s) {
+    if (e->aexec && e->a_ths == ths && e->a_k == k && e->a_nbins == nbins) return MCS_OK;
+    ens_analysis_free(e);
+    auto dalloc = [&](size_t b) -> void* {
+        void* p = nullptr;
+        if (hipMalloc(&p, b ? b : 1) != hipSuccess) return nullptr;
+        e->a_bufs.push_back(p);
+        return p;
+    };
+    const size_t fb = e->prec == 64 ? 8 : 4;
+    bool ok = true;
+    for (EnsGroup* g : e->groups) {
+        const mcs_replica& p = e->reps[g->members[0]];
+        const size_t n = (size_t)p.ext[0] * p.ext[1] * p.ext[2];
+        e->a_psi.push_back(dalloc(n * fb));
+        e->a_tmp.push_back(dalloc(n * fb));
+        e->a_ws.push_back(dalloc(peaks_workspace_bytes(p.ext)));
+        e->a_idx.push_back((int64_t*)dalloc(peak_cap(p) * sizeof(int64_t)));
+        e->a_ijk.push_back((int*)dalloc(3 * peak_cap(p) * sizeof(int)));
+        ok = ok && e->a_psi.back() && e->a_tmp.back() && e->a_ws.back() && e->a_idx.back() && e->a_ijk.back();
+    }
+    e->a_counts = (int64_t*)dalloc(e->reps.size() * sizeof(int64_t));
+    e->a_hist = (unsigned long long*)dalloc(2 * (size_t)nbins * e->reps.size() * sizeof(unsigned long long));
+    if (!ok || !e->a_counts || !e->a_hist) {
+        ens_analysis_free(e);
+        mcs_set_last_error(__FILE__, __LINE__, "analysis allocation failed");
+        return MCS_ERR_NOMEM;
+    }
+    e->a_ths = ths;
+    e->a_k = k;
+    e->a_nbins = nbins;
+    int st = ens_capture(e, s, true, &e->agraph, &e->aexec);
+    if (st) ens_analysis_free(e);
+    return st;
 }
 
 // ------------------------------------------------------------------ C ABI
 extern "C" int mcs_ensemble_destroy(mcs_ensemble* e) {
     if (!e) return MCS_OK;
+    ens_analysis_free(e);
     if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
     if (e->graph) (void)hipGraphDestroy(e->graph);
     for (hipStream_t b : e->side) (void)hipStreamDestroy(b);
@@ -459,8 +559,43 @@ extern "C" int mcs_ensemble_step(mcs_ensemble* e, int nsteps, int mode) {
     return MCS_OK;
 }
 
-int knn_hist_launch(const int64_t* d_idx, const int64_t* d_count, int64_t cap, const int ext[3], double h, int k,
-                    int* d_ijk, unsigned long long* d_h4, unsigned long long* d_h6, int nbins, hipStream_t s);
+extern "C" int mcs_ensemble_run(mcs_ensemble* e, int nsteps, double ths, int k, int nbins, int64_t* counts,
+                                int64_t* hist_q4, int64_t* hist_q6) {
+    MCS_CHECK(e && counts && hist_q4 && hist_q6 && nbins > 0 && k >= 1 && k <= 32, MCS_ERR_INVALID, "bad argument");
+    MCS_CHECK(nsteps >= GRAPH_STEPS, MCS_ERR_INVALID, "mcs_ensemble_run takes at least 8 steps");
+    MCS_CHECK(e->loaded, MCS_ERR_INVALID, "ensemble has no fields (call mcs_ensemble_init)");
+    for (const mcs_replica& p : e->reps)
+        MCS_CHECK(p.ext[2] > 1, MCS_ERR_UNSUPPORTED, "bond-order analysis needs 3D replicas");
+    hipStream_t s = mcs_stream();
+    int rc = ens_graph_build(e, s);
+    if (!rc) rc = ens_analysis_build(e, ths, k, nbins, s);
+    if (rc) return rc;
+    // the remainder eagerly first, then graph replays; the last replay carries the analysis
+    const int rem = nsteps % GRAPH_STEPS;
+    if (rem) rc = ens_steps(e, rem, MCS_ENS_BATCHED, s);
+    for (int i = 0; !rc && i < nsteps / GRAPH_STEPS - 1; ++i)
+        if (hipGraphLaunch(e->gexec, s) != hipSuccess) rc = MCS_ERR_HIP;
+    if (!rc && hipGraphLaunch(e->aexec, s) != hipSuccess) rc = MCS_ERR_HIP;
+    const int nrep = (int)e->reps.size();
+    std::vector<unsigned long long> hh(2 * (size_t)nbins * nrep);
+    if (!rc && hipMemcpyAsync(hh.data(), e->a_hist, hh.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                              s) != hipSuccess)
+        rc = MCS_ERR_HIP;
+    if (!rc && hipMemcpyAsync(counts, e->a_counts, nrep * sizeof(int64_t), hipMemcpyDeviceToHost, s) != hipSuccess)
+        rc = MCS_ERR_HIP;
+    if (!rc && hipStreamSynchronize(s) != hipSuccess) rc = MCS_ERR_HIP;
+    if (rc) {
+        mcs_set_last_error(__FILE__, __LINE__, "HIP failure in mcs_ensemble_run");
+        return rc;
+    }
+    e->step += nsteps;
+    for (int r = 0; r < nrep; ++r)
+        for (int b = 0; b < nbins; ++b) {
+            hist_q4[(long)r * nbins + b] = (int64_t)hh[2L * nbins * r + b];
+            hist_q6[(long)r * nbins + b] = (int64_t)hh[2L * nbins * r + nbins + b];
+        }
+    return MCS_OK;
+}
 
 extern "C" int mcs_ensemble_analyze(mcs_ensemble* e, double ths, int k, int nbins, int64_t* counts, int64_t* hist_q4,
                                     int64_t* hist_q6) {

@@ -157,6 +157,75 @@ __global__ void __launch_bounds__(256) k_row_x(const void* srcv, void* dstv, lon
     }
 }
 
+// x pass of the boundary transform for even nx on the register engine: G rows of
+// L = nx/2 packed points per workgroup, S = L/P threads per row.
+//   R2C: packed FFT in registers, then X_k = E_k + w^k O_k with the mirror Z_{L-k}
+//        read back through LDS; k = L (Nyquist) comes from k = 0 with w^L = -1
+//   C2R: Z_k = (X_k + X*_{L-k}) + i (X_k - X*_{L-k}) w^-k straight from global
+//        memory (the mirror element lies in the same row), inverse FFT in registers
+template <typename F, int P, bool WAVE>
+__global__ void __launch_bounds__(256) k_row_r2c_reg(const C2<F>* __restrict__ src, C2<F>* __restrict__ dst,
+                                                     long nrows, int G, FftPlanDev pl,
+                                                     const C2<F>* __restrict__ tw, const C2<F>* __restrict__ wph) {
+    extern __shared__ unsigned char smem_raw[];
+    C2<F>* lds = reinterpret_cast<C2<F>*>(smem_raw);
+    const int L = pl.L, S = L / P;
+    const int t = threadIdx.x % S, g = threadIdx.x / S;
+    const long row = (long)blockIdx.x * G + g;
+    const bool valid = row < nrows && g < G;
+    const int base = (g < G ? g : 0) * L;
+    C2<F> a[P];
+#pragma unroll
+    for (int m = 0; m < P; ++m) a[m] = valid ? src[row * L + t + m * S] : mk<F>(0, 0);
+    reg_fft<F, P, -1, WAVE>(a, pl, tw, t, lds, base, 1);
+    lds_sync<WAVE>();
+#pragma unroll
+    for (int m = 0; m < P; ++m) lds[base + t + m * S] = a[m];
+    lds_sync<WAVE>();
+    if (!valid) return;
+    C2<F>* out = dst + row * (L + 1);
+#pragma unroll
+    for (int m = 0; m < P; ++m) {
+        const int k = t + m * S;
+        const C2<F> zk = a[m];
+        const C2<F> zp = cconj(lds[base + (k == 0 ? 0 : L - k)]);
+        const C2<F> e = cscale(cadd(zk, zp), F(0.5));
+        const C2<F> o = cscale(cmuli<-1>(csub(zk, zp)), F(0.5));
+        out[k] = cadd(e, cmul(wph[k], o));
+        if (k == 0) out[L] = csub(e, o);
+    }
+}
+
+template <typename F, int P, bool WAVE>
+__global__ void __launch_bounds__(256) k_row_c2r_reg(const C2<F>* __restrict__ src, C2<F>* __restrict__ dst,
+                                                     long nrows, int G, FftPlanDev pl,
+                                                     const C2<F>* __restrict__ tw, const C2<F>* __restrict__ wph) {
+    extern __shared__ unsigned char smem_raw[];
+    C2<F>* lds = reinterpret_cast<C2<F>*>(smem_raw);
+    const int L = pl.L, S = L / P;
+    const int t = threadIdx.x % S, g = threadIdx.x / S;
+    const long row = (long)blockIdx.x * G + g;
+    const bool valid = row < nrows && g < G;
+    const int base = (g < G ? g : 0) * L;
+    const C2<F>* in = src + (valid ? row : 0) * (L + 1);
+    C2<F> a[P];
+#pragma unroll
+    for (int m = 0; m < P; ++m) {
+        const int k = t + m * S;
+        C2<F> xk = valid ? in[k] : mk<F>(0, 0);
+        C2<F> xp = valid ? in[L - k] : mk<F>(0, 0);
+        if (k == 0) { xk.y = F(0); xp.y = F(0); }
+        const C2<F> sm = cadd(xk, cconj(xp));
+        const C2<F> d = cmulc(csub(xk, cconj(xp)), wph[k]);
+        a[m] = cadd(sm, cmuli<+1>(d));
+    }
+    reg_fft<F, P, +1, WAVE>(a, pl, tw, t, lds, base, 1);
+    if (valid) {
+#pragma unroll
+        for (int m = 0; m < P; ++m) dst[row * L + t + m * S] = a[m];
+    }
+}
+
 // ------------------------------------------------------------------ launchers
 static int set_lds(const void* fn, size_t bytes) {
     if (bytes > 65536) MCS_HIP_TRY(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
@@ -234,9 +303,38 @@ static int launch_row_t(const LinePlan& lp, const void* wph, const void* src, vo
     return MCS_OK;
 }
 
+template <typename F, int P>
+static int launch_row_reg_t(int mode, const LinePlan& lp, const void* wph, const void* src, void* dst, long nrows,
+                            hipStream_t s) {
+    const int S = lp.L / P;
+    int G = S >= 256 ? 1 : 256 / S;
+    const size_t esz = sizeof(C2<F>);
+    while (G > 1 && (size_t)G * lp.L * esz > 65536) --G;
+    const size_t lds = (size_t)G * lp.L * esz;
+    const bool wave = 64 % S == 0;
+    auto fn = mode == 0 ? (wave ? k_row_r2c_reg<F, P, true> : k_row_r2c_reg<F, P, false>)
+                        : (wave ? k_row_c2r_reg<F, P, true> : k_row_c2r_reg<F, P, false>);
+    int st = set_lds((const void*)fn, lds);
+    if (st) return st;
+    hipLaunchKernelGGL(fn, dim3((unsigned)((nrows + G - 1) / G)), dim3(G * S), lds, s, (const C2<F>*)src, (C2<F>*)dst,
+                       nrows, G, lp.dev, (const C2<F>*)lp.tw, (const C2<F>*)wph);
+    MCS_HIP_TRY(hipGetLastError());
+    return MCS_OK;
+}
+
 template <typename F>
 static int launch_row_f(int mode, const LinePlan& lp, const void* wph, const void* src, void* dst, long nrows, int nx,
                         hipStream_t s) {
+    if (lp.engine == ENGINE_REG && (mode == 0 || mode == 2)) {
+        switch (lp.P) {
+            case 4: return launch_row_reg_t<F, 4>(mode, lp, wph, src, dst, nrows, s);
+            case 8: return launch_row_reg_t<F, 8>(mode, lp, wph, src, dst, nrows, s);
+            case 12: return launch_row_reg_t<F, 12>(mode, lp, wph, src, dst, nrows, s);
+            case 16: return launch_row_reg_t<F, 16>(mode, lp, wph, src, dst, nrows, s);
+            case 20: return launch_row_reg_t<F, 20>(mode, lp, wph, src, dst, nrows, s);
+            default: break;
+        }
+    }
     switch (mode) {
         case 0: return launch_row_t<F, 0>(lp, wph, src, dst, nrows, nx, s);
         case 1: return launch_row_t<F, 1>(lp, wph, src, dst, nrows, nx, s);
@@ -267,7 +365,21 @@ extern "C" int mcs_fft_plan_create(mcs_fft_plan** plan, const int ext[3], int pr
     p->dir = dir;
     const int nx = ext[0];
     const bool even = nx % 2 == 0;
-    int st = line_plan_build(&p->px, even ? nx / 2 : nx, prec, false);
+    // even nx: the packed row of nx/2 points on the register engine when its length allows
+    // (16 points per thread for powers of two, else the default register plan up to 20, at
+    // most 256 threads per row), otherwise the generic engine
+    int st = MCS_OK;
+    bool reg = false;
+    if (even && line_plan_build_reg(&p->px, nx / 2, prec, 16) == MCS_OK) reg = true;
+    if (even && !reg && line_plan_build(&p->px, nx / 2, prec, true, 8) == MCS_OK) {
+        reg = p->px.engine == ENGINE_REG && p->px.P <= 20;
+        if (!reg) line_plan_free(&p->px);
+    }
+    if (reg && nx / 2 / p->px.P > 256) {
+        line_plan_free(&p->px);
+        reg = false;
+    }
+    if (!reg) st = line_plan_build(&p->px, even ? nx / 2 : nx, prec, false);
     if (!st && ext[1] > 1) st = line_plan_build(&p->py, ext[1], prec);
     if (!st && ext[2] > 1) st = line_plan_build(&p->pz, ext[2], prec);
     if (!st && even) st = make_phase_table(&p->wph, nx, nx / 2 + 1, prec);

@@ -120,9 +120,21 @@ MCS_DEV void xupd_pair(C2<F> nA, C2<F> nBr, C2<F> pA, C2<F> pBr, int p, int iy, 
     const F kx2 = F(ar.L - p) * ar.dkx;  // |k_x| of index p + nx/2 (the Nyquist index for p = 0)
     C2<F> xi1 = mk<F>(0, 0), xi2 = mk<F>(0, 0);
     if (ar.noise) {
-        xi1 = noise_xi<F>(p, iy, iz, ar.nx, ar.ny, ar.nz, ar.key0, ar.key1, draw);
-        xi2 = p == 0 ? noise_xi<F>(ar.L, iy, iz, ar.nx, ar.ny, ar.nz, ar.key0, ar.key1, draw)
-                     : cconj(noise_xi<F>(ar.L - p, my, mz, ar.nx, ar.ny, ar.nz, ar.key0, ar.key1, draw));
+        // one Philox block gives both modes; a self-mirrored row takes mode p + L as the
+        // conjugate of its partner L - p (a second block) and real modes at p = 0
+        const long rc = (long)iz * ar.ny + iy;
+        const XiPair<F> x = xi_block<F>((uint64_t)rc * ar.L + p, ar.key0, ar.key1, draw);
+        xi1 = x.a;
+        xi2 = x.b;
+        if (iy == my && iz == mz) {
+            const F s2 = F(1.41421356237309504880);
+            if (p == 0) {
+                xi1 = mk<F>(s2 * x.a.x, F(0));
+                xi2 = mk<F>(s2 * x.b.x, F(0));
+            } else {
+                xi2 = cconj(xi_block<F>((uint64_t)rc * ar.L + (ar.L - p), ar.key0, ar.key1, draw).a);
+            }
+        }
     }
     const C2<F> q1 = pfc_mode_update<F>(p1, n1, xi1, kx1 * kx1 + kyz, ar.dt, ar.lin0, ar.amp);
     const C2<F> q2 = pfc_mode_update<F>(p2, n2, xi2, kx2 * kx2 + kyz, ar.dt, ar.lin0, ar.amp);

@@ -11,6 +11,7 @@
 #define MCS_TAG_KNOISE 0u
 #define MCS_TAG_INIT 1u
 #define MCS_TAG_REAL 2u
+#define MCS_TAG_KROW 3u
 
 struct u32x4 { uint32_t x, y, z, w; };
 
@@ -73,4 +74,41 @@ __device__ __noinline__ C2<F> noise_xi(int i0, int i1, int i2, int nx, int ny, i
         return cconj(raw_xi<F>(mlin * nxc + i0, k0, k1, draw, &z0));
     }
     return raw_xi<F>(((uint64_t)i2 * ny + i1) * nxc + i0, k0, k1, draw, &z0);
+}
+
+// Unit complex Gaussian from two 32-bit draws (Box-Muller); *z0 = r cos(theta), a unit
+// normal for real modes.  In FP32 the radius uses the hardware base-2 logarithm and
+// square root (1 ulp each); the angle keeps the library sincospi, since the hardware
+// sine and cosine (about 1e-6 absolute) made FP32 and FP64 noisy runs drift apart
+// ten times faster.
+template <typename F> MCS_DEV C2<F> box_muller(uint32_t x, uint32_t y, F* z0) {
+    const F u1 = u01_24<F>(x), u2 = u01_24<F>(y);
+    F r, sn, cs;
+    if constexpr (sizeof(F) == 4) {
+        r = __builtin_amdgcn_sqrtf(-1.3862943611198906f * __builtin_amdgcn_logf(u1));  // -2 ln u1
+    } else {
+        r = sqrt(-2.0 * log(u1));
+    }
+    mcs_sincospi(F(2) * u2, &sn, &cs);
+    *z0 = r * cs;
+    const F q = F(0.70710678118654752440);
+    return mk<F>(r * cs * q, r * sn * q);
+}
+
+// The fused stepper's noise layout (even nx, L = nx/2): the Philox block at counter
+// (c*L + p, draw, KROW) of row c gives .a for the full x mode p of that row (from x, y)
+// and .b for the full x mode p + L (from z, w).  Of a mirror pair of rows the smaller
+// index draws and the other conjugates; in a self-mirrored row, 0 < p < L draws a at p
+// and the modes 0 and L are real: sqrt(2) Re a and sqrt(2) Re b (= r cos theta).
+template <typename F> struct XiPair {
+    C2<F> a, b;
+};
+template <typename F>
+__device__ __noinline__ XiPair<F> xi_block(uint64_t c, uint32_t k0, uint32_t k1, uint32_t draw) {
+    const u32x4 o = philox4x32_10(u32x4{(uint32_t)c, (uint32_t)(c >> 32), draw, MCS_TAG_KROW}, k0, k1);
+    F z;
+    XiPair<F> r;
+    r.a = box_muller<F>(o.x, o.y, &z);
+    r.b = box_muller<F>(o.z, o.w, &z);
+    return r;
 }

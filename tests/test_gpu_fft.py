@@ -57,6 +57,18 @@ def test_pfc_lengths_against_oracle(mcs, orc, shp):
     assert np.abs(mcs.rfftn(x32) - orc.r2c(x32)).max() / np.sqrt(x.size) < 1e-5
 
 
+@pytest.mark.parametrize("dt,tol", [(np.float32, 1e-5), (np.float64, 2e-14)])
+@pytest.mark.parametrize("shp", [(24,), (40,), (4096,), (8192,), (6, 96), (5, 3, 160), (2, 2048), (3, 12)])
+def test_register_row_pass_lengths(mcs, shp, dt, tol):
+    # even nx whose half length is 2^a, 3*2^a or 5*2^a takes the register-engine x pass
+    x = np.random.default_rng(len(shp) + shp[-1]).standard_normal(shp).astype(dt)
+    k = mcs.rfftn(x)
+    want = np.fft.rfftn(x.astype(np.float64))
+    assert np.abs(want - k).max() / np.sqrt(x.size) < tol
+    y = mcs.irfftn(want.astype(k.dtype), shp)
+    assert np.abs(y.astype(np.float64) / x.size - x).max() < tol * 4
+
+
 def test_c2r_non_hermitian_follows_numpy(mcs):
     rng = np.random.default_rng(4)
     for shp in [(4, 6, 8), (5, 9), (6, 7)]:

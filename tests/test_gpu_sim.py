@@ -1,0 +1,74 @@
+"""The simulator driver (mcs_sim): its fields match the oracle, its outputs follow the
+reference formats, and a run resumed from its checkpoint continues bit for bit."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(REPO, "mcstar-libs_amd", "bin")
+H = np.pi / 4
+
+
+def sim(prec, *args):
+    exe = os.path.join(BIN, f"mcs_sim_{prec}")
+    if not os.path.exists(exe):
+        pytest.fail(f"{exe} is not built (make sim)")
+    r = subprocess.run([exe] + [str(a) for a in args], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    return r
+
+
+def test_fields_match_oracle_and_outputs_follow_reference_formats(mcs, orc, tmp_path):
+    out = str(tmp_path)
+    sim("f64", "--dims", "32x32x32", "--eps", 0.3, "--dt", 0.25, "--psi0", -0.2, "--amp", 0.1, "--seed", 7,
+        "--temp", 1e-3, "--noise-seed", 5, "--steps", 20, "--every", 10, "--out", out, "--ths", -1.0, "--rlim", 5.5)
+    psi = orc.init_field((32, 32, 32), -0.2, 0.1, 7)
+    mid = orc.pfc_run(psi, [H] * 3, 0.3, 0.25, 1e-3, 5, 0, 10)
+    end = orc.pfc_run(mid, [H] * 3, 0.3, 0.25, 1e-3, 5, 10, 10)
+    for step, want in ((10, mid), (20, end)):
+        got = np.load(os.path.join(out, f"psi_{step:08d}.npy"))
+        assert got.shape == (32, 32, 32) and got.dtype == np.float64
+        assert np.linalg.norm(got - want) / np.linalg.norm(want) < 1e-11
+    log = np.loadtxt(os.path.join(out, "log.txt"))
+    assert log.shape == (2, 5) and list(log[:, 0]) == [10, 20]
+    assert np.abs(log[:, 1] - psi.mean()).max() < 1e-15 and np.abs(log[:, 2] - psi.mean()).max() < 1e-13
+    # peaks: "%f %f %f %f" lines (i*h, field) of the cells >= ths, ordered by cell index
+    got = np.load(os.path.join(out, "psi_00000020.npy"))
+    idx = orc.findpeaks(got, -1.0)
+    assert idx.size > 12
+    pk = np.loadtxt(os.path.join(out, "peaks_00000020.txt"), ndmin=2)
+    assert pk.shape == (idx.size, 4)
+    want_xyz = np.column_stack([idx % 32, (idx // 32) % 32, idx // 1024]) * np.float32(H)
+    assert np.abs(pk[:, :3] - want_xyz).max() < 1e-5 and np.abs(pk[:, 3] - got.reshape(-1)[idx]).max() < 1e-5
+    bo = np.loadtxt(os.path.join(out, "bondorder_00000020.txt"), ndmin=2)
+    assert bo.shape[1] == 13 and bo.shape[0] <= idx.size
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_resume_continues_bit_for_bit(tmp_path, prec):
+    a, b = str(tmp_path / "a"), str(tmp_path / "b")
+    os.makedirs(a)
+    os.makedirs(b)
+    common = ["--dims", "32x24x16", "--eps", 0.3, "--dt", 0.25, "--temp", 1e-3, "--noise-seed", 3]
+    sim(prec, *common, "--steps", 16, "--out", a)
+    sim(prec, *common, "--steps", 7, "--out", b)
+    sim(prec, "--resume", b, "--steps", 9, "--out", b)
+    assert np.array_equal(np.load(os.path.join(a, "psi_00000016.npy")), np.load(os.path.join(b, "psi_00000016.npy")))
+    assert "step 16" in open(os.path.join(b, "checkpoint.txt")).read()
+
+
+def test_load_2d_and_bad_flags(mcs, orc, tmp_path):
+    out = str(tmp_path)
+    psi = orc.init_field((48, 64), -0.25, 0.05, 2)
+    np.save(os.path.join(out, "in.npy"), psi.astype(np.float32))
+    sim("f32", "--dims", "64x48", "--load", os.path.join(out, "in.npy"), "--eps", 0.25, "--dt", 0.5, "--steps", 30,
+        "--out", out)
+    got = np.load(os.path.join(out, "psi_00000030.npy"))
+    want = orc.pfc_run(psi.astype(np.float32).astype(np.float64), [H] * 3, 0.25, 0.5, 0.0, 0, 0, 30)
+    assert np.linalg.norm(got - want) / np.linalg.norm(want) < 1e-5
+    exe = os.path.join(BIN, "mcs_sim_f32")
+    r = subprocess.run([exe, "--dims", "32y32"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "not of form" in r.stderr

@@ -126,6 +126,43 @@ def test_kspace_noise_statistics(orc):
     assert abs(np.mean(inner * np.conj(other[:, :, 1:16].reshape(-1)))) < 0.02  # steps independent
 
 
+@pytest.mark.parametrize("shp", [(6, 8, 10), (4, 6, 12), (8, 12), (10, 16)])
+def test_stepper_noise_hermitian_and_full_rank(orc, shp):
+    xi = orc.stepper_noise(shp, seed=77, step=3)
+    nx = shp[-1]
+    full = shp if len(shp) == 3 else (1,) + shp
+    x3 = xi.reshape(full[0], full[1], -1)
+    nreal = 0
+    for i0 in (0, nx // 2):
+        for i2 in range(full[0]):
+            for i1 in range(full[1]):
+                a = x3[i2, i1, i0]
+                m2, m1 = _mirror(i2, full[0]), _mirror(i1, full[1])
+                assert a == np.conj(x3[m2, m1, i0])
+                if (m2, m1) == (i2, i1):
+                    assert a.imag == 0
+                    nreal += 1
+    r = orc.c2r(xi, shp)
+    assert np.abs(orc.r2c(r) / r.size - xi).max() < 1e-12
+    # full rank: every real degree of freedom of the grid gets its own normal, so the
+    # magnitudes are distinct except for Hermitian partners (a reused Philox output would repeat one)
+    n = int(np.prod(shp))
+    assert np.unique(np.round(np.abs(xi), 12)).size == (n - nreal) // 2 + nreal
+
+
+def test_stepper_noise_statistics(orc):
+    xi = orc.stepper_noise((32, 32, 32), seed=5, step=0)
+    inner = xi[:, :, 1:16].reshape(-1)
+    assert abs(np.mean(np.abs(xi) ** 2) - 1.0) < 0.01
+    assert abs(np.mean(inner.real)) < 0.01 and abs(np.mean(inner.imag)) < 0.01
+    assert abs(np.mean(inner.real ** 2) - 0.5) < 0.01
+    assert abs(np.mean(inner.real * inner.imag)) < 0.01
+    other = orc.stepper_noise((32, 32, 32), seed=5, step=1)
+    assert abs(np.mean(inner * np.conj(other[:, :, 1:16].reshape(-1)))) < 0.02
+    # independent of the boundary layout's stream (a different Philox tag)
+    assert abs(np.mean(inner * np.conj(orc.kspace_noise((32, 32, 32), 5, 0)[:, :, 1:16].reshape(-1)))) < 0.02
+
+
 # ---------------------------------------------------------------- initial field
 def test_init_field_from_philox(orc):
     shp = (4, 5, 6)
