@@ -36,12 +36,12 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def make_stepper(mcs, n, dtype, cfg):
+def make_stepper(mcs, n, dtype, cfg, plan=None):
     shp = (n, n, n)
     buf = mcs.DeviceBuffer(n ** 3 * np.dtype(dtype).itemsize)
     prec = 64 if dtype == np.float64 else 32
     mcs.check(mcs.lib().mcs_init_field(buf.ptr, mcs._ext(shp), cfg["psi0"], cfg["amp"], cfg["seed"], prec))
-    st = mcs.Stepper(shp, [cfg["dx"]] * 3, cfg["eps"], cfg["dt"], temp=0.0, seed=0, dtype=dtype)
+    st = mcs.Stepper(shp, [cfg["dx"]] * 3, cfg["eps"], cfg["dt"], temp=0.0, seed=0, dtype=dtype, plan=plan)
     st.set_psi_device(buf)
     return st, buf
 
@@ -75,14 +75,12 @@ def measured_traffic(pass_name, n):
     """HBM bytes per launch of a pass from the committed rocprofv3 --pmc summary (512^3 FP32 only)"""
     if n != 512:
         return None
-    kern = {"x_update": "`k_xupd<", "z_nonlinear": "`k_col_nl<", "y_inverse": "`k_col_reg<float, 32, 1>",
-            "y_forward": "`k_col_reg<float, 32, -1>"}[pass_name]
     path = os.path.join(REPO, "profiles", "pfc_step_512cubed_fp32_hbm_traffic.md")
     if not os.path.exists(path):
         return None
     for line in open(path):
         cols = [c.strip() for c in line.split("|")]
-        if len(cols) > 6 and cols[1].startswith(kern):
+        if len(cols) > 7 and cols[1] == pass_name:
             return float(cols[5].split()[0]) * 1e9
     return None
 
@@ -144,6 +142,9 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--only-ensemble", action="store_true", help="run only the ensemble lines (profiling)")
     ap.add_argument("--only-fft", action="store_true", help="run only the boundary FFT lines (profiling)")
+    ap.add_argument("--plan", default=None,
+                    help="explicit FP32 plans Px,Gx,Py,Gy,Pz,Gz instead of the tuned ones (profiling without "
+                         "the tuner's trial launches)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a 1 GiB device-to-device copy (reference dispatch for HBM counters)")
     a = ap.parse_args()
@@ -168,7 +169,8 @@ def main():
     cfg = dict(eps=0.3, psi0=-0.25, amp=0.05, seed=11, dx=np.pi / 4, dt=0.5)
     N = n ** 3
     log(f"[bench] fp32 stepper {n}^3: warmup {a.warmup}")
-    st, _ = make_stepper(mcs, n, np.float32, cfg)
+    plan = [int(v) for v in a.plan.split(",")] if a.plan else None
+    st, _ = make_stepper(mcs, n, np.float32, cfg, plan)
     st.step(a.warmup)
     log(f"[bench] fp32 timed {a.steps} steps")
     ms = timed_steps(mcs, st, a.steps)

@@ -12,16 +12,28 @@
 #include <vector>
 
 // ------------------------------------------------------------------ kernels
+// Workgroups are dealt to the 8 XCDs round-robin (workgroup b runs on XCD b % 8,
+// each XCD with its own L2).  With swz, workgroup b takes logical tile
+// xcd_tile(b): every XCD gets a contiguous run of tiles, so the cache lines two
+// neighbouring tiles share (rows whose length is not a multiple of the tile, as
+// the nx/2+1 half spectrum) are fetched once into one L2 instead of into two.
+MCS_DEV unsigned xcd_tile(unsigned b, unsigned nb, bool swz) {
+    if (!swz) return b;
+    const unsigned q = nb / 8, r = nb % 8, k = b % 8;
+    return k * q + (k < r ? k : r) + b / 8;
+}
+
 // Column pass, register engine.  src/dst may alias (each workgroup reads its
 // whole tile before it writes).
 template <typename F, int P, int DIR>
 __global__ void __launch_bounds__(MCS_REG_MAX_THREADS(P, F)) k_col_reg(const C2<F>* src, C2<F>* dst, ColGeom g, int G, FftPlanDev pl,
-                                                  const C2<F>* __restrict__ tw) {
+                                                  const C2<F>* __restrict__ tw, int swz) {
     extern __shared__ unsigned char smem_raw[];
     C2<F>* lds = reinterpret_cast<C2<F>*>(smem_raw);
     const int S = pl.L / P;
     const int tilesx = (g.nxc + G - 1) / G;
-    const int tile = blockIdx.x % tilesx, outer = blockIdx.x / tilesx;
+    const int blk = (int)xcd_tile(blockIdx.x, gridDim.x, swz != 0);
+    const int tile = blk % tilesx, outer = blk / tilesx;
     const int c = threadIdx.x % G, t = threadIdx.x / G;
     const int x = tile * G + c;
     const bool valid = x < g.nxc;
@@ -39,13 +51,14 @@ __global__ void __launch_bounds__(MCS_REG_MAX_THREADS(P, F)) k_col_reg(const C2<
 // Column pass, generic engine.
 template <typename F, int DIR>
 __global__ void __launch_bounds__(256) k_col_gen(const C2<F>* src, C2<F>* dst, ColGeom g, int G, int Lp,
-                                                 FftPlanDev pl, const C2<F>* __restrict__ tw) {
+                                                 FftPlanDev pl, const C2<F>* __restrict__ tw, int swz) {
     extern __shared__ unsigned char smem_raw[];
     C2<F>* b0 = reinterpret_cast<C2<F>*>(smem_raw);
     C2<F>* b1 = b0 + (long)G * Lp;
     const int L = pl.L;
     const int tilesx = (g.nxc + G - 1) / G;
-    const int tile = blockIdx.x % tilesx, outer = blockIdx.x / tilesx;
+    const int blk = (int)xcd_tile(blockIdx.x, gridDim.x, swz != 0);
+    const int tile = blk % tilesx, outer = blk / tilesx;
     const long base = (long)outer * g.os + (long)tile * G;
     for (int idx = threadIdx.x; idx < G * L; idx += blockDim.x) {
         const int c = idx % G, pos = idx / G;
@@ -242,8 +255,10 @@ static int launch_col_reg_t(const LinePlan& lp, const void* src, void* dst, cons
     int st = set_lds((const void*)fn, lds);
     if (st) return st;
     const long tiles = (long)((g.nxc + G - 1) / G) * g.nouter;
+    // swizzle when tiles share cache lines: rows not a whole number of 128-byte tiles
+    const int swz = (g.nxc % G != 0 || (g.nxc * esz) % 128 != 0) ? 1 : 0;
     hipLaunchKernelGGL(fn, dim3((unsigned)tiles), dim3(G * S), lds, s, (const C2<F>*)src, (C2<F>*)dst, g, G, lp.dev,
-                       (const C2<F>*)lp.tw);
+                       (const C2<F>*)lp.tw, swz);
     MCS_HIP_TRY(hipGetLastError());
     return MCS_OK;
 }
@@ -259,8 +274,9 @@ static int launch_col_gen_t(const LinePlan& lp, const void* src, void* dst, cons
     int st = set_lds((const void*)fn, lds);
     if (st) return st;
     const long tiles = (long)((g.nxc + G - 1) / G) * g.nouter;
+    const int swz = (g.nxc % G != 0 || (g.nxc * esz) % 128 != 0) ? 1 : 0;
     hipLaunchKernelGGL(fn, dim3((unsigned)tiles), dim3(256), lds, s, (const C2<F>*)src, (C2<F>*)dst, g, G, Lp, lp.dev,
-                       (const C2<F>*)lp.tw);
+                       (const C2<F>*)lp.tw, swz);
     MCS_HIP_TRY(hipGetLastError());
     return MCS_OK;
 }
