@@ -45,9 +45,9 @@ $(OBJDIR)/%.o: $(PKG)/csrc/%.hip $(HDR)
 	@mkdir -p $(OBJDIR)
 	$(HIPCC) $(HIPFLAGS) -c $< -o $@
 
-$(LIB): $(OBJ)
+$(LIB): $(OBJ) $(PKG)/csrc/exports.map
 	@mkdir -p $(dir $@)
-	$(HIPCC) -shared -fPIC --offload-arch=$(ARCH) -o $@.tmp $(OBJ)
+	$(HIPCC) -shared -fPIC --offload-arch=$(ARCH) -Wl,--version-script=$(PKG)/csrc/exports.map -o $@.tmp $(OBJ)
 	mv $@.tmp $@
 
 oracle:

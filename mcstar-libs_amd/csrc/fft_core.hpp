@@ -116,6 +116,28 @@ constexpr double ce_cos2pi(long j, long n) {
     for (int k = 1; k < 30; ++k) { term *= -x * x / ((2.0 * k - 1.0) * (2.0 * k)); sum += term; }
     return sum;
 }
+// t * exp(DIR * 2 pi i j / N).  Called from fully unrolled loops, so j is a constant and the
+// branches fold: multiples of N/4 become sign changes and swaps, odd multiples of N/8 two
+// adds and two multiplies, the rest a full product with constants from the FP64 series above
+// (IEEE arithmetic cannot drop a product with an exact 0 or 1 by itself)
+template <typename F, int DIR, int N> MCS_DEV C2<F> twiddle_n(C2<F> t, int j) {
+    j %= N;
+    if (j == 0) return t;
+    if ((4 * j) % N == 0) {
+        const int q = 4 * j / N;  // quarter turns
+        if (q == 2) return mk<F>(-t.x, -t.y);
+        return q == 1 ? cmuli<DIR>(t) : cmuli<-DIR>(t);
+    }
+    if ((8 * j) % N == 0) {
+        const int e = 8 * j / N;  // odd eighth turns: w = (c + i s) / sqrt(2)
+        const F c = (e == 1 || e == 7) ? F(1) : F(-1);
+        const F sn = F(DIR) * ((e == 1 || e == 3) ? F(1) : F(-1));
+        const F r = F(0.70710678118654752440);
+        return mk<F>(r * (c * t.x - sn * t.y), r * (sn * t.x + c * t.y));
+    }
+    return cmul(t, mk<F>((F)ce_cos2pi(j, N), (F)(DIR * ce_cos2pi(j - N / 4, N))));
+}
+
 // DFT of size N = N1 * N2 from N2 DFTs of size N1 (stride N2), twiddles W_N^{n2 k1}, and N1 DFTs of size N2:
 //   X[k1 + N1 k2] = sum_{n2} W_N2^{n2 k2} W_N^{n2 k1} sum_{n1} W_N1^{n1 k1} x[N2 n1 + n2]
 template <typename F, int N1, int N2, int DIR, typename D1, typename D2>
@@ -129,12 +151,7 @@ MCS_DEV void dft_split(C2<F>* v, D1 inner, D2 outer) {
         for (int n1 = 0; n1 < N1; ++n1) t[n1] = v[N2 * n1 + n2];
         inner(t);
 #pragma unroll
-        for (int k1 = 0; k1 < N1; ++k1) {
-            const int j = n2 * k1;
-            if (j == 0) { y[n2 * N1 + k1] = t[k1]; continue; }
-            const C2<F> w = mk<F>((F)ce_cos2pi(j, N), (F)(DIR * ce_cos2pi(j - N / 4, N)));
-            y[n2 * N1 + k1] = cmul(t[k1], w);
-        }
+        for (int k1 = 0; k1 < N1; ++k1) y[n2 * N1 + k1] = twiddle_n<F, DIR, N>(t[k1], n2 * k1);
     }
 #pragma unroll
     for (int k1 = 0; k1 < N1; ++k1) {
