@@ -28,6 +28,24 @@ inline void call_kernel_update_pfc(GPUArray& kpsi, GPUArray& knl, Float dt, Floa
                          split, Array::prec_of<Float>()));
 }
 
+// call_kernel_update_eom_with_fen (cuda/kernels/eom.hpp:103-109): the update, and knoise <- lk kpsi
+// (the linear operator of the free energy on the old kpsi); C2R of knoise then feeds calc_fen
+inline void call_kernel_update_eom_with_fen(GPUArray& kpsi, GPUArray& knonlin, GPUArray& knoise, Float namp, Float dt,
+                                            Float eps, Float3 lens) {
+    const double l[3] = {(double)lens.x, (double)lens.y, (double)lens.z};
+    CUERR(mcs_update_eom_with_fen(kpsi.ptr_void(), knonlin.ptr_void(), knoise.ptr_void(), kpsi.ext_ptr(),
+                                  (double)namp, (double)dt, (double)eps, l, Array::prec_of<Float>()));
+}
+
+// call_kernel_calc_fen (eom.hpp:130-141): per-cell free energy written into lpsi, the total
+// (times the cell volume) returned as float like the reference; the sum itself is FP64
+inline float call_kernel_calc_fen(GPUArray& psi, GPUArray& lpsi, Float3 hh) {
+    const double h[3] = {(double)hh.x, (double)hh.y, (double)hh.z};
+    double f = 0;
+    CUERR(mcs_calc_fen(psi.ptr_void(), lpsi.ptr_void(), psi.ext_ptr(), h, Array::prec_of<Float>(), &f));
+    return (float)f;
+}
+
 // FP64 free energy of a real field: dV sum[psi/2 (-eps + (1+lap)^2) psi + psi^4/4]
 inline double calc_free_energy(GPUArray& psi, Float3 hh, Float eps) {
     const double h[3] = {(double)hh.x, (double)hh.y, (double)hh.z};

@@ -122,6 +122,21 @@ int mcs_init_field(void* real, const int ext[3], double psi0, double amp, uint64
 int mcs_field_stats(const void* real, const int ext[3], const double h[3], double eps, int prec,
                     double* mean, double* free_energy);
 
+/* call_kernel_update_eom_with_fen (cuda/kernels/eom.hpp:88-109): the update_eom step in
+ * that kernel's operation order, and knoise <- lk * kpsi (the kpsi before the update) with
+ * lk = 2 - eps - 2 k^2 + k^4, the linear operator of the free energy.  knoise is read (noise)
+ * and overwritten.  namp is scaled by sqrtf(N) as in update_eom. */
+int mcs_update_eom_with_fen(void* kpsi, const void* knonlin, void* knoise, const int ext[3], double namp,
+                            double dt, double eps, const double lens[3], int prec);
+/* call_kernel_calc_fen (eom.hpp:111-141): per cell f = psi/2 (lpsi - psi + psi^3/2), written into
+ * lpsi; *fen = h0 h1 h2 * sum f, summed in FP64 in a fixed order (the reference used FP32 atomics). */
+int mcs_calc_fen(const void* psi, void* lpsi, const int ext[3], const double h[3], int prec, double* fen);
+/* call_kernel_update_el (cuda/kernels/el.hpp:3-45): non-conserved relaxation
+ * kpsi <- (kpsi - dt knl) / (1 + dt (2 - eps - 2 k^2 + k^4)); *maxdiff = max over the half
+ * spectrum of |kpsi_new - kpsi_old| (the convergence measure). */
+int mcs_update_el(void* kpsi, const void* knonlin, const int ext[3], double dt, double eps, const double lens[3],
+                  int prec, double* maxdiff);
+
 /* ---------------------------------------------------------------- stepper
  * The fused PFC step: the state lives in a packed spectrum (x pairs packed
  * into one complex value), each step is four passes in 3D (two in 2D):

@@ -74,6 +74,10 @@ _SIGS = {
                            _vp], ctypes.c_int),
     "mcs_psi3_split": ([_vp, _vp, ctypes.c_double, ctypes.c_int, _ip, ctypes.c_int], ctypes.c_int),
     "mcs_norm": ([_vp, _ip, ctypes.c_int], ctypes.c_int),
+    "mcs_update_eom_with_fen": ([_vp, _vp, _vp, _ip, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp,
+                                 ctypes.c_int], ctypes.c_int),
+    "mcs_calc_fen": ([_vp, _vp, _ip, _dp, ctypes.c_int, _dp], ctypes.c_int),
+    "mcs_update_el": ([_vp, _vp, _ip, ctypes.c_double, ctypes.c_double, _dp, ctypes.c_int, _dp], ctypes.c_int),
     "mcs_update_eom": ([_vp, _vp, _vp, _ip, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp,
                         ctypes.c_int], ctypes.c_int),
     "mcs_update_pfc": ([_vp, _vp, _ip, _dp, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_uint64,
@@ -266,6 +270,38 @@ def update_eom(kpsi, knl, knoise, shape_zyx, namp, dt, eps, lens):
     check(lib().mcs_update_eom(dk.ptr, dn.ptr, dz.ptr if dz else None, _ext(shape_zyx), namp, dt, eps, ln, prec),
           "mcs_update_eom")
     return dk.download(kpsi.shape, kpsi.dtype)
+
+
+def update_eom_with_fen(kpsi, knl, knoise, shape_zyx, namp, dt, eps, lens):
+    """returns (updated kpsi, lk * old kpsi) (call_kernel_update_eom_with_fen)"""
+    prec = _prec(kpsi.dtype)
+    nb = buffer_bytes(shape_zyx, np.float64 if prec == 64 else np.float32)
+    dk, dn, dz = to_device(kpsi, nb), to_device(knl, nb), to_device(knoise, nb)
+    ln = (ctypes.c_double * 3)(*lens)
+    check(lib().mcs_update_eom_with_fen(dk.ptr, dn.ptr, dz.ptr, _ext(shape_zyx), namp, dt, eps, ln, prec),
+          "mcs_update_eom_with_fen")
+    return dk.download(kpsi.shape, kpsi.dtype), dz.download(knoise.shape, knoise.dtype)
+
+
+def calc_fen(psi, lpsi, h):
+    """returns (total free energy, per-cell density) (call_kernel_calc_fen)"""
+    psi = np.ascontiguousarray(psi)
+    d1, d2 = to_device(psi), to_device(np.ascontiguousarray(lpsi, dtype=psi.dtype))
+    f = ctypes.c_double()
+    check(lib().mcs_calc_fen(d1.ptr, d2.ptr, _ext(psi.shape), (ctypes.c_double * 3)(*h), _prec(psi.dtype),
+                             ctypes.byref(f)), "mcs_calc_fen")
+    return f.value, d2.download(psi.shape, psi.dtype)
+
+
+def update_el(kpsi, knl, shape_zyx, lens, dt, eps):
+    """returns (updated kpsi, max change of a mode) (call_kernel_update_el)"""
+    prec = _prec(kpsi.dtype)
+    nb = buffer_bytes(shape_zyx, np.float64 if prec == 64 else np.float32)
+    dk, dn = to_device(kpsi, nb), to_device(knl, nb)
+    m = ctypes.c_double()
+    check(lib().mcs_update_el(dk.ptr, dn.ptr, _ext(shape_zyx), dt, eps, (ctypes.c_double * 3)(*lens), prec,
+                              ctypes.byref(m)), "mcs_update_el")
+    return dk.download(kpsi.shape, kpsi.dtype), m.value
 
 
 def update_pfc(kpsi, knl, shape_zyx, lens, dt, eps, temp, seed, draw, split):

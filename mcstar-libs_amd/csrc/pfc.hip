@@ -14,6 +14,7 @@
 #include <vector>
 
 static constexpr int BS = 256;
+static constexpr int RED_BLOCKS_EL = 1024;  // blocks of the relaxation update (their maxima are reduced after)
 
 static inline unsigned grid_for(long n, int per_thread = 1) {
     long g = (n + (long)BS * per_thread - 1) / ((long)BS * per_thread);
@@ -320,6 +321,127 @@ extern "C" int mcs_philox_blocks(uint64_t seed, uint64_t c0, uint32_t c2, uint32
     return MCS_OK;
 }
 
+// ------------------------------------------------------------------ update with the free-energy operator
+// eom.hpp:88-109 (kernel_update_eom_with_fen) in its own operation order:
+//   lk = (2 - eps) + 2 dop.x + dop.y, denom = 1 - dt dop.x lk,
+//   kpsi <- (kpsi + dt dop.x knl + dt namp k knoise) / denom, and knoise <- lk * kpsi (the old
+//   kpsi): the linear operator of the free energy applied to psi, for mcs_calc_fen after a C2R
+template <typename F>
+__global__ void __launch_bounds__(BS) k_update_eom_fen(C2<F>* kpsi, const C2<F>* knl, C2<F>* knoise, int nx, int ny,
+                                                       int nz, F namp, F dt, F eps, F lx, F ly, F lz) {
+    const int nxc = nx / 2 + 1;
+    const long total = (long)nxc * ny * nz;
+    for (long idx = (long)blockIdx.x * BS + threadIdx.x; idx < total; idx += (long)gridDim.x * BS) {
+        const int i0 = (int)(idx % nxc);
+        const long r = idx / nxc;
+        const int i1 = (int)(r % ny), i2 = (int)(r / ny);
+        const F k0 = kwave<F>(i0, nx, lx), k1 = kwave<F>(i1, ny, ly), k2 = kwave<F>(i2, nz, lz);
+        const F dx = -k0 * k0 - k1 * k1 - k2 * k2;
+        const F dy = dx * dx;
+        const F k = sqrt(-dx);
+        const F lk = (F(2) - eps) + F(2) * dx + dy;
+        const F denom = F(1) - dt * dx * lk;
+        const C2<F> p = kpsi[idx], q = knl[idx], z = knoise[idx];
+        kpsi[idx] = mk<F>((p.x + dt * dx * q.x + dt * namp * k * z.x) / denom,
+                          (p.y + dt * dx * q.y + dt * namp * k * z.y) / denom);
+        knoise[idx] = mk<F>(lk * p.x, lk * p.y);
+    }
+}
+
+extern "C" int mcs_update_eom_with_fen(void* kpsi, const void* knonlin, void* knoise, const int ext[3], double namp,
+                                       double dt, double eps, const double lens[3], int prec) {
+    MCS_CHECK(kpsi && knonlin && knoise && lens && ext_ok(ext), MCS_ERR_INVALID, "bad argument");
+    MCS_CHECK(prec == 32 || prec == 64, MCS_ERR_INVALID, "precision must be 32 or 64");
+    const long n = (long)ext[0] * ext[1] * ext[2];
+    const long nc = (long)(ext[0] / 2 + 1) * ext[1] * ext[2];
+    hipStream_t s = mcs_stream();
+    if (prec == 64) {
+        const double na = namp * (double)sqrtf((float)n);  // eom.hpp:90 scales by sqrtf(N)
+        hipLaunchKernelGGL(k_update_eom_fen<double>, dim3(grid_for(nc)), dim3(BS), 0, s, (double2*)kpsi,
+                           (const double2*)knonlin, (double2*)knoise, ext[0], ext[1], ext[2], na, dt, eps, lens[0],
+                           lens[1], lens[2]);
+    } else {
+        const float na = (float)namp * sqrtf((float)n);
+        hipLaunchKernelGGL(k_update_eom_fen<float>, dim3(grid_for(nc)), dim3(BS), 0, s, (float2*)kpsi,
+                           (const float2*)knonlin, (float2*)knoise, ext[0], ext[1], ext[2], na, (float)dt, (float)eps,
+                           (float)lens[0], (float)lens[1], (float)lens[2]);
+    }
+    MCS_HIP_TRY(hipGetLastError());
+    MCS_HIP_TRY(hipStreamSynchronize(s));
+    return MCS_OK;
+}
+
+// ------------------------------------------------------------------ non-conserved relaxation
+// el.hpp:3-45 (kernel_update_el, call_kernel_update_el): kpsi' = (kpsi - dt knl) / (1 + dt ((2 - eps) + 2 dop.x + dop.y)) and, per block,
+// the largest |kpsi' - kpsi|; the block maxima are reduced by one more block (an exact max)
+template <typename F>
+__global__ void __launch_bounds__(BS) k_update_el(C2<F>* kpsi, const C2<F>* knl, int nx, int ny, int nz, F dt, F eps,
+                                                  F lx, F ly, F lz, double* part) {
+    __shared__ double m[BS];
+    const int nxc = nx / 2 + 1;
+    const long total = (long)nxc * ny * nz;
+    double mx = 0;
+    for (long idx = (long)blockIdx.x * BS + threadIdx.x; idx < total; idx += (long)gridDim.x * BS) {
+        const int i0 = (int)(idx % nxc);
+        const long r = idx / nxc;
+        const int i1 = (int)(r % ny), i2 = (int)(r / ny);
+        const F k0 = kwave<F>(i0, nx, lx), k1 = kwave<F>(i1, ny, ly), k2 = kwave<F>(i2, nz, lz);
+        const F dx = -k0 * k0 - k1 * k1 - k2 * k2;
+        const F dy = dx * dx;
+        const F denom = F(1) + dt * ((F(2) - eps) + F(2) * dx + dy);
+        const C2<F> p = kpsi[idx], q = knl[idx];
+        const C2<F> pn = mk<F>((p.x - dt * q.x) / denom, (p.y - dt * q.y) / denom);
+        kpsi[idx] = pn;
+        const F ex = p.x - pn.x, ey = p.y - pn.y;
+        const F d = sqrt(ex * ex + ey * ey);
+        mx = (double)d > mx ? (double)d : mx;
+    }
+    m[threadIdx.x] = mx;
+    __syncthreads();
+    for (int w = BS / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) m[threadIdx.x] = m[threadIdx.x] > m[threadIdx.x + w] ? m[threadIdx.x] : m[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = m[0];
+}
+
+__global__ void __launch_bounds__(BS) k_max_partials(const double* part, int n, double* out) {
+    __shared__ double m[BS];
+    double mx = 0;
+    for (int i = threadIdx.x; i < n; i += BS) mx = part[i] > mx ? part[i] : mx;
+    m[threadIdx.x] = mx;
+    __syncthreads();
+    for (int w = BS / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) m[threadIdx.x] = m[threadIdx.x] > m[threadIdx.x + w] ? m[threadIdx.x] : m[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = m[0];
+}
+
+extern "C" int mcs_update_el(void* kpsi, const void* knonlin, const int ext[3], double dt, double eps,
+                             const double lens[3], int prec, double* maxdiff) {
+    MCS_CHECK(kpsi && knonlin && lens && maxdiff && ext_ok(ext), MCS_ERR_INVALID, "bad argument");
+    MCS_CHECK(prec == 32 || prec == 64, MCS_ERR_INVALID, "precision must be 32 or 64");
+    hipStream_t s = mcs_stream();
+    double* part = nullptr;
+    MCS_HIP_TRY(hipMalloc((void**)&part, sizeof(double) * (RED_BLOCKS_EL + 1)));
+    if (prec == 64)
+        hipLaunchKernelGGL(k_update_el<double>, dim3(RED_BLOCKS_EL), dim3(BS), 0, s, (double2*)kpsi,
+                           (const double2*)knonlin, ext[0], ext[1], ext[2], dt, eps, lens[0], lens[1], lens[2], part);
+    else
+        hipLaunchKernelGGL(k_update_el<float>, dim3(RED_BLOCKS_EL), dim3(BS), 0, s, (float2*)kpsi,
+                           (const float2*)knonlin, ext[0], ext[1], ext[2], (float)dt, (float)eps, (float)lens[0],
+                           (float)lens[1], (float)lens[2], part);
+    hipLaunchKernelGGL(k_max_partials, dim3(1), dim3(BS), 0, s, part, RED_BLOCKS_EL, part + RED_BLOCKS_EL);
+    int st = hipGetLastError() == hipSuccess ? MCS_OK : MCS_ERR_HIP;
+    if (!st && hipMemcpyAsync(maxdiff, part + RED_BLOCKS_EL, sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess)
+        st = MCS_ERR_HIP;
+    if (!st && hipStreamSynchronize(s) != hipSuccess) st = MCS_ERR_HIP;
+    (void)hipFree(part);
+    if (st) mcs_set_last_error(__FILE__, __LINE__, "HIP failure in mcs_update_el");
+    return st;
+}
+
 // ------------------------------------------------------------------ FP64 reductions
 // Deterministic: a fixed grid of per-block LDS tree sums, then one block sums the partials in order.
 static constexpr int RED_BLOCKS = 1024;
@@ -435,5 +557,55 @@ extern "C" int mcs_field_stats(const void* real, const int ext[3], const double 
         const double dV = h[0] * (ext[1] > 1 ? h[1] : 1.0) * (ext[2] > 1 ? h[2] : 1.0);
         *free_energy = dV * (0.5 * hres[2] / (double)n + 0.25 * hres[1]);
     }
+    return MCS_OK;
+}
+
+// eom.hpp:111-141 (kernel_calc_fen, call_kernel_calc_fen): per cell f = 0.5 psi (lpsi - psi + 0.5 psi^3), written into
+// lpsi as the reference does, and F = dV sum f.  The sum is a deterministic FP64 reduction (the
+// reference accumulated FP32 atomics); f itself is formed in double and stored in the field's type.
+template <typename F>
+__global__ void __launch_bounds__(BS) k_calc_fen(const F* psi, F* lpsi, long n, double* part) {
+    __shared__ double s[BS];
+    double acc = 0;
+    for (long i = (long)blockIdx.x * BS + threadIdx.x; i < n; i += (long)gridDim.x * BS) {
+        const F p = psi[i];
+        const F f = (F)(0.5 * p * (lpsi[i] - p + 0.5 * p * p * p));
+        lpsi[i] = f;
+        acc += (double)f;
+    }
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = BS / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = s[0];
+}
+
+extern "C" int mcs_calc_fen(const void* psi, void* lpsi, const int ext[3], const double h[3], int prec, double* fen) {
+    MCS_CHECK(psi && lpsi && h && fen && ext_ok(ext), MCS_ERR_INVALID, "bad argument");
+    MCS_CHECK(prec == 32 || prec == 64, MCS_ERR_INVALID, "precision must be 32 or 64");
+    hipStream_t s = mcs_stream();
+    const long n = (long)ext[0] * ext[1] * ext[2];
+    double* part = nullptr;
+    MCS_HIP_TRY(hipMalloc((void**)&part, sizeof(double) * (RED_BLOCKS + 1)));
+    if (prec == 64)
+        hipLaunchKernelGGL(k_calc_fen<double>, dim3(RED_BLOCKS), dim3(BS), 0, s, (const double*)psi, (double*)lpsi, n,
+                           part);
+    else
+        hipLaunchKernelGGL(k_calc_fen<float>, dim3(RED_BLOCKS), dim3(BS), 0, s, (const float*)psi, (float*)lpsi, n,
+                           part);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(BS), 0, s, part, RED_BLOCKS, 1, part + RED_BLOCKS);
+    double sum = 0;
+    int st = hipGetLastError() == hipSuccess ? MCS_OK : MCS_ERR_HIP;
+    if (!st && hipMemcpyAsync(&sum, part + RED_BLOCKS, sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess)
+        st = MCS_ERR_HIP;
+    if (!st && hipStreamSynchronize(s) != hipSuccess) st = MCS_ERR_HIP;
+    (void)hipFree(part);
+    if (st) {
+        mcs_set_last_error(__FILE__, __LINE__, "HIP failure in mcs_calc_fen");
+        return st;
+    }
+    *fen = sum * h[0] * h[1] * h[2];  // eom.hpp:137 multiplies by hh.x hh.y hh.z
     return MCS_OK;
 }

@@ -388,6 +388,66 @@ static void hermitian_planes(cplx* k, const int n[3]) {
     }
 }
 
+/* cuda/kernels/eom.hpp:88-101 (kernel_update_eom_with_fen), in FP64: the reference split's
+   update written with lk = (2 - eps) + 2 dop.x + dop.y and denom = 1 - dt dop.x lk (dop.x = -k^2,
+   dop.y = k^4), noise namp*sqrt(N)*|k|*xi, and knoise <- lk * kpsi (the kpsi before the update). */
+void orc_update_eom_fen(double* kpsid, const double* knld, double* knoised, const int n[3], const double lens[3],
+                        double namp, double dt, double eps) {
+    const int nx = n[0], ny = n[1], nz = n[2], nxc = nx / 2 + 1;
+    const double na = namp * sqrt((double)nx * ny * nz);
+    cplx* kpsi = (cplx*)kpsid;
+    const cplx* knl = (const cplx*)knld;
+    cplx* knoise = (cplx*)knoised;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int i2 = 0; i2 < nz; i2++)
+        for (int i1 = 0; i1 < ny; i1++)
+            for (int i0 = 0; i0 < nxc; i0++) {
+                const long idx = ((long)i2 * ny + i1) * nxc + i0;
+                const double k0 = kval(i0, nx, lens[0]), k1 = kval(i1, ny, lens[1]), k2 = kval(i2, nz, lens[2]);
+                const double dx = -(k0 * k0 + k1 * k1 + k2 * k2), dy = dx * dx;
+                const double lk = (2.0 - eps) + 2.0 * dx + dy;
+                const double denom = 1.0 - dt * dx * lk;
+                const cplx p = kpsi[idx];
+                kpsi[idx] = (p + dt * dx * knl[idx] + dt * na * sqrt(-dx) * knoise[idx]) / denom;
+                knoise[idx] = lk * p;
+            }
+}
+
+/* cuda/kernels/eom.hpp:111-141 (kernel_calc_fen): f = 0.5 psi (lpsi - psi + 0.5 psi^3) per cell,
+   written into lpsi; returns h0 h1 h2 * sum f. */
+double orc_calc_fen(const double* psi, double* lpsi, const int n[3], const double h[3]) {
+    const long N = (long)n[0] * n[1] * n[2];
+    double sum = 0;
+    for (long c = 0; c < N; c++) {
+        const double p = psi[c];
+        lpsi[c] = 0.5 * p * (lpsi[c] - p + 0.5 * p * p * p);
+        sum += lpsi[c];
+    }
+    return sum * h[0] * h[1] * h[2];
+}
+
+/* cuda/kernels/el.hpp:3-45 (kernel_update_el): non-conserved relaxation
+   kpsi' = (kpsi - dt knl) / (1 + dt ((2 - eps) + 2 dop.x + dop.y)); returns max |kpsi' - kpsi|. */
+double orc_update_el(double* kpsid, const double* knld, const int n[3], const double lens[3], double dt, double eps) {
+    const int nx = n[0], ny = n[1], nz = n[2], nxc = nx / 2 + 1;
+    cplx* kpsi = (cplx*)kpsid;
+    const cplx* knl = (const cplx*)knld;
+    double mx = 0;
+    for (int i2 = 0; i2 < nz; i2++)
+        for (int i1 = 0; i1 < ny; i1++)
+            for (int i0 = 0; i0 < nxc; i0++) {
+                const long idx = ((long)i2 * ny + i1) * nxc + i0;
+                const double k0 = kval(i0, nx, lens[0]), k1 = kval(i1, ny, lens[1]), k2 = kval(i2, nz, lens[2]);
+                const double dx = -(k0 * k0 + k1 * k1 + k2 * k2), dy = dx * dx;
+                const double denom = 1.0 + dt * ((2.0 - eps) + 2.0 * dx + dy);
+                const cplx pn = (kpsi[idx] - dt * knl[idx]) / denom;
+                const double d = cabs(kpsi[idx] - pn);
+                if (d > mx) mx = d;
+                kpsi[idx] = pn;
+            }
+    return mx;
+}
+
 int orc_pfc_run(double* psi, const int n[3], const double h[3], double eps, double dt, double temp,
                 uint64_t seed, int step0, int nsteps, int split, double* mass_hist) {
     const int nx = n[0], ny = n[1], nz = n[2], nxc = nx / 2 + 1;

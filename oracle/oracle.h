@@ -58,6 +58,13 @@ void orc_update(double* kpsi, const double* knl, const double* xi, const int n[3
    mass_hist (optional, nsteps+1 entries) receives the mean density after each step. */
 int orc_pfc_run(double* psi, const int n[3], const double h[3], double eps, double dt, double temp,
                 uint64_t seed, int step0, int nsteps, int split, double* mass_hist);
+/* update with the free-energy operator (eom.hpp:88-101): kpsi updated, knoise <- lk * old kpsi */
+void orc_update_eom_fen(double* kpsi, const double* knl, double* knoise, const int n[3], const double lens[3],
+                        double namp, double dt, double eps);
+/* per-cell free energy 0.5 psi (lpsi - psi + 0.5 psi^3) into lpsi; returns dV * sum (eom.hpp:111-141) */
+double orc_calc_fen(const double* psi, double* lpsi, const int n[3], const double h[3]);
+/* non-conserved relaxation (el.hpp:3-45); returns max |kpsi_new - kpsi_old| */
+double orc_update_el(double* kpsi, const double* knl, const int n[3], const double lens[3], double dt, double eps);
 /* free energy F = dV * sum[ psi/2 * (-eps + (1-k^2)^2) psi + psi^4/4 ] (cuda/kernels/eom.hpp:111-141) */
 double orc_free_energy(const double* psi, const int n[3], const double h[3], double eps);
 

@@ -46,6 +46,11 @@ def lib():
                                  ctypes.c_int]
         L.orc_pfc_run.argtypes = [_d, _i, _d, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                   ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_int, _d]
+        L.orc_update_eom_fen.argtypes = [_d, _d, _d, _i, _d, ctypes.c_double, ctypes.c_double, ctypes.c_double]
+        L.orc_calc_fen.argtypes = [_d, _d, _i, _d]
+        L.orc_calc_fen.restype = ctypes.c_double
+        L.orc_update_el.argtypes = [_d, _d, _i, _d, ctypes.c_double, ctypes.c_double]
+        L.orc_update_el.restype = ctypes.c_double
         L.orc_free_energy.argtypes = [_d, _i, _d, ctypes.c_double]
         L.orc_free_energy.restype = ctypes.c_double
         L.orc_findpeaks.argtypes = [_d, _i, ctypes.c_double, _i64, ctypes.c_int64]
@@ -143,6 +148,31 @@ def pfc_run(psi, h, eps, dt, temp, seed, step0, nsteps, split=SPLIT_NORTHSTAR, w
     lib().orc_pfc_run(_p(p), _p(_dims(p.shape), _i), _p(hh), eps, dt, temp, seed, step0, nsteps, split,
                       None if mass is None else _p(mass))
     return (p, mass) if want_mass else p
+
+
+def update_eom_fen(kpsi, knl, knoise, shape_zyx, lens, namp, dt, eps):
+    """returns (updated kpsi, lk * old kpsi)"""
+    kp = np.ascontiguousarray(kpsi, dtype=np.complex128).copy()
+    kn = np.ascontiguousarray(knoise, dtype=np.complex128).copy()
+    lib().orc_update_eom_fen(_p(kp), _p(np.ascontiguousarray(knl, dtype=np.complex128)), _p(kn),
+                             _p(_dims(shape_zyx), _i), _p(np.asarray(lens, dtype=np.float64)), namp, dt, eps)
+    return kp, kn
+
+
+def calc_fen(psi, lpsi, h):
+    """returns (total, per-cell density)"""
+    lp = np.ascontiguousarray(lpsi, dtype=np.float64).copy()
+    f = lib().orc_calc_fen(_p(np.ascontiguousarray(psi, dtype=np.float64)), _p(lp), _p(_dims(np.shape(psi)), _i),
+                           _p(np.asarray(h, dtype=np.float64)))
+    return f, lp
+
+
+def update_el(kpsi, knl, shape_zyx, lens, dt, eps):
+    """returns (updated kpsi, max change)"""
+    kp = np.ascontiguousarray(kpsi, dtype=np.complex128).copy()
+    m = lib().orc_update_el(_p(kp), _p(np.ascontiguousarray(knl, dtype=np.complex128)), _p(_dims(shape_zyx), _i),
+                            _p(np.asarray(lens, dtype=np.float64)), dt, eps)
+    return kp, m
 
 
 def free_energy(psi, h, eps):

@@ -8,11 +8,15 @@
 //   pfc   n0 n1 n2 in.npy out.npy steps eps dt namp seed h
 //         c2r -> psi3_split(norm) -> r2c -> fill_kspace -> update_eom, per step
 //   peaks n0 n1 n2 in.npy peaks.txt bo.txt h ths rlim   (findpeaks.hpp:31-63)
+//   fen   n0 n1 n2 in.npy eps h        update_eom_with_fen -> c2r -> norm -> calc_fen
+//   relax n0 n1 n2 in.npy out.npy steps eps dt   c2r -> psi3_split(norm) -> r2c -> update_el
 #include <cuda/array_abstraction.hpp>
 #include <cuda/cufft_wrapper.hpp>
 #include <cuda/math.hpp>
 #include <cuda/noise.hpp>
 
+#include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <iostream>
 #include <string>
@@ -24,6 +28,7 @@ typedef Array::GPUArray<Float> GPUArray;
 typedef FFT::CufftPlan<Float, FFT::R2C> PlanR2C;
 typedef FFT::CufftPlan<Float, FFT::C2R> PlanC2R;
 
+#include <cuda/kernels/el.hpp>
 #include <cuda/kernels/eom.hpp>
 #include <cuda/kernels/findpeaks.hpp>
 #include <cuda/kernels/norm.hpp>
@@ -89,6 +94,53 @@ int main(int argc, char* argv[]) {
             } else {
                 call_kernel_update_eom(kpsi, knl, namp, dt, eps, lens);
             }
+        }
+        c2r.execute(kpsi, psi);
+        call_kernel_norm(psi);
+        psi >> host;
+        host.save(argv[6]);
+        return 0;
+    }
+    if (mode == "fen" && argc == 8) {
+        // free energy the reference way: update_eom_with_fen leaves lk * kpsi in knoise,
+        // C2R and norm give L psi, calc_fen sums f (cuda/kernels/eom.hpp:88-141)
+        const int3 shape = shape_arg(argv + 2);
+        const Float eps = (Float)atof(argv[6]);
+        const double h = atof(argv[7]);
+        CPUArray host(shape);
+        host.load(argv[5]);
+        GPUArray psi(host), kpsi(shape), knl(shape), knoise(shape), lpsi(shape);
+        PlanR2C r2c(shape);
+        PlanC2R c2r(shape);
+        const Float3 lens{(Float)(shape.x * h), (Float)(shape.y * h), (Float)(shape.z * h)};
+        r2c.execute(psi, kpsi);
+        CUERR(mcs_memset(knl.ptr_void(), 0, knl.bytes()));
+        CUERR(mcs_memset(knoise.ptr_void(), 0, knoise.bytes()));
+        call_kernel_update_eom_with_fen(kpsi, knl, knoise, Float(0), Float(0), eps, lens);
+        c2r.execute(knoise, lpsi);
+        call_kernel_norm(lpsi);
+        const Float3 hh{(Float)h, (Float)h, (Float)h};
+        printf("%.17g\n", (double)call_kernel_calc_fen(psi, lpsi, hh));
+        return 0;
+    }
+    if (mode == "relax" && argc == 10) {
+        // non-conserved relaxation to a stationary state (cuda/kernels/el.hpp:27-45)
+        const int3 shape = shape_arg(argv + 2);
+        const int steps = atoi(argv[7]);
+        const Float eps = (Float)atof(argv[8]), dt = (Float)atof(argv[9]);
+        const double h = M_PI / 4;
+        CPUArray host(shape);
+        host.load(argv[5]);
+        GPUArray psi(host), kpsi(shape), kscratch(shape), knl(shape);
+        PlanR2C r2c(shape);
+        PlanC2R c2r(shape);
+        const Float3 lens{(Float)(shape.x * h), (Float)(shape.y * h), (Float)(shape.z * h)};
+        r2c.execute(psi, kpsi);
+        for (int s = 0; s < steps; ++s) {
+            c2r.execute(kpsi, psi);
+            call_kernel_psi3_split(psi, NULL, Float(0), true);
+            r2c.execute(psi, knl);
+            printf("%.17g\n", (double)call_kernel_update_el(kpsi, knl, dt, eps, lens));
         }
         c2r.execute(kpsi, psi);
         call_kernel_norm(psi);

@@ -102,3 +102,42 @@ def test_peaks_and_bond_order_flow(tmp_path, orc):
     assert table.shape == want.shape
     assert np.array_equal(table[:, 3:5], want[:, 3:5])
     assert np.abs(table - want).max() <= 6e-7
+
+
+# call_kernel_calc_fen returns float in both builds (eom.hpp:130), so the FP64 flow is exact to float rounding
+@pytest.mark.parametrize("prec,dt,tol", [("f32", np.float32, 1e-5), ("f64", np.float64, 1.2e-7)])
+def test_free_energy_flow(orc, tmp_path, prec, dt, tol):
+    # update_eom_with_fen -> c2r -> norm -> calc_fen (eom.hpp:88-141) gives the free energy functional
+    shp, eps, h = (16, 24, 32), 0.25, 0.7
+    psi = orc.init_field(shp, -0.2, 0.2, 6).astype(dt)
+    fin = str(tmp_path / "in.npy")
+    np.save(fin, psi)
+    r = run(prec, "fen", shp[2], shp[1], shp[0], fin, eps, h)
+    got = float(r.stdout.split()[-1])
+    want = orc.free_energy(psi.astype(np.float64), [h] * 3, eps)
+    assert abs(got - want) < tol * abs(want)
+
+
+@pytest.mark.parametrize("prec,dt,tol", [("f32", np.float32, 1e-4), ("f64", np.float64, 1e-11)])
+def test_relaxation_flow(orc, tmp_path, prec, dt, tol):
+    # c2r -> psi3_split(norm) -> r2c -> update_el: the relaxation of el.hpp:27-45, its convergence measure
+    shp, eps, dtau, steps = (24, 32), 0.3, 0.2, 30
+    psi = orc.init_field(shp, -0.3, 0.2, 2).astype(dt)
+    fin, fout = str(tmp_path / "in.npy"), str(tmp_path / "out.npy")
+    np.save(fin, psi)
+    r = run(prec, "relax", shp[1], shp[0], 1, fin, fout, steps, eps, dtau)
+    maxes = np.array([float(v) for v in r.stdout.split()])
+    h = np.pi / 4
+    lens = [shp[1] * h, shp[0] * h, 1.0]
+    p = psi.astype(np.float64)
+    want_m = []
+    for _ in range(steps):
+        kp = np.fft.rfftn(p)
+        kp, m = orc.update_el(kp, np.fft.rfftn(p ** 3 - p), shp, lens, dtau, eps)
+        p = np.fft.irfftn(kp, s=shp, axes=(0, 1))
+        want_m.append(m)
+    assert maxes.size == steps
+    assert np.abs(maxes - want_m).max() < tol * max(want_m) * 10
+    got = np.load(fout)
+    assert np.abs(got - p).max() < tol * np.abs(p).max() * 10
+    assert maxes[-1] < maxes[0]

@@ -113,3 +113,37 @@ def test_field_stats_fp64_reductions(mcs, orc, shp):
     mean32, fen32 = mcs.field_stats(psi.astype(np.float32), h3, 0.25)
     assert abs(mean32 - psi.astype(np.float32).astype(np.float64).mean()) < 1e-15
     assert abs(fen32 - want) < 1e-5 * abs(want)
+
+
+@pytest.mark.parametrize("dt,tol", [(np.float64, 1e-13), (np.float32, 3e-6)])
+@pytest.mark.parametrize("shp", [(8, 12, 16), (24, 20)])
+def test_update_eom_with_fen_and_calc_fen(mcs, orc, shp, dt, tol):
+    kp, kn = _spectra(shp, dt)
+    xi = orc.kspace_noise(shp, 3, 0).astype(kp.dtype)
+    lens = [n * 0.7 for n in shp][::-1] + [1.0] * (3 - len(shp))
+    got_k, got_l = mcs.update_eom_with_fen(kp, kn, xi, shp, 0.01, 0.4, 0.25, lens)
+    # eom.hpp:90 scales namp by sqrtf(N) (single precision) in both builds
+    n = int(np.prod(shp))
+    namp_eff = 0.01 * float(np.sqrt(np.float32(n))) / np.sqrt(n)
+    want_k, want_l = orc.update_eom_fen(kp, kn, xi, shp, lens, namp_eff, 0.4, 0.25)
+    assert np.abs(got_k - want_k).max() < tol * np.abs(want_k).max()
+    assert np.abs(got_l - want_l).max() < tol * np.abs(want_l).max()
+    # the free-energy density of a field and its L psi, summed in FP64
+    h = [0.7, 0.7, 0.7]
+    psi = orc.init_field(shp, -0.2, 0.2, 5)
+    lpsi = orc.c2r(want_l, shp) / psi.size
+    f, dens = mcs.calc_fen(psi.astype(dt), lpsi.astype(dt), h)
+    wf, wd = orc.calc_fen(psi.astype(dt).astype(np.float64), lpsi.astype(dt).astype(np.float64), h)
+    assert abs(f - wf) < tol * 10 * np.abs(wd).sum() * np.prod(h)
+    assert np.abs(dens - wd).max() < tol * np.abs(wd).max() + 1e-30
+
+
+@pytest.mark.parametrize("dt,tol", [(np.float64, 1e-13), (np.float32, 3e-6)])
+def test_update_el_matches_oracle(mcs, orc, dt, tol):
+    shp = (16, 24, 32)
+    kp, kn = _spectra(shp, dt)
+    lens = [n * 0.7 for n in shp][::-1]
+    got, m = mcs.update_el(kp, kn, shp, lens, 0.2, 0.3)
+    want, wm = orc.update_el(kp, kn, shp, lens, 0.2, 0.3)
+    assert np.abs(got - want).max() < tol * np.abs(want).max()
+    assert abs(m - wm) < tol * wm

@@ -280,6 +280,64 @@ def test_free_energy_decreases_noise_free(orc):
     assert all(b < a for a, b in zip(f, f[1:]))
 
 
+def test_update_eom_fen_is_the_reference_split_update(orc):
+    shp, lens = (6, 8, 10), [6 * 0.7, 8 * 0.7, 10 * 0.6]
+    rng = np.random.default_rng(3)
+    kp = np.fft.rfftn(rng.standard_normal(shp))
+    kn = np.fft.rfftn(0.1 * rng.standard_normal(shp))
+    xi = orc.kspace_noise(shp, 5, 0)
+    namp, dt, eps = 0.02, 0.3, 0.2
+    got, lpsi = orc.update_eom_fen(kp, kn, xi, shp, lens[::-1], namp, dt, eps)
+    # the same step as the reference-split update with noise amplitude dt*namp*sqrt(N)
+    want = orc.update(kp, kn, xi, shp, lens[::-1], dt, eps, dt * namp * np.sqrt(np.prod(shp)), orc.SPLIT_REFERENCE)
+    assert np.abs(got - want).max() < 1e-13 * np.abs(want).max()
+    k2 = _k2(shp, [l / n for l, n in zip(lens, shp)])
+    want_l = ((1 - k2) ** 2 + 1 - eps) * kp
+    assert np.abs(lpsi - want_l).max() < 1e-13 * np.abs(want_l).max()
+
+
+def test_calc_fen_is_the_free_energy(orc):
+    # with lpsi = L psi (the real field of lk * psi_k), calc_fen is the free energy functional
+    h = [0.6, 0.7, 0.5]
+    shp = (8, 10, 12)
+    psi = orc.init_field(shp, -0.2, 0.2, 9)
+    eps = 0.25
+    k2 = _k2(shp, h[::-1])
+    lpsi = np.fft.irfftn(((1 - k2) ** 2 + 1 - eps) * np.fft.rfftn(psi), s=shp, axes=(0, 1, 2))
+    f, dens = orc.calc_fen(psi, lpsi, h)
+    assert abs(f - orc.free_energy(psi, h, eps)) < 1e-12 * abs(f)
+    assert np.allclose(dens, 0.5 * psi * (lpsi - psi + 0.5 * psi ** 3), rtol=1e-14, atol=1e-16)
+    # uniform field closed form (common/pfc.hpp:4-5 without mu)
+    u = np.full(shp, -0.3)
+    fu, _ = orc.calc_fen(u, (2 - eps) * u, h)
+    V = np.prod(shp) * np.prod(h)
+    assert abs(fu / V - (0.5 * (1 - eps) * 0.09 + 0.25 * 0.3 ** 4)) < 1e-14
+
+
+def test_update_el_relaxes(orc):
+    shp, h = (32, 32), [np.pi / 4, np.pi / 4, 1.0]
+    lens = [32 * h[0], 32 * h[1], 1.0]
+    # single mode, no nonlinearity: psi_k / denom and the change closed form
+    kp = np.zeros((32, 17), complex)
+    kp[3, 5] = 2.0 + 1.0j
+    k2 = (2 * np.pi * 3 / lens[1]) ** 2 + (2 * np.pi * 5 / lens[0]) ** 2
+    den = 1 + 0.1 * ((1 - k2) ** 2 + 1 - 0.3)
+    got, m = orc.update_el(kp, np.zeros_like(kp), shp, lens, 0.1, 0.3)
+    assert abs(got[3, 5] - kp[3, 5] / den) < 1e-15 and abs(m - abs(kp[3, 5]) * (1 - 1 / den)) < 1e-15
+    # with the nonlinearity psi^3 - psi the relaxation lowers F and the change shrinks
+    psi = orc.init_field(shp, -0.3, 0.2, 4)
+    f0, dm = orc.free_energy(psi, h, 0.3), []
+    fs = [f0]
+    for _ in range(40):
+        kp = np.fft.rfftn(psi)
+        got, m = orc.update_el(kp, np.fft.rfftn(psi ** 3 - psi), shp, lens, 0.1, 0.3)
+        psi = np.fft.irfftn(got, s=shp, axes=(0, 1))
+        fs.append(orc.free_energy(psi, h, 0.3))
+        dm.append(m)
+    assert all(b < a for a, b in zip(fs, fs[1:]))
+    assert dm[-1] < dm[0]
+
+
 # ---------------------------------------------------------------- peaks
 def _brute_peaks(v, ths):
     nd = v.ndim
