@@ -46,6 +46,27 @@ inline float call_kernel_calc_fen(GPUArray& psi, GPUArray& lpsi, Float3 hh) {
     return (float)f;
 }
 
+// call_kernel_update_eom_hpfcnoise (eom.hpp:47-58): 2D update with three noise arrays
+inline void call_kernel_update_eom_hpfcnoise(GPUArray& kpsi, GPUArray& knonlin, GPUArray& knoise1, GPUArray& knoise2,
+                                             GPUArray& knoise3, Float namp, Float dt, Float eps, Float3 lens) {
+    if (kpsi.real_vext().z != 1) {
+        std::cerr << "the hpfc noise update is 2D only" << std::endl;
+        std::exit(EXIT_FAILURE);
+    }
+    const double l[3] = {(double)lens.x, (double)lens.y, (double)lens.z};
+    CUERR(mcs_update_eom_hpfc(kpsi.ptr_void(), knonlin.ptr_void(), knoise1.ptr_void(), knoise2.ptr_void(),
+                              knoise3.ptr_void(), kpsi.ext_ptr(), (double)namp, (double)dt, (double)eps, l,
+                              Array::prec_of<Float>()));
+}
+
+// call_kernel_update_eom_py (eom.hpp:173-180): the polynomial linear operator
+inline void call_kernel_update_eom_py(GPUArray& kpsi, GPUArray& knonlin, GPUArray& knoise, Float namp, Float dt,
+                                      Float eps, Float3 lens) {
+    const double l[3] = {(double)lens.x, (double)lens.y, (double)lens.z};
+    CUERR(mcs_update_eom_poly(kpsi.ptr_void(), knonlin.ptr_void(), knoise.ptr_void(), kpsi.ext_ptr(), (double)namp,
+                              (double)dt, (double)eps, l, Array::prec_of<Float>()));
+}
+
 // FP64 free energy of a real field: dV sum[psi/2 (-eps + (1+lap)^2) psi + psi^4/4]
 inline double calc_free_energy(GPUArray& psi, Float3 hh, Float eps) {
     const double h[3] = {(double)hh.x, (double)hh.y, (double)hh.z};

@@ -137,6 +137,29 @@ int mcs_calc_fen(const void* psi, void* lpsi, const int ext[3], const double h[3
 int mcs_update_el(void* kpsi, const void* knonlin, const int ext[3], double dt, double eps, const double lens[3],
                   int prec, double* maxdiff);
 
+/* ---------------------------------------------------------------- alternative integrators
+ * Exponential time differencing (cuda/kernels/eom-exp.hpp, el-exp.hpp): the init calls fill
+ * a half-spectrum table kop of (E, S) pairs; conserved: E = exp(-k^2 L' dt),
+ * S = (E - 1) / (k^2 L') (-dt where the operator vanishes); non-conserved: E = exp(-L' dt),
+ * S = (E - 1) / L', with L' = 2 - eps - 2 k^2 + k^4.  expf in both precisions, as there.
+ *   conserved:     kpsi <- kpsi E - dop.x knl S - namp sqrtf(N) |k| knoise S   (eom-exp.hpp:18-69)
+ *                  (knoise may be null; with_fen also writes lk * old kpsi into knoise)
+ *   non-conserved: kpsi <- kpsi E + knl S, *maxdiff = max |change|            (el-exp.hpp:17-61) */
+int mcs_init_eom_op(void* kop, const int ext[3], double dt, double eps, const double lens[3], int prec);
+int mcs_init_el_op(void* kop, const int ext[3], double dt, double eps, const double lens[3], int prec);
+int mcs_update_eom_etd(const void* kop, void* kpsi, const void* knonlin, void* knoise, int with_fen,
+                       const int ext[3], double namp, double eps, const double lens[3], int prec);
+int mcs_update_el_etd(const void* kop, void* kpsi, const void* knonlin, const int ext[3], int prec, double* maxdiff);
+/* call_kernel_update_eom_hpfcnoise (eom.hpp:16-58), 2D only: the semi-implicit update with the
+ * noise (c1 kx^2 + c2 ky^2) xi1 + (c2 kx^2 + c1 ky^2) xi2 + 2 kx ky xi3 for |k| <= 1/4 */
+int mcs_update_eom_hpfc(void* kpsi, const void* knonlin, const void* knoise1, const void* knoise2,
+                        const void* knoise3, const int ext[3], double namp, double dt, double eps,
+                        const double lens[3], int prec);
+/* call_kernel_update_eom_py (eom.hpp:143-180): the operator 1 - eps - P(k^2) with the reference's
+ * degree-6 polynomial P; knoise may be null */
+int mcs_update_eom_poly(void* kpsi, const void* knonlin, const void* knoise, const int ext[3], double namp,
+                        double dt, double eps, const double lens[3], int prec);
+
 /* ---------------------------------------------------------------- stepper
  * The fused PFC step: the state lives in a packed spectrum (x pairs packed
  * into one complex value), each step is four passes in 3D (two in 2D):

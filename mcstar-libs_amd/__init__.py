@@ -74,6 +74,15 @@ _SIGS = {
                            _vp], ctypes.c_int),
     "mcs_psi3_split": ([_vp, _vp, ctypes.c_double, ctypes.c_int, _ip, ctypes.c_int], ctypes.c_int),
     "mcs_norm": ([_vp, _ip, ctypes.c_int], ctypes.c_int),
+    "mcs_init_eom_op": ([_vp, _ip, ctypes.c_double, ctypes.c_double, _dp, ctypes.c_int], ctypes.c_int),
+    "mcs_init_el_op": ([_vp, _ip, ctypes.c_double, ctypes.c_double, _dp, ctypes.c_int], ctypes.c_int),
+    "mcs_update_eom_etd": ([_vp, _vp, _vp, _vp, ctypes.c_int, _ip, ctypes.c_double, ctypes.c_double, _dp,
+                            ctypes.c_int], ctypes.c_int),
+    "mcs_update_el_etd": ([_vp, _vp, _vp, _ip, ctypes.c_int, _dp], ctypes.c_int),
+    "mcs_update_eom_hpfc": ([_vp, _vp, _vp, _vp, _vp, _ip, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp,
+                             ctypes.c_int], ctypes.c_int),
+    "mcs_update_eom_poly": ([_vp, _vp, _vp, _ip, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp,
+                             ctypes.c_int], ctypes.c_int),
     "mcs_update_eom_with_fen": ([_vp, _vp, _vp, _ip, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp,
                                  ctypes.c_int], ctypes.c_int),
     "mcs_calc_fen": ([_vp, _vp, _ip, _dp, ctypes.c_int, _dp], ctypes.c_int),
@@ -302,6 +311,57 @@ def update_el(kpsi, knl, shape_zyx, lens, dt, eps):
     check(lib().mcs_update_el(dk.ptr, dn.ptr, _ext(shape_zyx), dt, eps, (ctypes.c_double * 3)(*lens), prec,
                               ctypes.byref(m)), "mcs_update_el")
     return dk.download(kpsi.shape, kpsi.dtype), m.value
+
+
+def _spec_nb(shape_zyx, ct):
+    return buffer_bytes(shape_zyx, np.float64 if np.dtype(ct) == np.complex128 else np.float32)
+
+
+def etd_op(shape_zyx, lens, dt, eps, el, dtype=np.float32):
+    """(E, S) table of exponential time differencing, conserved (el False) or not (cuda/kernels/*-exp.hpp)"""
+    ct = np.complex128 if np.dtype(dtype) == np.float64 else np.complex64
+    d = DeviceBuffer(_spec_nb(shape_zyx, ct))
+    fn = lib().mcs_init_el_op if el else lib().mcs_init_eom_op
+    check(fn(d.ptr, _ext(shape_zyx), dt, eps, (ctypes.c_double * 3)(*lens), _prec(dtype)), "mcs_init_*_op")
+    return d.download(half_shape(shape_zyx), ct)
+
+
+def update_eom_etd(kop, kpsi, knl, knoise, shape_zyx, lens, namp, eps=0.0, with_fen=False):
+    nb = _spec_nb(shape_zyx, kpsi.dtype)
+    do, dk, dn = to_device(kop, nb), to_device(kpsi, nb), to_device(knl, nb)
+    dz = to_device(knoise, nb) if knoise is not None else None
+    check(lib().mcs_update_eom_etd(do.ptr, dk.ptr, dn.ptr, dz.ptr if dz else None, 1 if with_fen else 0,
+                                   _ext(shape_zyx), namp, eps, (ctypes.c_double * 3)(*lens), _prec(kpsi.dtype)),
+          "mcs_update_eom_etd")
+    out = dk.download(kpsi.shape, kpsi.dtype)
+    return (out, dz.download(knoise.shape, knoise.dtype)) if with_fen else out
+
+
+def update_el_etd(kop, kpsi, knl, shape_zyx):
+    nb = _spec_nb(shape_zyx, kpsi.dtype)
+    do, dk, dn = to_device(kop, nb), to_device(kpsi, nb), to_device(knl, nb)
+    m = ctypes.c_double()
+    check(lib().mcs_update_el_etd(do.ptr, dk.ptr, dn.ptr, _ext(shape_zyx), _prec(kpsi.dtype), ctypes.byref(m)),
+          "mcs_update_el_etd")
+    return dk.download(kpsi.shape, kpsi.dtype), m.value
+
+
+def update_eom_hpfc(kpsi, knl, x1, x2, x3, shape_zyx, lens, namp, dt, eps):
+    nb = _spec_nb(shape_zyx, kpsi.dtype)
+    dk, dn = to_device(kpsi, nb), to_device(knl, nb)
+    d1, d2, d3 = to_device(x1, nb), to_device(x2, nb), to_device(x3, nb)
+    check(lib().mcs_update_eom_hpfc(dk.ptr, dn.ptr, d1.ptr, d2.ptr, d3.ptr, _ext(shape_zyx), namp, dt, eps,
+                                    (ctypes.c_double * 3)(*lens), _prec(kpsi.dtype)), "mcs_update_eom_hpfc")
+    return dk.download(kpsi.shape, kpsi.dtype)
+
+
+def update_eom_poly(kpsi, knl, knoise, shape_zyx, lens, namp, dt, eps):
+    nb = _spec_nb(shape_zyx, kpsi.dtype)
+    dk, dn = to_device(kpsi, nb), to_device(knl, nb)
+    dz = to_device(knoise, nb) if knoise is not None else None
+    check(lib().mcs_update_eom_poly(dk.ptr, dn.ptr, dz.ptr if dz else None, _ext(shape_zyx), namp, dt, eps,
+                                    (ctypes.c_double * 3)(*lens), _prec(kpsi.dtype)), "mcs_update_eom_poly")
+    return dk.download(kpsi.shape, kpsi.dtype)
 
 
 def update_pfc(kpsi, knl, shape_zyx, lens, dt, eps, temp, seed, draw, split):

@@ -448,6 +448,112 @@ double orc_update_el(double* kpsid, const double* knld, const int n[3], const do
     return mx;
 }
 
+/* cuda/kernels/eom-exp.hpp:3-10 and el-exp.hpp:3-9: the ETD tables (E, S) per half-spectrum mode.
+   conserved (el = 0): linop = k^2 L', S = (E - 1) / linop or -dt where that is not a number;
+   non-conserved (el = 1): linop = L'; L' = (2 - eps) + 2 dop.x + dop.y.  E = expf(-linop dt): the
+   reference takes the single-precision exponential in both builds. */
+void orc_etd_op(double* kopd, const int n[3], const double lens[3], double dt, double eps, int el) {
+    const int nx = n[0], ny = n[1], nz = n[2], nxc = nx / 2 + 1;
+    for (int i2 = 0; i2 < nz; i2++)
+        for (int i1 = 0; i1 < ny; i1++)
+            for (int i0 = 0; i0 < nxc; i0++) {
+                const long idx = ((long)i2 * ny + i1) * nxc + i0;
+                const double k0 = kval(i0, nx, lens[0]), k1 = kval(i1, ny, lens[1]), k2 = kval(i2, nz, lens[2]);
+                const double dx = -(k0 * k0 + k1 * k1 + k2 * k2), dy = dx * dx;
+                const double lp = (2.0 - eps) + 2.0 * dx + dy;
+                const double linop = el ? lp : -dx * lp;
+                const double e = (double)expf((float)(-linop * dt));
+                double second = (e - 1.0) / linop;
+                if (second != second) second = -dt;
+                kopd[2 * idx] = e;
+                kopd[2 * idx + 1] = second;
+            }
+}
+
+/* eom-exp.hpp:18-34: kpsi <- kpsi E - dop.x knl S - namp sqrt(N) |k| knoise S (knoise may be NULL) */
+void orc_update_eom_etd(const double* kopd, double* kpsid, const double* knld, const double* knoised, const int n[3],
+                        const double lens[3], double namp) {
+    const int nx = n[0], ny = n[1], nz = n[2], nxc = nx / 2 + 1;
+    const double na = namp * sqrt((double)nx * ny * nz);
+    cplx* kpsi = (cplx*)kpsid;
+    const cplx* knl = (const cplx*)knld;
+    const cplx* knoise = (const cplx*)knoised;
+    for (int i2 = 0; i2 < nz; i2++)
+        for (int i1 = 0; i1 < ny; i1++)
+            for (int i0 = 0; i0 < nxc; i0++) {
+                const long idx = ((long)i2 * ny + i1) * nxc + i0;
+                const double k0 = kval(i0, nx, lens[0]), k1 = kval(i1, ny, lens[1]), k2 = kval(i2, nz, lens[2]);
+                const double dx = -(k0 * k0 + k1 * k1 + k2 * k2);
+                const double e = kopd[2 * idx], sv = kopd[2 * idx + 1];
+                cplx v = kpsi[idx] * e - dx * knl[idx] * sv;
+                if (knoise) v -= na * sqrt(-dx) * knoise[idx] * sv;
+                kpsi[idx] = v;
+            }
+}
+
+/* el-exp.hpp:17-34: kpsi <- kpsi E + knl S; returns max |change| */
+double orc_update_el_etd(const double* kopd, double* kpsid, const double* knld, const int n[3]) {
+    const long nc = (long)(n[0] / 2 + 1) * n[1] * n[2];
+    cplx* kpsi = (cplx*)kpsid;
+    const cplx* knl = (const cplx*)knld;
+    double mx = 0;
+    for (long idx = 0; idx < nc; idx++) {
+        const cplx pn = kpsi[idx] * kopd[2 * idx] + knl[idx] * kopd[2 * idx + 1];
+        const double d = cabs(kpsi[idx] - pn);
+        if (d > mx) mx = d;
+        kpsi[idx] = pn;
+    }
+    return mx;
+}
+
+/* eom.hpp:16-46 (2D): semi-implicit update with noise (c1 kx^2 + c2 ky^2) xi1 + (c2 kx^2 + c1 ky^2) xi2
+   + 2 kx ky xi3 for |k| <= 1/4, kappa = 4/3, c2 = sqrt(1 + kappa/2 + sqrt(1 + kappa)), c1 = kappa/(2 c2) */
+void orc_update_eom_hpfc(double* kpsid, const double* knld, const double* x1d, const double* x2d, const double* x3d,
+                         const int n[3], const double lens[3], double namp, double dt, double eps) {
+    const int nx = n[0], ny = n[1], nxc = nx / 2 + 1;
+    const double na = namp * sqrt((double)nx * ny);
+    const double kappa = 4.0 / 3.0, c2 = sqrt(1.0 + kappa / 2.0 + sqrt(1.0 + kappa)), c1 = kappa / c2 / 2.0;
+    cplx* kpsi = (cplx*)kpsid;
+    const cplx *knl = (const cplx*)knld, *x1 = (const cplx*)x1d, *x2 = (const cplx*)x2d, *x3 = (const cplx*)x3d;
+    for (int i1 = 0; i1 < ny; i1++)
+        for (int i0 = 0; i0 < nxc; i0++) {
+            const long idx = (long)i1 * nxc + i0;
+            const double k0 = kval(i0, nx, lens[0]), k1 = kval(i1, ny, lens[1]);
+            const double dx = -(k0 * k0 + k1 * k1), dy = dx * dx;
+            const double denom = 1.0 - dt * ((2.0 - eps) * dx + 2.0 * dy + dx * dy);
+            cplx noise = (c1 * k0 * k0 + c2 * k1 * k1) * x1[idx] + (c2 * k0 * k0 + c1 * k1 * k1) * x2[idx] +
+                         2.0 * k0 * k1 * x3[idx];
+            if (!(sqrt(-dx) <= 0.25)) noise = 0;
+            kpsi[idx] = (kpsi[idx] + dt * dx * knl[idx] + dt * na * noise) / denom;
+        }
+}
+
+/* eom.hpp:143-171: operator 1 - eps - P(k^2), P(x) = sum_j c_2j x^j with the reference's fitted
+   coefficients, denom = 1 - dt dop.x (1 - eps - P); knoise may be NULL */
+void orc_update_eom_poly(double* kpsid, const double* knld, const double* knoised, const int n[3],
+                         const double lens[3], double namp, double dt, double eps) {
+    static const double c[7] = {-57.94636917114257, 164.4074088310241, -181.1481995196213, 100.6942650417575,
+                                -30.20985042718309, 4.678065276755401, -0.293423255328944};
+    const int nx = n[0], ny = n[1], nz = n[2], nxc = nx / 2 + 1;
+    const double na = namp * sqrt((double)nx * ny * nz);
+    cplx* kpsi = (cplx*)kpsid;
+    const cplx* knl = (const cplx*)knld;
+    const cplx* knoise = (const cplx*)knoised;
+    for (int i2 = 0; i2 < nz; i2++)
+        for (int i1 = 0; i1 < ny; i1++)
+            for (int i0 = 0; i0 < nxc; i0++) {
+                const long idx = ((long)i2 * ny + i1) * nxc + i0;
+                const double k0 = kval(i0, nx, lens[0]), k1 = kval(i1, ny, lens[1]), k2 = kval(i2, nz, lens[2]);
+                const double ksq = k0 * k0 + k1 * k1 + k2 * k2;
+                double p = 0, xp = 1;
+                for (int j = 0; j < 7; j++) { p += c[j] * xp; xp *= ksq; }
+                const double denom = 1.0 + dt * ksq * (1.0 - eps - p);
+                cplx v = kpsi[idx] - dt * ksq * knl[idx];
+                if (knoise) v += dt * na * sqrt(ksq) * knoise[idx];
+                kpsi[idx] = v / denom;
+            }
+}
+
 int orc_pfc_run(double* psi, const int n[3], const double h[3], double eps, double dt, double temp,
                 uint64_t seed, int step0, int nsteps, int split, double* mass_hist) {
     const int nx = n[0], ny = n[1], nz = n[2], nxc = nx / 2 + 1;

@@ -65,6 +65,17 @@ void orc_update_eom_fen(double* kpsi, const double* knl, double* knoise, const i
 double orc_calc_fen(const double* psi, double* lpsi, const int n[3], const double h[3]);
 /* non-conserved relaxation (el.hpp:3-45); returns max |kpsi_new - kpsi_old| */
 double orc_update_el(double* kpsi, const double* knl, const int n[3], const double lens[3], double dt, double eps);
+/* exponential time differencing tables (E, S) per mode, conserved (el = 0) or not (el = 1) */
+void orc_etd_op(double* kop, const int n[3], const double lens[3], double dt, double eps, int el);
+void orc_update_eom_etd(const double* kop, double* kpsi, const double* knl, const double* knoise, const int n[3],
+                        const double lens[3], double namp);
+double orc_update_el_etd(const double* kop, double* kpsi, const double* knl, const int n[3]);
+/* 2D update with the three-array hpfc noise (eom.hpp:16-46) */
+void orc_update_eom_hpfc(double* kpsi, const double* knl, const double* x1, const double* x2, const double* x3,
+                         const int n[3], const double lens[3], double namp, double dt, double eps);
+/* update with the polynomial linear operator (eom.hpp:143-171) */
+void orc_update_eom_poly(double* kpsi, const double* knl, const double* knoise, const int n[3],
+                         const double lens[3], double namp, double dt, double eps);
 /* free energy F = dV * sum[ psi/2 * (-eps + (1-k^2)^2) psi + psi^4/4 ] (cuda/kernels/eom.hpp:111-141) */
 double orc_free_energy(const double* psi, const int n[3], const double h[3], double eps);
 

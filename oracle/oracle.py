@@ -51,6 +51,13 @@ def lib():
         L.orc_calc_fen.restype = ctypes.c_double
         L.orc_update_el.argtypes = [_d, _d, _i, _d, ctypes.c_double, ctypes.c_double]
         L.orc_update_el.restype = ctypes.c_double
+        L.orc_etd_op.argtypes = [_d, _i, _d, ctypes.c_double, ctypes.c_double, ctypes.c_int]
+        L.orc_update_eom_etd.argtypes = [_d, _d, _d, _d, _i, _d, ctypes.c_double]
+        L.orc_update_el_etd.argtypes = [_d, _d, _d, _i]
+        L.orc_update_el_etd.restype = ctypes.c_double
+        L.orc_update_eom_hpfc.argtypes = [_d, _d, _d, _d, _d, _i, _d, ctypes.c_double, ctypes.c_double,
+                                          ctypes.c_double]
+        L.orc_update_eom_poly.argtypes = [_d, _d, _d, _i, _d, ctypes.c_double, ctypes.c_double, ctypes.c_double]
         L.orc_free_energy.argtypes = [_d, _i, _d, ctypes.c_double]
         L.orc_free_energy.restype = ctypes.c_double
         L.orc_findpeaks.argtypes = [_d, _i, ctypes.c_double, _i64, ctypes.c_int64]
@@ -173,6 +180,45 @@ def update_el(kpsi, knl, shape_zyx, lens, dt, eps):
     m = lib().orc_update_el(_p(kp), _p(np.ascontiguousarray(knl, dtype=np.complex128)), _p(_dims(shape_zyx), _i),
                             _p(np.asarray(lens, dtype=np.float64)), dt, eps)
     return kp, m
+
+
+def _c(a):
+    return np.ascontiguousarray(a, dtype=np.complex128)
+
+
+def etd_op(shape_zyx, lens, dt, eps, el):
+    out = np.empty(half_shape(shape_zyx), dtype=np.complex128)  # (E, S) pairs
+    lib().orc_etd_op(_p(out), _p(_dims(shape_zyx), _i), _p(np.asarray(lens, dtype=np.float64)), dt, eps, int(el))
+    return out
+
+
+def update_eom_etd(kop, kpsi, knl, knoise, shape_zyx, lens, namp):
+    kp = _c(kpsi).copy()
+    kz = None if knoise is None else _c(knoise)
+    lib().orc_update_eom_etd(_p(_c(kop)), _p(kp), _p(_c(knl)), None if kz is None else _p(kz),
+                             _p(_dims(shape_zyx), _i), _p(np.asarray(lens, dtype=np.float64)), namp)
+    return kp
+
+
+def update_el_etd(kop, kpsi, knl, shape_zyx):
+    kp = _c(kpsi).copy()
+    m = lib().orc_update_el_etd(_p(_c(kop)), _p(kp), _p(_c(knl)), _p(_dims(shape_zyx), _i))
+    return kp, m
+
+
+def update_eom_hpfc(kpsi, knl, x1, x2, x3, shape_zyx, lens, namp, dt, eps):
+    kp = _c(kpsi).copy()
+    lib().orc_update_eom_hpfc(_p(kp), _p(_c(knl)), _p(_c(x1)), _p(_c(x2)), _p(_c(x3)), _p(_dims(shape_zyx), _i),
+                              _p(np.asarray(lens, dtype=np.float64)), namp, dt, eps)
+    return kp
+
+
+def update_eom_poly(kpsi, knl, knoise, shape_zyx, lens, namp, dt, eps):
+    kp = _c(kpsi).copy()
+    kz = None if knoise is None else _c(knoise)
+    lib().orc_update_eom_poly(_p(kp), _p(_c(knl)), None if kz is None else _p(kz), _p(_dims(shape_zyx), _i),
+                              _p(np.asarray(lens, dtype=np.float64)), namp, dt, eps)
+    return kp
 
 
 def free_energy(psi, h, eps):

@@ -338,6 +338,80 @@ def test_update_el_relaxes(orc):
     assert dm[-1] < dm[0]
 
 
+def _etd_case():
+    shp = (6, 8, 10)
+    h = [0.7, 0.6, 0.5]
+    lens = [n * hh for n, hh in zip(shp[::-1], h)]
+    rng = np.random.default_rng(11)
+    kp = np.fft.rfftn(rng.standard_normal(shp))
+    nl = np.fft.rfftn(0.3 * rng.standard_normal(shp))
+    k2 = _k2(shp, h[::-1])
+    return shp, lens, kp, nl, k2
+
+
+def test_etd_conserved_is_exact_for_constant_forcing(orc):
+    # d psi_k/dt = -k^2 L' psi_k - k^2 N_k with N fixed: one ETD1 step of any length is the exact solution
+    shp, lens, kp, nl, k2 = _etd_case()
+    eps, dt = 0.25, 0.7
+    lp = (1 - k2) ** 2 + 1 - eps
+    kop = orc.etd_op(shp, lens, dt, eps, 0)
+    got = orc.update_eom_etd(kop, kp, nl, None, shp, lens, 0.0)
+    lam = k2 * lp
+    e = np.exp(-lam * dt)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        s = np.where(lam == 0, -dt, (e - 1) / lam)
+    want = e * kp + s * k2 * nl
+    assert np.abs(got - want).max() < 2e-7 * np.abs(want).max()  # the table holds single-precision exponentials
+    assert kop[0, 0, 0] == 1 - 1j * dt and got[0, 0, 0] == kp[0, 0, 0]  # k = 0: mass unchanged, S = -dt
+
+
+def test_etd_nonconserved_is_exact_for_constant_forcing(orc):
+    shp, lens, kp, nl, k2 = _etd_case()
+    eps, dt = 0.25, 0.4
+    lp = (1 - k2) ** 2 + 1 - eps
+    kop = orc.etd_op(shp, lens, dt, eps, 1)
+    got, m = orc.update_el_etd(kop, kp, nl, shp)
+    e = np.exp(-lp * dt)
+    want = e * kp + (e - 1) / lp * nl
+    assert np.abs(got - want).max() < 2e-7 * np.abs(want).max()
+    assert abs(m - np.abs(got - kp).max()) < 1e-12 * m
+
+
+def test_hpfc_update_reduces_to_reference_split_and_filters(orc):
+    shp = (48, 64)
+    lens = [64 * 0.7, 48 * 0.7, 1.0]
+    rng = np.random.default_rng(2)
+    kp = np.fft.rfftn(rng.standard_normal(shp))
+    nl = np.fft.rfftn(0.2 * rng.standard_normal(shp))
+    z = np.zeros_like(kp)
+    got = orc.update_eom_hpfc(kp, nl, z, z, z, shp, lens, 0.05, 0.3, 0.2)
+    want = orc.update(kp, nl, None, shp, lens, 0.3, 0.2, 0.0, orc.SPLIT_REFERENCE)
+    assert np.abs(got - want).max() < 1e-13 * np.abs(want).max()
+    # the noise acts on |k| <= 1/4 only, with weights (c1 kx^2 + c2 ky^2, c2 kx^2 + c1 ky^2, 2 kx ky)
+    xs = [orc.kspace_noise(shp, 7 + j, 0) for j in range(3)]
+    gotn = orc.update_eom_hpfc(kp, nl, *xs, shp, lens, 0.05, 0.3, 0.2)
+    ky = 2 * np.pi * np.fft.fftfreq(48, d=0.7)[:, None]
+    kx = 2 * np.pi * np.fft.rfftfreq(64, d=0.7)[None, :]
+    k2 = kx ** 2 + ky ** 2
+    den = 1 + 0.3 * k2 * ((1 - k2) ** 2 + 1 - 0.2)
+    c2 = np.sqrt(1 + 2 / 3 + np.sqrt(1 + 4 / 3))
+    c1 = 4 / 3 / c2 / 2
+    noise = (c1 * kx ** 2 + c2 * ky ** 2) * xs[0] + (c2 * kx ** 2 + c1 * ky ** 2) * xs[1] + 2 * kx * ky * xs[2]
+    noise = np.where(np.sqrt(k2) <= 0.25, noise, 0)
+    assert np.abs((gotn - got) * den - 0.3 * 0.05 * np.sqrt(48 * 64) * noise).max() < 1e-12
+    assert np.any(np.sqrt(k2) > 0.25) and np.any((np.sqrt(k2) <= 0.25) & (k2 > 0))
+
+
+def test_polynomial_operator_update(orc):
+    shp, lens, kp, nl, k2 = _etd_case()
+    c = [-57.94636917114257, 164.4074088310241, -181.1481995196213, 100.6942650417575, -30.20985042718309,
+         4.678065276755401, -0.293423255328944]
+    got = orc.update_eom_poly(kp, np.zeros_like(nl), None, shp, lens, 0.0, 0.2, 0.3)
+    den = 1 + 0.2 * k2 * (1 - 0.3 - np.polynomial.polynomial.polyval(k2, c))
+    assert np.abs(got - kp / den).max() < 1e-12 * np.abs(kp).max()
+    assert got[0, 0, 0] == kp[0, 0, 0]
+
+
 # ---------------------------------------------------------------- peaks
 def _brute_peaks(v, ths):
     nd = v.ndim
