@@ -21,11 +21,11 @@ sim: $(SIM)
 
 $(PKG)/bin/mcs_sim_f32: $(PKG)/sim/mcs_sim.cpp $(LIB) $(wildcard include/*/*.hpp include/*/*/*.hpp)
 	@mkdir -p $(dir $@)
-	$(HIPCC) -O2 -std=c++17 -DFLOAT=float -I include $< -o $@ -L$(PKG)/lib -lmcstar_hip -Wl,-rpath,'$$ORIGIN/../lib'
+	$(HIPCC) -O2 -std=c++17 -DFLOAT=float -I include $< -o $@ -L$(PKG)/lib -lmcstar_hip -lz -Wl,-rpath,'$$ORIGIN/../lib'
 
 $(PKG)/bin/mcs_sim_f64: $(PKG)/sim/mcs_sim.cpp $(LIB) $(wildcard include/*/*.hpp include/*/*/*.hpp)
 	@mkdir -p $(dir $@)
-	$(HIPCC) -O2 -std=c++17 -DFLOAT=double -I include $< -o $@ -L$(PKG)/lib -lmcstar_hip -Wl,-rpath,'$$ORIGIN/../lib'
+	$(HIPCC) -O2 -std=c++17 -DFLOAT=double -I include $< -o $@ -L$(PKG)/lib -lmcstar_hip -lz -Wl,-rpath,'$$ORIGIN/../lib'
 
 # C++ drop-in driver of the kept host API (tests/cpp), one build per precision
 DRV := tests/cpp/bin/kept_api_driver_f32 tests/cpp/bin/kept_api_driver_f64
@@ -33,11 +33,11 @@ driver: $(DRV)
 
 tests/cpp/bin/kept_api_driver_f32: tests/cpp/kept_api_driver.cpp $(LIB) $(wildcard include/*/*.hpp include/*/*/*.hpp)
 	@mkdir -p $(dir $@)
-	$(HIPCC) -O2 -std=c++17 -DFLOAT=float -I include $< -o $@ -L$(PKG)/lib -lmcstar_hip -Wl,-rpath,'$$ORIGIN/../../../$(PKG)/lib'
+	$(HIPCC) -O2 -std=c++17 -DFLOAT=float -I include $< -o $@ -L$(PKG)/lib -lmcstar_hip -lz -Wl,-rpath,'$$ORIGIN/../../../$(PKG)/lib'
 
 tests/cpp/bin/kept_api_driver_f64: tests/cpp/kept_api_driver.cpp $(LIB) $(wildcard include/*/*.hpp include/*/*/*.hpp)
 	@mkdir -p $(dir $@)
-	$(HIPCC) -O2 -std=c++17 -DFLOAT=double -I include $< -o $@ -L$(PKG)/lib -lmcstar_hip -Wl,-rpath,'$$ORIGIN/../../../$(PKG)/lib'
+	$(HIPCC) -O2 -std=c++17 -DFLOAT=double -I include $< -o $@ -L$(PKG)/lib -lmcstar_hip -lz -Wl,-rpath,'$$ORIGIN/../../../$(PKG)/lib'
 
 lib: $(LIB)
 

@@ -291,6 +291,27 @@ int mcs_bond_order_cutoff(const double* xyz, int npts, double box, double rlim, 
 int mcs_bond_order_knn(const int* ijk, int npts, int n, double h, int k, int* nbr, double* out4,
                        int64_t* hist_q4, int64_t* hist_q6, int nbins);
 
+/* ---------------------------------------------------------------- post-processing
+ * call_kernel_gaussian_init (cuda/kernels/gaussian.hpp:23-40): kgauss[mode] =
+ * (e, 1 - e), e = exp(-(sx^2 kx^2 + sy^2 ky^2 + sz^2 kz^2) / 2), s_i = 1.1 divs h_i,
+ * over the half spectrum of the real extent ext (device array). */
+int mcs_gaussian_init(void* kgauss, const int ext[3], const double lens[3], const double h[3],
+                      double divs_per_sigma, int prec);
+/* call_kernel_kmult_real / _imag (gaussian.hpp:42-67) and
+ * _call_kernel_multassign_cmpl_with_real / _imag (lowpass.hpp:3-35):
+ * karr1[mode] *= part == 0 ? Re karr2[mode] : Im karr2[mode] (device arrays). */
+int mcs_kmult(void* karr1, const void* karr2, const int ext[3], int part, int prec);
+/* _call_kernel_square (lowpass.hpp:37-53): a <- a^2 over the real array (device). */
+int mcs_square(void* real, const int ext[3], int prec);
+/* particlepairs/main.cpp:8-53: the distance of every ordered pair (i, j), j != i,
+ * under the minimum image of a cubic periodic box, in the order i-major, j
+ * ascending (npts*(npts-1) values).  Host arrays; FP64 on the GPU.  *count gets
+ * npts*(npts-1); MCS_ERR_CAPACITY when cap is smaller. */
+int mcs_pair_distances(const double* xyz, int npts, double box, double* out, int64_t cap, int64_t* count);
+/* histogram of the same distances over unordered pairs i < j: nbins (<= 16384)
+ * equal bins over [0, rmax); pairs at rmax or beyond are not counted. */
+int mcs_pair_histogram(const double* xyz, int npts, double box, double rmax, int nbins, int64_t* hist);
+
 #ifdef __cplusplus
 }
 #endif

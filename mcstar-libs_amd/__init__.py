@@ -129,6 +129,12 @@ _SIGS = {
     "mcs_bond_order_cutoff": ([_dp, ctypes.c_int, ctypes.c_double, ctypes.c_double, _ip, _dp], ctypes.c_int),
     "mcs_bond_order_knn": ([_ip, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, _ip, _dp, _i64p,
                             _i64p, ctypes.c_int], ctypes.c_int),
+    "mcs_gaussian_init": ([_vp, _ip, _dp, _dp, ctypes.c_double, ctypes.c_int], ctypes.c_int),
+    "mcs_kmult": ([_vp, _vp, _ip, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+    "mcs_square": ([_vp, _ip, ctypes.c_int], ctypes.c_int),
+    "mcs_pair_distances": ([_dp, ctypes.c_int, ctypes.c_double, _dp, ctypes.c_int64, _i64p], ctypes.c_int),
+    "mcs_pair_histogram": ([_dp, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int, _i64p],
+                           ctypes.c_int),
 }
 
 _LIB = None
@@ -474,6 +480,99 @@ def bond_order_knn(ijk, n, h, k=12, nbins=64):
                                    nbins), "mcs_bond_order_knn")
     return nbr[:P], out[:P], h4, h6
 
+
+
+# ---------------------------------------------------------------- post-processing
+
+def gaussian_init(shape_zyx, lens, h, divs_per_sigma, dtype=np.float32):
+    """(e, 1 - e) per half-spectrum mode (call_kernel_gaussian_init, cuda/kernels/gaussian.hpp:23-40)"""
+    ct = np.complex128 if np.dtype(dtype) == np.float64 else np.complex64
+    d = DeviceBuffer(buffer_bytes(shape_zyx, dtype))
+    check(lib().mcs_gaussian_init(d.ptr, _ext(shape_zyx), (ctypes.c_double * 3)(*lens), (ctypes.c_double * 3)(*h),
+                                  divs_per_sigma, _prec(dtype)), "mcs_gaussian_init")
+    return d.download(half_shape(shape_zyx), ct)
+
+
+def kmult(k1, k2, shape_zyx, part):
+    """k1 * Re k2 (part 0) or k1 * Im k2 (part 1) (call_kernel_kmult_real / _imag)"""
+    nb = _spec_nb(shape_zyx, k1.dtype)
+    d1, d2 = to_device(k1, nb), to_device(k2, nb)
+    check(lib().mcs_kmult(d1.ptr, d2.ptr, _ext(shape_zyx), part, _prec(k1.dtype)), "mcs_kmult")
+    return d1.download(k1.shape, k1.dtype)
+
+
+def square(a):
+    d = to_device(a)
+    check(lib().mcs_square(d.ptr, _ext(a.shape), _prec(a.dtype)), "mcs_square")
+    return d.download(a.shape, a.dtype)
+
+
+def amplitude_lowpass(psi, lens, h, divs_per_sigma):
+    """calc_amplitude_lowpass (cuda/kernels/lowpass.hpp:79-93) of a 2D field, all on the GPU:
+    returns (low-passed spectrum of the squared high-passed field, its real-space amplitude)"""
+    psi = np.ascontiguousarray(psi)
+    shp = psi.shape
+    if len(shp) != 2 or min(shp) < 2:
+        raise McsError(ERR_INVALID, "amplitude_lowpass", "field is not 2d")
+    prec = _prec(psi.dtype)
+    ct = np.complex128 if prec == 64 else np.complex64
+    nb = buffer_bytes(shp, psi.dtype)
+    spec, aux, kg = DeviceBuffer(nb).upload(psi), DeviceBuffer(nb), DeviceBuffer(nb)
+    ext = _ext(shp)
+    r2c, c2r = FFTPlan(shp, psi.dtype, R2C), FFTPlan(shp, psi.dtype, C2R)
+    L = lib()
+    check(L.mcs_gaussian_init(kg.ptr, ext, (ctypes.c_double * 3)(*lens), (ctypes.c_double * 3)(*h), divs_per_sigma,
+                              prec), "mcs_gaussian_init")
+    r2c.execute(spec, aux)  # aux = spectrum of psi
+    check(L.mcs_kmult(aux.ptr, kg.ptr, ext, 1, prec), "mcs_kmult")
+    c2r.execute(aux, spec)
+    check(L.mcs_norm(spec.ptr, ext, prec), "mcs_norm")
+    check(L.mcs_square(spec.ptr, ext, prec), "mcs_square")
+    r2c.execute(spec, aux)
+    check(L.mcs_kmult(aux.ptr, kg.ptr, ext, 0, prec), "mcs_kmult")
+    k = aux.download(half_shape(shp), ct)
+    c2r.execute(aux, spec)
+    check(L.mcs_norm(spec.ptr, ext, prec), "mcs_norm")
+    return k, spec.download(shp, psi.dtype)
+
+
+def write_lowpass_2d(spec, shape_zyx, ll, fn):
+    """gzip text of a 2D low-passed half spectrum in the format of write_lowpass_2d
+    (cuda/kernels/lowpass.hpp:54-78): "LOWPASS", the header length, a header naming
+    n0 n1 l0 l1, then "i0 i1 re im" (C's %g) for modes with re^2 + im^2 > 1e-6, computed
+    in the spectrum's precision; i0 runs over the half axis in the outer loop."""
+    import gzip
+    ny, nx = shape_zyx
+    head = "(S'n0'\nI%d\nS'n1'\nI%d\nS'l0'\nF%f\nS'l1'\nF%f\nd.\n" % (nx, ny, ll[0], ll[1])
+    sp = np.asarray(spec)
+    rt = np.float64 if sp.dtype == np.complex128 else np.float32
+    re, im = sp.real.astype(rt).T, sp.imag.astype(rt).T  # (i0, i1)
+    keep = (re * re + im * im).astype(np.float64) > 1e-6
+    i0, i1 = np.nonzero(keep)  # row-major over (i0, i1): the file's loop order
+    body = "".join("%d %d %g %g\n" % (a, b, float(re[a, b]), float(im[a, b])) for a, b in zip(i0, i1))
+    with gzip.open(fn, "wb") as f:
+        f.write(("LOWPASS\n%d\n" % len(head) + head + body).encode())
+
+
+def pair_distances(xyz, box):
+    """minimum-image distances of all ordered pairs i != j, i-major (particlepairs/main.cpp:8-53)"""
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+    P = xyz.shape[0]
+    total = P * (P - 1)
+    out = np.empty(max(total, 1), dtype=np.float64)
+    cnt = ctypes.c_int64()
+    check(lib().mcs_pair_distances(xyz.ctypes.data_as(_dp), P, box, out.ctypes.data_as(_dp), total,
+                                   ctypes.byref(cnt)), "mcs_pair_distances")
+    return out[:cnt.value].copy()
+
+
+def pair_histogram(xyz, box, rmax, nbins):
+    """counts of unordered pairs per distance bin over [0, rmax)"""
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+    hist = np.zeros(nbins, dtype=np.int64)
+    check(lib().mcs_pair_histogram(xyz.ctypes.data_as(_dp), xyz.shape[0], box, rmax, nbins,
+                                   hist.ctypes.data_as(_i64p)), "mcs_pair_histogram")
+    return hist
 
 class Stepper:
     """The fused PFC stepper (mcs_stepper_*)."""

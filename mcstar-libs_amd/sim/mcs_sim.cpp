@@ -5,7 +5,8 @@
 // noise, periodic npy snapshots of psi, mean density and free energy, local
 // maxima written in the findpeaks format (cuda/kernels/findpeaks.hpp:45-53) and
 // the 13-column bond-order table (bond-order-analysis/bond-order.cpp:333-338),
-// and an exact checkpoint to resume from.  Flags are --name value pairs; grid
+// the low-pass amplitude of a 2D field with its gz mode dump
+// (cuda/kernels/lowpass.hpp:54-93), and an exact checkpoint to resume from.  Flags are --name value pairs; grid
 // sizes are written NxNxN or NxN, x first, as common/tclap_defs.hpp:3-30 parses
 // them.  Built with -DFLOAT=float or -DFLOAT=double.
 //
@@ -14,8 +15,11 @@
 //   --temp 0 --noise-seed 0                          --split northstar|reference
 //   --steps 100 --every 0 --out DIR                  --load psi.npy | --resume DIR
 //   --ths 0 --rlim 5.5   (peaks >= ths and bond order within rlim at every output)
+//   --lowpass DIVS       (2D: amplitude_<step>.npy and lowpass_<step>.gz at every output,
+//                         Gaussian width 1.1 DIVS h)
 //   --plan Px,Gx,Py,Gy,Pz,Gz  (explicit plans; default: tuned)   --no-tune
 #include <cuda/array_abstraction.hpp>
+#include <cuda/cufft_wrapper.hpp>
 #include <cuda/math.hpp>
 
 #include <chrono>
@@ -26,6 +30,7 @@
 #include <fstream>
 #include <iostream>
 #include <map>
+#include <memory>
 #include <sstream>
 #include <string>
 #include <vector>
@@ -34,14 +39,18 @@ typedef FLOAT Float;
 struct Float3 { Float x, y, z; };
 typedef Array::CPUArray<Float> CPUArray;
 typedef Array::GPUArray<Float> GPUArray;
+typedef FFT::CufftPlan<Float, FFT::R2C> PlanR2C;
+typedef FFT::CufftPlan<Float, FFT::C2R> PlanC2R;
 
 #include <cuda/kernels/findpeaks.hpp>
+#include <cuda/kernels/gaussian.hpp>
+#include <cuda/kernels/lowpass.hpp>
 
 namespace {
 
 struct Opts {
     int3 dims{64, 64, 64};
-    double h = M_PI / 4, eps = 0.3, dt = 0.5, psi0 = -0.25, amp = 0.05, temp = 0, ths = 0, rlim = 5.5;
+    double h = M_PI / 4, eps = 0.3, dt = 0.5, psi0 = -0.25, amp = 0.05, temp = 0, ths = 0, rlim = 5.5, lowpass = 0;
     uint64_t seed = 11, noise_seed = 0;
     int split = MCS_SPLIT_NORTHSTAR, steps = 100, every = 0;
     std::string out = ".", load, resume;
@@ -100,6 +109,7 @@ Opts parse(int argc, char** argv) {
         else if (k == "--resume") o.resume = v;
         else if (k == "--ths") o.ths = std::atof(v.c_str());
         else if (k == "--rlim") o.rlim = std::atof(v.c_str());
+        else if (k == "--lowpass") o.lowpass = std::atof(v.c_str());
         else if (k == "--plan") o.plan = parse_list(v);
         else usage("unknown flag " + k);
     }
@@ -168,7 +178,22 @@ Resume read_checkpoint(Opts& o) {
 }
 
 // ------------------------------------------------------------------ outputs
-void write_outputs(const Opts& o, mcs_stepper* st, GPUArray& psi, CPUArray& host, int64_t step, double ms_per_step) {
+// plans, buffers and filter table of the 2D low-pass amplitude, made once
+struct Lowpass {
+    PlanR2C r2c;
+    PlanC2R c2r;
+    GPUArray spec, aux, kg;
+    Float3 lens;
+    Lowpass(const Opts& o)
+        : r2c(o.dims), c2r(o.dims), spec(o.dims), aux(o.dims), kg(o.dims),
+          lens{(Float)(o.h * o.dims.x), (Float)(o.h * o.dims.y), (Float)o.h} {
+        const Float3 hh{(Float)o.h, (Float)o.h, (Float)o.h};
+        call_kernel_gaussian_init(kg, lens, hh, (Float)o.lowpass);
+    }
+};
+
+void write_outputs(const Opts& o, mcs_stepper* st, GPUArray& psi, CPUArray& host, int64_t step, double ms_per_step,
+                   Lowpass* lp) {
     CUERR(mcs_stepper_get_psi(st, psi.ptr_void()));
     char tag[64];
     std::snprintf(tag, sizeof tag, "%08lld", (long long)step);
@@ -195,6 +220,12 @@ void write_outputs(const Opts& o, mcs_stepper* st, GPUArray& psi, CPUArray& host
         call_kernel_findpeaks(psi, o.out + "/peaks_" + tag + ".txt", o.out + "/bondorder_" + tag + ".txt", h3, len,
                               (Float)o.ths, (Float)o.rlim);
     }
+    if (lp) {
+        lp->r2c.execute(psi, lp->spec);
+        calc_amplitude_lowpass(lp->r2c, lp->c2r, lp->spec, lp->aux, lp->kg, host, lp->lens,
+                               o.out + "/lowpass_" + tag + ".gz");
+        host.save(o.out + "/amplitude_" + tag + ".npy");
+    }
 }
 
 }  // namespace
@@ -216,8 +247,10 @@ int main(int argc, char** argv) {
     if (o.plan.empty()) CUERR(mcs_stepper_create(&st, &p));
     else CUERR(mcs_stepper_create_planned(&st, &p, o.plan.data()));
 
+    if (o.lowpass > 0 && (o.dims.z != 1 || o.dims.y < 2 || o.dims.x < 2)) usage("--lowpass needs a 2D grid");
     CPUArray host(o.dims);
     GPUArray psi(host);
+    std::unique_ptr<Lowpass> lp(o.lowpass > 0 ? new Lowpass(o) : nullptr);
     int64_t step0 = 0;
     if (resuming) {
         CUERR(mcs_stepper_set_state(st, res.spec.data(), res.mass, res.step));
@@ -241,9 +274,9 @@ int main(int argc, char** argv) {
         CUERR(mcs_stepper_step(st, chunk));
         ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() / chunk;
         done += chunk;
-        if (o.every > 0 || done == o.steps) write_outputs(o, st, psi, host, step0 + done, ms);
+        if (o.every > 0 || done == o.steps) write_outputs(o, st, psi, host, step0 + done, ms, lp.get());
     }
-    if (o.steps == 0) write_outputs(o, st, psi, host, step0, 0.0);
+    if (o.steps == 0) write_outputs(o, st, psi, host, step0, 0.0, lp.get());
     save_checkpoint(o, st);
     CUERR(mcs_stepper_destroy(st));
     return 0;

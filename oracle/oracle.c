@@ -875,3 +875,45 @@ void orc_bond_order_knn(const int* ijk, int P, int n, double h, int k, int* nbr,
         free(bestd); free(bestj);
     }
 }
+
+/* ------------------------------------------------------------------------ */
+/* Post-processing.                                                          */
+/* ------------------------------------------------------------------------ */
+
+/* Gaussian filter table, restating cuda/kernels/gaussian.hpp:23-33:
+   e = exp(-(sx^2 kx^2 + sy^2 ky^2 + sz^2 kz^2)/2), s_i = 1.1 divs h_i, kg = (e, 1 - e) */
+void orc_gaussian_init(double* kgd, const int n[3], const double lens[3], const double h[3], double divs) {
+    const int nx = n[0], ny = n[1], nz = n[2], nxc = nx / 2 + 1;
+    const double s[3] = {1.1 * divs * h[0], 1.1 * divs * h[1], 1.1 * divs * h[2]};
+    cplx* kg = (cplx*)kgd;
+    for (int i2 = 0; i2 < nz; i2++)
+        for (int i1 = 0; i1 < ny; i1++)
+            for (int i0 = 0; i0 < nxc; i0++) {
+                const double a = s[0] * kval(i0, nx, lens[0]), b = s[1] * kval(i1, ny, lens[1]),
+                             c = s[2] * kval(i2, nz, lens[2]);
+                const double e = exp(-0.5 * (a * a + b * b + c * c));
+                kg[((long)i2 * ny + i1) * nxc + i0] = e + I * (1.0 - e);
+            }
+}
+
+/* minimum image of x2 - x1 in a periodic box of width w, restating particlepairs/main.cpp:8-15 */
+static inline double image_diff(double x2, double x1, double w) {
+    const double dx = x2 - x1;
+    if (fabs(dx) > w / 2) return dx > 0 ? dx - w : dx + w;
+    return dx;
+}
+
+/* all ordered pairs i != j, i-major (particlepairs/main.cpp:41-50); returns P (P - 1) */
+int64_t orc_pair_distances(const double* xyz, int P, double box, double* out) {
+    int64_t t = 0;
+    for (int i = 0; i < P; i++)
+        for (int j = 0; j < P; j++) {
+            if (i == j) continue;
+            const double dx = image_diff(xyz[3 * j], xyz[3 * i], box);
+            const double dy = image_diff(xyz[3 * j + 1], xyz[3 * i + 1], box);
+            const double dz = image_diff(xyz[3 * j + 2], xyz[3 * i + 2], box);
+            const double dx2 = dx * dx, dy2 = dy * dy, dz2 = dz * dz;
+            out[t++] = sqrt(dx2 + dy2 + dz2);
+        }
+    return t;
+}

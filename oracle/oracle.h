@@ -96,6 +96,12 @@ void orc_ylm(int l, int m, double ct, double phi, double* re, double* im);
 /* Wigner 3j symbol (j1 j2 j3; m1 m2 m3) for integer arguments */
 double orc_wigner3j(int j1, int j2, int j3, int m1, int m2, int m3);
 
+/* ---- post-processing ---- */
+/* Gaussian filter table (cuda/kernels/gaussian.hpp:23-33): kg[mode] = (e, 1 - e) over the half spectrum */
+void orc_gaussian_init(double* kg, const int n[3], const double lens[3], const double h[3], double divs);
+/* minimum-image distances of all ordered pairs (particlepairs/main.cpp:8-53), i-major; returns P (P - 1) */
+int64_t orc_pair_distances(const double* xyz, int P, double box, double* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -67,6 +67,9 @@ def lib():
         L.orc_ylm.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, _d, _d]
         L.orc_wigner3j.argtypes = [ctypes.c_int] * 6
         L.orc_wigner3j.restype = ctypes.c_double
+        L.orc_gaussian_init.argtypes = [_d, _i, _d, _d, ctypes.c_double]
+        L.orc_pair_distances.argtypes = [_d, ctypes.c_int, ctypes.c_double, _d]
+        L.orc_pair_distances.restype = ctypes.c_int64
         _LIB = L
     return _LIB
 
@@ -268,4 +271,74 @@ def ref_bond_order_binary():
     """Path of the reference's own bond-order analysis built from its sources
     (bond-order-analysis/standalone.cpp + bond-order.cpp), or None."""
     p = os.path.join(_HERE, "_ref", "bond_order_standalone")
+    return p if os.path.exists(p) else None
+
+
+# ---------------------------------------------------------------- post-processing
+
+def gaussian_init(shape_zyx, lens, h, divs):
+    """(e, 1 - e) per half-spectrum mode (cuda/kernels/gaussian.hpp:23-33)"""
+    out = np.empty(half_shape(shape_zyx), dtype=np.complex128)
+    lib().orc_gaussian_init(_p(out), _p(_dims(shape_zyx), _i), _p(np.asarray(lens, dtype=np.float64)),
+                            _p(np.asarray(h, dtype=np.float64)), divs)
+    return out
+
+
+def kmult(k1, k2, part):
+    """k1 * Re k2 (part 0) or k1 * Im k2 (part 1) (gaussian.hpp:42-67, lowpass.hpp:3-35)"""
+    f = k2.real if part == 0 else k2.imag
+    return k1 * f
+
+
+def amplitude_lowpass(psi, lens, h, divs):
+    """calc_amplitude_lowpass (cuda/kernels/lowpass.hpp:79-93) of a 2D field in FP64:
+    returns (low-passed spectrum written to the text file, amplitude field)"""
+    shp = psi.shape
+    N = psi.size
+    kg = gaussian_init(shp, lens, h, divs)
+    hi = c2r(kmult(r2c(psi), kg, 1), shp) / N
+    spec = kmult(r2c(hi * hi), kg, 0)
+    return spec, c2r(spec, shp) / N
+
+
+def lowpass_text(spec, shape_zyx, ll, real_type=np.float64):
+    """the text of write_lowpass_2d (lowpass.hpp:54-78) for a 2D half spectrum
+    (ny, nx/2+1), values as real_type: C's %g (the stream's default 6 digits)"""
+    ny, nx = shape_zyx
+    head = "(S'n0'\nI%d\nS'n1'\nI%d\nS'l0'\nF%f\nS'l1'\nF%f\nd.\n" % (nx, ny, ll[0], ll[1])
+    lines = ["LOWPASS\n%d\n" % len(head), head]
+    sp = np.asarray(spec)
+    re = sp.real.astype(real_type)
+    im = sp.imag.astype(real_type)
+    p = re * re + im * im  # in real_type, compared in double
+    for i0 in range(nx // 2 + 1):
+        for i1 in range(ny):
+            if float(p[i1, i0]) > 1e-6:
+                lines.append("%d %d %g %g\n" % (i0, i1, float(re[i1, i0]), float(im[i1, i0])))
+    return "".join(lines)
+
+
+def pair_distances(xyz, box):
+    """minimum-image distances of all ordered pairs, i-major (particlepairs/main.cpp:8-53)"""
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+    P = xyz.shape[0]
+    out = np.empty(max(P * (P - 1), 1), dtype=np.float64)
+    n = lib().orc_pair_distances(_p(xyz), P, box, _p(out))
+    return out[:n].copy()
+
+
+def pair_histogram(xyz, box, rmax, nbins):
+    """counts of unordered pairs i < j per bin floor(d / rmax * nbins) for d < rmax"""
+    P = len(xyz)
+    d = pair_distances(xyz, box).reshape(P, P - 1) if P > 1 else np.zeros((P, 0))
+    iu = [d[i, i:] for i in range(P)]  # row i holds j = 0..i-1, i+1..; j > i are columns i..
+    dd = np.concatenate(iu) if iu else np.zeros(0)
+    dd = dd[dd < rmax]
+    b = np.minimum((dd / rmax * nbins).astype(np.int64), nbins - 1)
+    return np.bincount(b, minlength=nbins).astype(np.int64)
+
+
+def ref_particlepairs_binary():
+    """Path of the reference's particlepairs tool built from particlepairs/main.cpp, or None."""
+    p = os.path.join(_HERE, "_ref", "particlepairs")
     return p if os.path.exists(p) else None

@@ -10,6 +10,7 @@
 //   peaks n0 n1 n2 in.npy peaks.txt bo.txt h ths rlim   (findpeaks.hpp:31-63)
 //   fen   n0 n1 n2 in.npy eps h        update_eom_with_fen -> c2r -> norm -> calc_fen
 //   relax n0 n1 n2 in.npy out.npy steps eps dt   c2r -> psi3_split(norm) -> r2c -> update_el
+//   lowpass n0 n1 1 in.npy out.gz amp.npy h divs   gaussian_init -> r2c -> calc_amplitude_lowpass
 #include <cuda/array_abstraction.hpp>
 #include <cuda/cufft_wrapper.hpp>
 #include <cuda/math.hpp>
@@ -31,13 +32,15 @@ typedef FFT::CufftPlan<Float, FFT::C2R> PlanC2R;
 #include <cuda/kernels/el.hpp>
 #include <cuda/kernels/eom.hpp>
 #include <cuda/kernels/findpeaks.hpp>
+#include <cuda/kernels/gaussian.hpp>
+#include <cuda/kernels/lowpass.hpp>
 #include <cuda/kernels/norm.hpp>
 #include <cuda/kernels/psi3_split.hpp>
 
 static int3 shape_arg(char** a) { return int3{atoi(a[0]), atoi(a[1]), atoi(a[2])}; }
 
 int main(int argc, char* argv[]) {
-    if (argc < 2) { std::cerr << "usage: " << argv[0] << " fft|noise|pfc|peaks ..." << std::endl; return 1; }
+    if (argc < 2) { std::cerr << "usage: " << argv[0] << " fft|noise|pfc|fen|relax|peaks|lowpass ..." << std::endl; return 1; }
     CUERR(mcs_init(0));
     const std::string mode = argv[1];
     if (mode == "fft" && argc == 8) {
@@ -156,6 +159,22 @@ int main(int argc, char* argv[]) {
         const Float h = (Float)atof(argv[8]);
         const Float3 hh{h, h, h}, len{shape.x * h, shape.y * h, shape.z * h};
         call_kernel_findpeaks(dev, argv[6], argv[7], hh, len, (Float)atof(argv[9]), (Float)atof(argv[10]));
+        return 0;
+    }
+    if (mode == "lowpass" && argc == 10) {
+        // calc_amplitude_lowpass (cuda/kernels/lowpass.hpp:79-93) of a 2D field (z extent 1)
+        const int3 shape = shape_arg(argv + 2);
+        const Float h = (Float)atof(argv[8]), divs = (Float)atof(argv[9]);
+        CPUArray host(shape);
+        host.load(argv[5]);
+        GPUArray psi(host), spec(shape), aux(shape), kg(shape);
+        PlanR2C r2c(shape);
+        PlanC2R c2r(shape);
+        const Float3 lens{shape.x * h, shape.y * h, shape.z * h}, hh{h, h, h};
+        call_kernel_gaussian_init(kg, lens, hh, divs);
+        r2c.execute(psi, spec);
+        calc_amplitude_lowpass(r2c, c2r, spec, aux, kg, host, lens, argv[6]);
+        host.save(argv[7]);
         return 0;
     }
     std::cerr << "bad arguments for mode " << mode << std::endl;

@@ -72,3 +72,23 @@ def test_load_2d_and_bad_flags(mcs, orc, tmp_path):
     exe = os.path.join(BIN, "mcs_sim_f32")
     r = subprocess.run([exe, "--dims", "32y32"], capture_output=True, text=True, timeout=60)
     assert r.returncode == 2 and "not of form" in r.stderr
+
+
+def test_lowpass_outputs_2d(mcs, orc, tmp_path):
+    """--lowpass: the amplitude and gz mode dump of calc_amplitude_lowpass at each output"""
+    out = str(tmp_path)
+    psi = orc.init_field((40, 48), -0.25, 0.3, 4)
+    np.save(os.path.join(out, "in.npy"), psi)
+    sim("f64", "--dims", "48x40", "--load", os.path.join(out, "in.npy"), "--steps", 0, "--lowpass", 0.9,
+        "--out", out)
+    lens = [48 * H, 40 * H, H]
+    want_k, want = orc.amplitude_lowpass(psi, lens, [H] * 3, 0.9)
+    got = np.load(os.path.join(out, "amplitude_00000000.npy"))
+    assert np.linalg.norm(got - want) / np.linalg.norm(want) < 1e-12
+    import gzip
+    with gzip.open(os.path.join(out, "lowpass_00000000.gz"), "rt") as f:
+        txt = f.read()
+    assert txt.startswith("LOWPASS\n") and txt.split("d.\n", 1)[0] == orc.lowpass_text(want_k, psi.shape, lens[:2]).split("d.\n", 1)[0]
+    r = subprocess.run([os.path.join(BIN, "mcs_sim_f64"), "--dims", "16x16x16", "--lowpass", "1"],
+                       capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "2D grid" in r.stderr
