@@ -105,6 +105,24 @@ def ensemble_bench(mcs, kind, steps):
     return out
 
 
+def analysis_bench(mcs, field, h):
+    """local maxima above the mean of an n^3 field, then 12-NN q4/q6/q8/w4 and 64-bin histograms of every peak"""
+    n = field.shape[0]
+    thr = float(field.mean())
+    t0 = time.perf_counter()
+    pk = mcs.findpeaks(field, thr, cap=field.size // 8 + 8)
+    t1 = time.perf_counter()
+    ijk = np.column_stack([pk % n, (pk // n) % n, pk // (n * n)]).astype(np.int32)
+    nbr, q, h4, h6 = mcs.bond_order_knn(ijk, n, h, 12)
+    t2 = time.perf_counter()
+    return {"workload": f"peaks >= mean of the {n}^3 FP32 field after 500 steps, then 12 nearest neighbours, "
+                        f"q4 q6 q8 w4 and 64-bin q4/q6 histograms of every peak (cell list, FP64)",
+            "peaks": int(pk.size), "findpeaks_ms": round(1e3 * (t1 - t0), 2), "bond_order_ms": round(1e3 * (t2 - t1), 2),
+            "timing": "wall clock of each call, including the field upload and the result downloads",
+            "q4_mean": float(q[:, 0].mean()), "q6_mean": float(q[:, 1].mean()),
+            "q6_mode_bin": int(np.argmax(h6))}
+
+
 def fft_bench(mcs, n, reps):
     """boundary FFT (the FFT::CufftPlan replacement) R2C and C2R on an n^3 FP32 field"""
     shp = (n, n, n)
@@ -220,13 +238,17 @@ def main():
         mcs.sync()
         assert st.steps_taken() == st64.steps_taken() == n_cmp, (st.steps_taken(), st64.steps_taken(), n_cmp)
         v64 = N * k64 / (ms64 / 1e3)
-        p32 = st.get_psi().astype(np.float64)
+        f32 = st.get_psi()
+        p32 = f32.astype(np.float64)
         p64 = st64.get_psi()
         rel = float(np.linalg.norm(p32 - p64) / np.linalg.norm(p64))
         out["fp64"] = {"value": v64, "ms_per_step": ms64 / k64, "speedup_fp32_over_fp64": value / v64,
                        "rel_l2_fp32_vs_fp64_gpu": rel, "max_abs_fp32_vs_fp64_gpu": float(np.abs(p32 - p64).max()),
                        "steps_compared": n_cmp}
         del st64, p32, p64
+        log("[bench] peaks and 12-NN bond order of the 500-step field")
+        out["analysis"] = analysis_bench(mcs, f32, cfg["dx"])
+        del f32
     log("[bench] boundary FFT")
     out["boundary_fft"] = fft_bench(mcs, n, 5)
     if not a.no_ensemble:

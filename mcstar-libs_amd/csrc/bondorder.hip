@@ -9,11 +9,14 @@
 // (267-287).  Y_lm terms with m < 0 vanish within 1e-3 of cos(theta) = -1 and
 // with m > 0 within 1e-3 of +1 (123-133).
 //
+// Cutoff mode: one thread per point scans all points in index order, so the
+// sums run in the reference's order (distances with contraction off, so
+// neighbour sets match the CPU bit for bit).
+//
 // k-nearest mode: integer squared distances on the periodic grid, ties broken
-// by the smaller index, exact Y_lm, q4 q6 q8 w4 and histograms of q4, q6.
-// One thread per point scans all points (pair distances are computed with
-// contraction off so neighbour sets match the CPU bit for bit).
-#include "mcs_common.hpp"
+// by the smaller index, exact Y_lm, q4 q6 q8 w4 and histograms of q4, q6.  The
+// neighbours come from a periodic cell list (below), not an all-pairs scan.
+#include "knn.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -174,118 +177,271 @@ __global__ void __launch_bounds__(128) k_bo_stats(const double* xyz, int P, doub
     o[8] = (double)xi;
 }
 
-MCS_DEV int min_image(int d, int n) {
-    d %= n;
+static constexpr int KMAX = 32;
+
+// ---------------------------------------------------------------- k nearest neighbours on a cell list
+//
+// Peaks sit on the integer grid, so squared periodic distances are exact
+// integers and the key (d2 << 32 | index) orders candidates strictly: the k
+// smallest keys form one set whatever order candidates arrive in, the same set
+// an all-pairs scan keeps (distance first, then the smaller index).
+//
+// Points are binned into a periodic grid of g0 x g1 x g2 cells; coordinate x
+// falls in cell x*g/n, so every cell spans floor(n/g) or ceil(n/g) grid lines.
+// A query visits shells of cells at Chebyshev cell distance s = 0, 1, 2, ...,
+// with per-axis offsets in [-(g-1)/2, g/2] so that each cell is visited once
+// also when a shell wraps around the box.  A cell not yet visited after shell
+// s lies s+1 or more cells away along some axis, so every point in it is at
+// least s*wmin + 1 grid lines away (wmin = the narrowest cell); the search
+// stops once the k-th key's d2 is below (s*wmin + 1)^2.  Cells are sized from
+// the point density so that the k-th neighbour usually lies within shell 1.
+
+struct KnnGrid {
+    int64_t P;   // points
+    int n[3];    // periodic extents
+    int g[3];    // cells per axis
+    int wmin;    // min over axes of floor(n / g)
+    int smax;    // shells that cover every cell: max over axes of g / 2
+    int ncell;
+};
+
+static constexpr int KNN_MAX_EXT = 32768;  // keeps 3 (n/2)^2 below 2^32 and x*g below 2^31
+
+MCS_DEV int knn_cell(const KnnGrid& G, int x, int y, int z) {
+    const int cx = x * G.g[0] / G.n[0], cy = y * G.g[1] / G.n[1], cz = z * G.g[2] / G.n[2];
+    return (cz * G.g[1] + cy) * G.g[0] + cx;
+}
+
+MCS_DEV int pmod(int a, int n) {
+    a %= n;
+    return a < 0 ? a + n : a;
+}
+
+// signed minimum image of a - b for a, b in [0, n): into (-n/2, n/2]
+MCS_DEV int min_image_norm(int a, int b, int n) {
+    int d = a - b;
     if (d < 0) d += n;
     if (2 * d > n) d -= n;
     return d;
 }
 
-static constexpr int KMAX = 32;
-
-__global__ void __launch_bounds__(128) k_bo_knn(const int* ijk, int P, int n, double h, int k, Wig4 W, int* nbr,
-                                                double* out, unsigned long long* h4, unsigned long long* h6,
-                                                int nbins) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= P) return;
-    long bd[KMAX];
-    int bj[KMAX];
-    int cnt = 0;
-    const int xi = ijk[3 * i], yi = ijk[3 * i + 1], zi = ijk[3 * i + 2];
-    for (int j = 0; j < P; ++j) {
-        if (j == i) continue;
-        const long dx = min_image(ijk[3 * j] - xi, n), dy = min_image(ijk[3 * j + 1] - yi, n),
-                   dz = min_image(ijk[3 * j + 2] - zi, n);
-        const long d2 = dx * dx + dy * dy + dz * dz;
-        if (cnt == k && !(d2 < bd[k - 1] || (d2 == bd[k - 1] && j < bj[k - 1]))) continue;
-        int pos = cnt < k ? cnt : k - 1;
-        while (pos > 0 && (bd[pos - 1] > d2 || (bd[pos - 1] == d2 && bj[pos - 1] > j))) {
-            bd[pos] = bd[pos - 1]; bj[pos] = bj[pos - 1]; --pos;
-        }
-        bd[pos] = d2; bj[pos] = j;
-        if (cnt < k) ++cnt;
+// cell grid from the point count: cells about as wide as the radius of the ball
+// that holds k points at the mean density, coarsened until at most cmax cells
+__global__ void k_knn_setup(const int64_t* d_count, int64_t P_host, int64_t cap, int nx, int ny, int nz, int k,
+                            int cmax, KnnGrid* out) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    int64_t P = d_count ? *d_count : P_host;
+    P = P < 0 ? 0 : (P > cap ? cap : P);
+    KnnGrid G;
+    G.P = P;
+    G.n[0] = nx; G.n[1] = ny; G.n[2] = nz;
+    const int dims = (nx > 1) + (ny > 1) + (nz > 1);
+    int w = KNN_MAX_EXT;
+    if (P > 1 && dims > 0) {
+        const double per = (double)nx * ny * nz * k / (double)P;  // volume that holds k points on average
+        const double r = dims == 3 ? cbrt(per * 3.0 / (4.0 * M_PI)) : (dims == 2 ? sqrt(per / M_PI) : 0.5 * per);
+        w = r < (double)KNN_MAX_EXT ? (int)ceil(r) : KNN_MAX_EXT;
+        w = w < 1 ? 1 : w;
     }
-    for (int t = 0; t < k; ++t) nbr[(long)k * i + t] = t < cnt ? bj[t] : -1;
+    for (;;) {
+        long cells = 1;
+        for (int a = 0; a < 3; ++a) {
+            const int g = G.n[a] / w;
+            G.g[a] = g < 1 ? 1 : g;
+            cells *= G.g[a];
+        }
+        if (cells <= cmax || w >= KNN_MAX_EXT) break;
+        w *= 2;
+    }
+    G.wmin = KNN_MAX_EXT;
+    G.smax = 0;
+    G.ncell = 1;
+    for (int a = 0; a < 3; ++a) {
+        const int wa = G.n[a] / G.g[a];
+        G.wmin = wa < G.wmin ? wa : G.wmin;
+        G.smax = G.g[a] / 2 > G.smax ? G.g[a] / 2 : G.smax;
+        G.ncell *= G.g[a];
+    }
+    *out = G;
+}
+
+__global__ void __launch_bounds__(256) k_knn_zero(const KnnGrid* gp, int* cnt) {
+    const int nc = gp->ncell;
+    for (int c = blockIdx.x * 256 + threadIdx.x; c <= nc; c += gridDim.x * 256) cnt[c] = 0;
+}
+
+// points -> (x, y, z, index) with coordinates in [0, n), and the count of every cell
+__global__ void __launch_bounds__(256) k_knn_bin(const KnnGrid* gp, const int64_t* idx, const int* ijk, int4* pts,
+                                                 int* cnt) {
+    const KnnGrid G = *gp;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < G.P; i += (int64_t)gridDim.x * 256) {
+        int x, y, z;
+        if (idx) {
+            const int64_t c = idx[i];
+            x = (int)(c % G.n[0]);
+            y = (int)((c / G.n[0]) % G.n[1]);
+            z = (int)((c / ((int64_t)G.n[0] * G.n[1])) % G.n[2]);
+        } else {
+            x = pmod(ijk[3 * i], G.n[0]);
+            y = pmod(ijk[3 * i + 1], G.n[1]);
+            z = pmod(ijk[3 * i + 2], G.n[2]);
+        }
+        pts[i] = make_int4(x, y, z, (int)i);
+        atomicAdd(&cnt[knn_cell(G, x, y, z)], 1);
+    }
+}
+
+// exclusive scan of the cell counts into start[0..ncell] (start[ncell] = P), one block
+__global__ void __launch_bounds__(1024) k_knn_scan(const KnnGrid* gp, const int* cnt, int* start) {
+    __shared__ int wsum[16];
+    const int nc = gp->ncell;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    int carry = 0;
+    for (int base = 0; base < nc; base += 4096) {
+        const int b = base + 4 * (int)threadIdx.x;
+        int v[4], s = 0;
+        for (int e = 0; e < 4; ++e) {
+            v[e] = b + e < nc ? cnt[b + e] : 0;
+            s += v[e];
+        }
+        int x = s;
+        for (int o = 1; o < 64; o <<= 1) {
+            const int y = __shfl_up(x, o);
+            if (lane >= o) x += y;
+        }
+        if (lane == 63) wsum[wv] = x;
+        __syncthreads();
+        int pre = 0, tot = 0;
+        for (int q = 0; q < 16; ++q) {
+            pre += q < wv ? wsum[q] : 0;
+            tot += wsum[q];
+        }
+        int run = carry + pre + x - s;
+        for (int e = 0; e < 4; ++e) {
+            if (b + e < nc) start[b + e] = run;
+            run += v[e];
+        }
+        carry += tot;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) start[nc] = carry;
+}
+
+// points into cell order (the order inside a cell is arbitrary; the result does not depend on it)
+__global__ void __launch_bounds__(256) k_knn_scatter(const KnnGrid* gp, const int4* pts, const int* start, int* cnt,
+                                                     int4* sorted) {
+    const KnnGrid G = *gp;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < G.P; i += (int64_t)gridDim.x * 256) {
+        const int4 p = pts[i];
+        const int c = knn_cell(G, p.x, p.y, p.z);
+        sorted[start[c] + atomicSub(&cnt[c], 1) - 1] = p;
+    }
+}
+
+// one thread per point, threads in cell order: the k nearest keys in registers
+// (kept sorted, KC >= k slots), then the bond harmonics in key order
+template <int KC, bool FULL>
+__global__ void __launch_bounds__(128) k_knn_query(const KnnGrid* gp, const int4* __restrict__ pts,
+                                                   const int4* __restrict__ sorted, const int* __restrict__ start,
+                                                   int k, double h, Wig4 W, int* nbr, double* out,
+                                                   unsigned long long* h4, unsigned long long* h6, int nbins) {
+    __shared__ int js[KC][128];
+    const KnnGrid G = *gp;
+    const int64_t t = (int64_t)blockIdx.x * 128 + threadIdx.x;
+    if (t >= G.P) return;
+    const int4 me = sorted[t];
+    const int c0 = me.x * G.g[0] / G.n[0], c1 = me.y * G.g[1] / G.n[1], c2 = me.z * G.g[2] / G.n[2];
+    unsigned long long best[KC];
+#pragma unroll
+    for (int e = 0; e < KC; ++e) best[e] = ~0ull;
+    unsigned long long kth = ~0ull;
+    int found = 0;
+    for (int s = 0; s <= G.smax; ++s) {
+        int lo[3], hi[3];
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = max(-s, -((G.g[a] - 1) / 2));
+            hi[a] = min(s, G.g[a] / 2);
+        }
+        for (int oz = lo[2]; oz <= hi[2]; ++oz) {
+            const int cz = pmod(c2 + oz, G.g[2]);
+            for (int oy = lo[1]; oy <= hi[1]; ++oy) {
+                const int cy = pmod(c1 + oy, G.g[1]);
+                const int row = (cz * G.g[1] + cy) * G.g[0];
+                // rows on the shell's faces take every x offset, inner rows only the two x ends
+                const bool face = abs(oz) == s || abs(oy) == s;
+                const int xb = face ? lo[0] : (-s >= lo[0] ? -s : s);
+                const int xe = face ? hi[0] : (s <= hi[0] ? s : -s);
+                const int xs = face ? 1 : 2 * s;
+                for (int ox = xb; ox <= xe; ox += xs) {
+                    const int cell = row + pmod(c0 + ox, G.g[0]);
+                    const int pe = start[cell + 1];
+                    for (int p = start[cell]; p < pe; ++p) {
+                        const int4 q = sorted[p];
+                        if (q.w == me.w) continue;
+                        int dx = abs(q.x - me.x), dy = abs(q.y - me.y), dz = abs(q.z - me.z);
+                        dx = min(dx, G.n[0] - dx);
+                        dy = min(dy, G.n[1] - dy);
+                        dz = min(dz, G.n[2] - dz);
+                        const unsigned long long d2 =
+                            (unsigned long long)(unsigned)(dx * dx) + (unsigned)(dy * dy) + (unsigned)(dz * dz);
+                        const unsigned long long key = (d2 << 32) | (unsigned)q.w;
+                        if (key >= kth) continue;
+                        unsigned long long carry = key;  // insert, dropping the largest
+#pragma unroll
+                        for (int e = 0; e < KC; ++e) {
+                            const unsigned long long lo_e = best[e] < carry ? best[e] : carry;
+                            carry = best[e] < carry ? carry : best[e];
+                            best[e] = lo_e;
+                        }
+                        if (found < k) ++found;
+                        if (found == k) {
+                            unsigned long long v = best[0];
+#pragma unroll
+                            for (int e = 1; e < KC; ++e) v = e == k - 1 ? best[e] : v;
+                            kth = v;
+                        }
+                    }
+                }
+            }
+        }
+        const unsigned long long b = (unsigned long long)s * G.wmin + 1;
+        if (found == k && (kth >> 32) < b * b) break;
+    }
+    const int i = me.w;
+#pragma unroll
+    for (int e = 0; e < KC; ++e) js[e][threadIdx.x] = e < found ? (int)(unsigned)(best[e] & 0xffffffffull) : -1;
+    if (FULL)
+        for (int e = 0; e < k; ++e) nbr[(int64_t)k * i + e] = js[e][threadIdx.x];
+    if (found == 0) {
+        if (FULL)
+            for (int e = 0; e < 4; ++e) out[4L * i + e] = __longlong_as_double(0x7ff8000000000000LL);
+        return;
+    }
     double acc[78];
-    for (int t = 0; t < 78; ++t) acc[t] = 0;
-    for (int t = 0; t < cnt; ++t) {
-        const int j = bj[t];
-        const double dx = h * min_image(ijk[3 * j] - xi, n), dy = h * min_image(ijk[3 * j + 1] - yi, n),
-                     dz = h * min_image(ijk[3 * j + 2] - zi, n);
+    for (int e = 0; e < 78; ++e) acc[e] = 0;
+    for (int e = 0; e < found; ++e) {
+        const int4 q = pts[js[e][threadIdx.x]];
+        const double dx = h * min_image_norm(q.x, me.x, G.n[0]), dy = h * min_image_norm(q.y, me.y, G.n[1]),
+                     dz = h * min_image_norm(q.z, me.z, G.n[2]);
         const double r = sqrt(dx * dx + dy * dy + dz * dz);
         add_bond(acc, dz / r, atan2(dy, dx), false);
     }
-    double* o = out + 4L * i;
-    if (cnt == 0) {
-        for (int t = 0; t < 4; ++t) o[t] = __longlong_as_double(0x7ff8000000000000LL);
-        return;
+    for (int e = 0; e < 78; ++e) acc[e] /= (double)found;
+    const double q4 = ql_dev(acc, 4), q6 = ql_dev(acc + 18, 6);
+    if (FULL) {
+        double* o = out + 4L * i;
+        o[0] = q4;
+        o[1] = q6;
+        o[2] = ql_dev(acc + 44, 8);
+        o[3] = w4_dev(acc, W);
     }
-    for (int t = 0; t < 78; ++t) acc[t] /= (double)cnt;
-    o[0] = ql_dev(acc, 4);
-    o[1] = ql_dev(acc + 18, 6);
-    o[2] = ql_dev(acc + 44, 8);
-    o[3] = w4_dev(acc, W);
     if (h4 && nbins > 0) {
-        int b4 = (int)(o[0] * nbins), b6 = (int)(o[1] * nbins);
+        int b4 = (int)(q4 * nbins), b6 = (int)(q6 * nbins);
         b4 = b4 < 0 ? 0 : (b4 >= nbins ? nbins - 1 : b4);
         b6 = b6 < 0 ? 0 : (b6 >= nbins ? nbins - 1 : b6);
         atomicAdd(&h4[b4], 1ull);
         atomicAdd(&h6[b6], 1ull);
     }
-}
-
-// device-resident peaks (linear cell indices, count on the device) -> ijk
-__global__ void __launch_bounds__(256) k_idx_to_ijk(const int64_t* idx, const int64_t* d_count, int64_t cap, int nx,
-                                                    int ny, int* ijk) {
-    const int64_t P = *d_count < cap ? *d_count : cap;
-    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < P; i += (int64_t)gridDim.x * 256) {
-        const int64_t c = idx[i];
-        ijk[3 * i] = (int)(c % nx);
-        ijk[3 * i + 1] = (int)((c / nx) % ny);
-        ijk[3 * i + 2] = (int)(c / ((int64_t)nx * ny));
-    }
-}
-
-// k nearest neighbours on a periodic nx*ny*nz grid (spacing h) and q4/q6 histograms only
-__global__ void __launch_bounds__(128) k_bo_knn_hist(const int* ijk, const int64_t* d_count, int64_t cap, int nx,
-                                                     int ny, int nz, double h, int k, Wig4 W,
-                                                     unsigned long long* h4, unsigned long long* h6, int nbins) {
-    const int P = (int)(*d_count < cap ? *d_count : cap);
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= P) return;
-    long bd[KMAX];
-    int bj[KMAX];
-    int cnt = 0;
-    const int xi = ijk[3 * i], yi = ijk[3 * i + 1], zi = ijk[3 * i + 2];
-    for (int j = 0; j < P; ++j) {
-        if (j == i) continue;
-        const long dx = min_image(ijk[3 * j] - xi, nx), dy = min_image(ijk[3 * j + 1] - yi, ny),
-                   dz = min_image(ijk[3 * j + 2] - zi, nz);
-        const long d2 = dx * dx + dy * dy + dz * dz;
-        if (cnt == k && !(d2 < bd[k - 1] || (d2 == bd[k - 1] && j < bj[k - 1]))) continue;
-        int pos = cnt < k ? cnt : k - 1;
-        while (pos > 0 && (bd[pos - 1] > d2 || (bd[pos - 1] == d2 && bj[pos - 1] > j))) {
-            bd[pos] = bd[pos - 1]; bj[pos] = bj[pos - 1]; --pos;
-        }
-        bd[pos] = d2; bj[pos] = j;
-        if (cnt < k) ++cnt;
-    }
-    if (cnt == 0) return;
-    double acc[78];
-    for (int t = 0; t < 78; ++t) acc[t] = 0;
-    for (int t = 0; t < cnt; ++t) {
-        const int j = bj[t];
-        const double dx = h * min_image(ijk[3 * j] - xi, nx), dy = h * min_image(ijk[3 * j + 1] - yi, ny),
-                     dz = h * min_image(ijk[3 * j + 2] - zi, nz);
-        const double r = sqrt(dx * dx + dy * dy + dz * dz);
-        add_bond(acc, dz / r, atan2(dy, dx), false);
-    }
-    for (int t = 0; t < 78; ++t) acc[t] /= (double)cnt;
-    int b4 = (int)(ql_dev(acc, 4) * nbins), b6 = (int)(ql_dev(acc + 18, 6) * nbins);
-    b4 = b4 < 0 ? 0 : (b4 >= nbins ? nbins - 1 : b4);
-    b6 = b6 < 0 ? 0 : (b6 >= nbins ? nbins - 1 : b6);
-    atomicAdd(&h4[b4], 1ull);
-    atomicAdd(&h6[b6], 1ull);
 }
 
 // Wigner 3j (4 4 4; m1 m2 m3) by the Racah formula, on the host
@@ -313,15 +469,78 @@ static Wig4 wig4() {
     return W;
 }
 
-int knn_hist_launch(const int64_t* d_idx, const int64_t* d_count, int64_t cap, const int ext[3], double h, int k,
-                    int* d_ijk, unsigned long long* d_h4, unsigned long long* d_h6, int nbins, hipStream_t s) {
+struct KnnWs {
+    KnnGrid* grid;
+    int4* pts;
+    int4* sorted;
+    int* cnt;
+    int* start;
+    int cmax;
+};
+
+static size_t al256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+static int knn_cmax(int64_t cap) { return (int)(cap + 64); }
+
+size_t knn_workspace_bytes(int64_t cap) {
+    const size_t c = (size_t)(cap > 0 ? cap : 1);
+    const size_t nc = (size_t)knn_cmax(cap) + 1;
+    return al256(sizeof(KnnGrid)) + 2 * al256(c * sizeof(int4)) + 2 * al256(nc * sizeof(int));
+}
+
+static KnnWs knn_carve(void* ws, int64_t cap) {
+    const size_t c = (size_t)(cap > 0 ? cap : 1);
+    const size_t nc = (size_t)knn_cmax(cap) + 1;
+    unsigned char* p = reinterpret_cast<unsigned char*>(ws);
+    KnnWs w;
+    w.grid = reinterpret_cast<KnnGrid*>(p); p += al256(sizeof(KnnGrid));
+    w.pts = reinterpret_cast<int4*>(p); p += al256(c * sizeof(int4));
+    w.sorted = reinterpret_cast<int4*>(p); p += al256(c * sizeof(int4));
+    w.cnt = reinterpret_cast<int*>(p); p += al256(nc * sizeof(int));
+    w.start = reinterpret_cast<int*>(p);
+    w.cmax = knn_cmax(cap);
+    return w;
+}
+
+template <int KC>
+static void knn_query_launch(bool full, unsigned grid, hipStream_t s, const KnnWs& w, int k, double h, const Wig4& W,
+                             int* nbr, double* out, unsigned long long* h4, unsigned long long* h6, int nbins) {
+    if (full)
+        hipLaunchKernelGGL((k_knn_query<KC, true>), dim3(grid), dim3(128), 0, s, w.grid, w.pts, w.sorted, w.start, k, h,
+                           W, nbr, out, h4, h6, nbins);
+    else
+        hipLaunchKernelGGL((k_knn_query<KC, false>), dim3(grid), dim3(128), 0, s, w.grid, w.pts, w.sorted, w.start, k,
+                           h, W, nbr, out, h4, h6, nbins);
+}
+
+int knn_launch(const int64_t* d_idx, const int* d_ijk, const int64_t* d_count, int64_t P_host, int64_t cap,
+               const int ext[3], double h, int k, void* ws, int* d_nbr, double* d_out4, unsigned long long* d_h4,
+               unsigned long long* d_h6, int nbins, hipStream_t s) {
     MCS_CHECK(k >= 1 && k <= KMAX, MCS_ERR_INVALID, "k must be in [1, 32]");
-    const unsigned g1 = (unsigned)std::min<int64_t>(4096, (cap + 255) / 256);
-    hipLaunchKernelGGL(k_idx_to_ijk, dim3(g1 ? g1 : 1), dim3(256), 0, s, d_idx, d_count, cap, ext[0], ext[1], d_ijk);
+    MCS_CHECK(ext[0] >= 1 && ext[1] >= 1 && ext[2] >= 1 && ext[0] <= KNN_MAX_EXT && ext[1] <= KNN_MAX_EXT &&
+                  ext[2] <= KNN_MAX_EXT,
+              MCS_ERR_INVALID, "bond-order grid extents must be in [1, 32768]");
+    MCS_CHECK(cap >= 0 && cap <= (int64_t)0x7fffff00, MCS_ERR_INVALID, "too many points for bond order");
+    MCS_CHECK((d_idx != nullptr) != (d_ijk != nullptr), MCS_ERR_INVALID, "pass either indices or coordinates");
+    MCS_CHECK(!(d_h4 || d_h6) || (d_h4 && d_h6 && nbins > 0), MCS_ERR_INVALID, "bad histogram args");
+    if (cap == 0) return MCS_OK;
+    const KnnWs w = knn_carve(ws, cap);
     const Wig4 W = wig4();
-    const unsigned g2 = (unsigned)((cap + 127) / 128);
-    hipLaunchKernelGGL(k_bo_knn_hist, dim3(g2 ? g2 : 1), dim3(128), 0, s, d_ijk, d_count, cap, ext[0], ext[1], ext[2],
-                       h, k, W, d_h4, d_h6, nbins);
+    const unsigned gp = (unsigned)std::min<int64_t>(4096, (cap + 255) / 256);
+    const unsigned gz = (unsigned)std::min<int64_t>(1024, ((int64_t)w.cmax + 256) / 256);
+    hipLaunchKernelGGL(k_knn_setup, dim3(1), dim3(64), 0, s, d_count, P_host, cap, ext[0], ext[1], ext[2], k, w.cmax,
+                       w.grid);
+    hipLaunchKernelGGL(k_knn_zero, dim3(gz), dim3(256), 0, s, w.grid, w.cnt);
+    hipLaunchKernelGGL(k_knn_bin, dim3(gp), dim3(256), 0, s, w.grid, d_idx, d_ijk, w.pts, w.cnt);
+    hipLaunchKernelGGL(k_knn_scan, dim3(1), dim3(1024), 0, s, w.grid, w.cnt, w.start);
+    hipLaunchKernelGGL(k_knn_scatter, dim3(gp), dim3(256), 0, s, w.grid, w.pts, w.start, w.cnt, w.sorted);
+    const unsigned gq = (unsigned)((cap + 127) / 128);
+    const bool full = d_nbr != nullptr;
+    MCS_CHECK(!full || d_out4, MCS_ERR_INVALID, "neighbour output needs the q output");
+    if (k <= 8) knn_query_launch<8>(full, gq, s, w, k, h, W, d_nbr, d_out4, d_h4, d_h6, nbins);
+    else if (k <= 12) knn_query_launch<12>(full, gq, s, w, k, h, W, d_nbr, d_out4, d_h4, d_h6, nbins);
+    else if (k <= 16) knn_query_launch<16>(full, gq, s, w, k, h, W, d_nbr, d_out4, d_h4, d_h6, nbins);
+    else knn_query_launch<32>(full, gq, s, w, k, h, W, d_nbr, d_out4, d_h4, d_h6, nbins);
     MCS_HIP_TRY(hipGetLastError());
     return MCS_OK;
 }
@@ -361,7 +580,8 @@ extern "C" int mcs_bond_order_cutoff(const double* xyz, int npts, double box, do
 
 extern "C" int mcs_bond_order_knn(const int* ijk, int npts, int n, double h, int k, int* nbr, double* out4,
                                   int64_t* hist_q4, int64_t* hist_q6, int nbins) {
-    MCS_CHECK(npts >= 0 && n >= 1 && k >= 1 && k <= KMAX, MCS_ERR_INVALID, "bad argument (1 <= k <= 32)");
+    MCS_CHECK(npts >= 0 && n >= 1 && n <= KNN_MAX_EXT && k >= 1 && k <= KMAX, MCS_ERR_INVALID,
+              "bad argument (1 <= k <= 32, 1 <= n <= 32768)");
     MCS_CHECK(!(hist_q4 || hist_q6) || (hist_q4 && hist_q6 && nbins > 0), MCS_ERR_INVALID, "bad histogram args");
     if (hist_q4) for (int b = 0; b < nbins; ++b) { hist_q4[b] = 0; hist_q6[b] = 0; }
     if (npts == 0) return MCS_OK;
@@ -370,35 +590,36 @@ extern "C" int mcs_bond_order_knn(const int* ijk, int npts, int n, double h, int
     int *dijk = nullptr, *dnbr = nullptr;
     double* dout = nullptr;
     unsigned long long* dh = nullptr;
+    void* ws = nullptr;
     int st = MCS_OK;
     const int nb2 = hist_q4 ? 2 * nbins : 2;
     if (hipMalloc((void**)&dijk, 3L * npts * sizeof(int)) != hipSuccess ||
         hipMalloc((void**)&dnbr, (long)k * npts * sizeof(int)) != hipSuccess ||
         hipMalloc((void**)&dout, 4L * npts * sizeof(double)) != hipSuccess ||
-        hipMalloc((void**)&dh, nb2 * sizeof(unsigned long long)) != hipSuccess) {
+        hipMalloc((void**)&dh, nb2 * sizeof(unsigned long long)) != hipSuccess ||
+        hipMalloc(&ws, knn_workspace_bytes(npts)) != hipSuccess) {
         mcs_set_last_error(__FILE__, __LINE__, "bond-order allocation failed");
         st = MCS_ERR_NOMEM;
     }
+    std::vector<unsigned long long> hh(nb2);
     if (!st) {
-        const Wig4 W = wig4();
-        const unsigned g = (unsigned)((npts + 127) / 128);
+        const int ext[3] = {n, n, n};
         if (hipMemcpyAsync(dijk, ijk, 3L * npts * sizeof(int), hipMemcpyHostToDevice, s) != hipSuccess) st = MCS_ERR_HIP;
         if (!st && hipMemsetAsync(dh, 0, nb2 * sizeof(unsigned long long), s) != hipSuccess) st = MCS_ERR_HIP;
-        hipLaunchKernelGGL(k_bo_knn, dim3(g), dim3(128), 0, s, dijk, npts, n, h, k, W, dnbr, dout,
-                           hist_q4 ? dh : nullptr, hist_q4 ? dh + nbins : nullptr, nbins);
-        if (!st && hipGetLastError() != hipSuccess) st = MCS_ERR_HIP;
+        if (!st) st = knn_launch(nullptr, dijk, nullptr, npts, npts, ext, h, k, ws, dnbr, dout,
+                                 hist_q4 ? dh : nullptr, hist_q4 ? dh + nbins : nullptr, nbins, s);
         if (!st && hipMemcpyAsync(nbr, dnbr, (long)k * npts * sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess)
             st = MCS_ERR_HIP;
         if (!st && hipMemcpyAsync(out4, dout, 4L * npts * sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess)
             st = MCS_ERR_HIP;
-        std::vector<unsigned long long> hh(nb2);
         if (!st && hipMemcpyAsync(hh.data(), dh, nb2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s) != hipSuccess)
             st = MCS_ERR_HIP;
         if (!st && hipStreamSynchronize(s) != hipSuccess) st = MCS_ERR_HIP;
         if (!st && hist_q4)
             for (int b = 0; b < nbins; ++b) { hist_q4[b] = (int64_t)hh[b]; hist_q6[b] = (int64_t)hh[nbins + b]; }
-        if (st) mcs_set_last_error(__FILE__, __LINE__, "HIP failure in bond order");
+        if (st == MCS_ERR_HIP) mcs_set_last_error(__FILE__, __LINE__, "HIP failure in bond order");
     }
-    (void)hipFree(dijk); (void)hipFree(dnbr); (void)hipFree(dout); (void)hipFree(dh);
+    if (st) (void)hipStreamSynchronize(s);
+    (void)hipFree(dijk); (void)hipFree(dnbr); (void)hipFree(dout); (void)hipFree(dh); (void)hipFree(ws);
     return st;
 }

@@ -12,6 +12,7 @@
 // looping baseline); MCS_ENS_GRAPH replays a captured graph of several steps
 // in which every size group is an independent branch, so that the groups'
 // kernels overlap on the GPU.
+#include "knn.hpp"
 #include "peaks.hpp"
 #include "stepper.hpp"
 
@@ -118,7 +119,7 @@ struct mcs_ensemble {
     std::vector<void*> a_bufs;               // every allocation below, freed together
     std::vector<void*> a_psi, a_tmp, a_ws;   // per group: one replica's field, export scratch, peak workspace
     std::vector<int64_t*> a_idx;             // per group: peak indices (capacity n/8 + 8)
-    std::vector<int*> a_ijk;                 // per group: peak coordinates
+    std::vector<void*> a_knn;                // per group: bond-order cell-list workspace
     int64_t* a_counts = nullptr;             // [nrep]
     unsigned long long* a_hist = nullptr;    // [nrep][2][nbins]: q4 then q6
 };
@@ -283,9 +284,6 @@ static int ens_steps(mcs_ensemble* e, int nsteps, int mode, hipStream_t s) {
     return e->prec == 64 ? ens_steps_t<double>(e, nsteps, mode, s) : ens_steps_t<float>(e, nsteps, mode, s);
 }
 
-int knn_hist_launch(const int64_t* d_idx, const int64_t* d_count, int64_t cap, const int ext[3], double h, int k,
-                    int* d_ijk, unsigned long long* d_h4, unsigned long long* d_h6, int nbins, hipStream_t s);
-
 static int64_t peak_cap(const mcs_replica& p) { return (int64_t)p.ext[0] * p.ext[1] * p.ext[2] / 8 + 8; }
 
 // analysis of the members of group gi on stream b: export, peaks >= ths, kNN q4/q6 histograms
@@ -300,8 +298,8 @@ static int ens_analysis_group(mcs_ensemble* e, int gi, hipStream_t b) {
         int st = step_export(g->sg, e->U[r], e->a_psi[gi], e->a_tmp[gi], b);
         if (!st) st = findpeaks_launch(e->a_psi[gi], p.ext, e->a_ths, e->a_idx[gi], nullptr, peak_cap(p),
                                        e->a_counts + r, e->a_ws[gi], e->prec, b);
-        if (!st) st = knn_hist_launch(e->a_idx[gi], e->a_counts + r, peak_cap(p), p.ext, p.h[0], e->a_k, e->a_ijk[gi],
-                                      h, h + nb, nb, b);
+        if (!st) st = knn_launch(e->a_idx[gi], nullptr, e->a_counts + r, 0, peak_cap(p), p.ext, p.h[0], e->a_k,
+                                 e->a_knn[gi], nullptr, nullptr, h, h + nb, nb, b);
         if (st) return st;
     }
     return MCS_OK;
@@ -367,7 +365,7 @@ static void ens_analysis_free(mcs_ensemble* e) {
     e->agraph = nullptr;
     for (void* p : e->a_bufs) (void)hipFree(p);
     e->a_bufs.clear();
-    e->a_psi.clear(); e->a_tmp.clear(); e->a_ws.clear(); e->a_idx.clear(); e->a_ijk.clear();
+    e->a_psi.clear(); e->a_tmp.clear(); e->a_ws.clear(); e->a_idx.clear(); e->a_knn.clear();
     e->a_counts = nullptr;
     e->a_hist = nullptr;
     e->a_k = e->a_nbins = 0;
@@ -392,8 +390,8 @@ static int ens_analysis_build(mcs_ensemble* e, double ths, int k, int nbins, hip
         e->a_tmp.push_back(dalloc(n * fb));
         e->a_ws.push_back(dalloc(peaks_workspace_bytes(p.ext)));
         e->a_idx.push_back((int64_t*)dalloc(peak_cap(p) * sizeof(int64_t)));
-        e->a_ijk.push_back((int*)dalloc(3 * peak_cap(p) * sizeof(int)));
-        ok = ok && e->a_psi.back() && e->a_tmp.back() && e->a_ws.back() && e->a_idx.back() && e->a_ijk.back();
+        e->a_knn.push_back(dalloc(knn_workspace_bytes(peak_cap(p))));
+        ok = ok && e->a_psi.back() && e->a_tmp.back() && e->a_ws.back() && e->a_idx.back() && e->a_knn.back();
     }
     e->a_counts = (int64_t*)dalloc(e->reps.size() * sizeof(int64_t));
     e->a_hist = (unsigned long long*)dalloc(2 * (size_t)nbins * e->reps.size() * sizeof(unsigned long long));
@@ -623,13 +621,13 @@ extern "C" int mcs_ensemble_analyze(mcs_ensemble* e, double ths, int k, int nbin
         void* psi = dalloc(n * (e->prec == 64 ? 8 : 4));
         void* tmp = dalloc(n * (e->prec == 64 ? 8 : 4));
         int64_t* idx = (int64_t*)dalloc(cap * sizeof(int64_t));
-        int* ijk = (int*)dalloc(3 * cap * sizeof(int));
+        void* kws = dalloc(knn_workspace_bytes(cap));
         void* ws = dalloc(peaks_workspace_bytes(p.ext));
-        if (!psi || !tmp || !idx || !ijk || !ws) { rc = MCS_ERR_NOMEM; break; }
+        if (!psi || !tmp || !idx || !kws || !ws) { rc = MCS_ERR_NOMEM; break; }
         rc = step_export(e->groups[e->group_of[r]]->sg, e->U[r], psi, tmp, s);
         if (!rc) rc = findpeaks_launch(psi, p.ext, ths, idx, nullptr, cap, dcnt + r, ws, e->prec, s);
-        if (!rc) rc = knn_hist_launch(idx, dcnt + r, cap, p.ext, p.h[0], k, ijk, dh + 2L * nbins * r,
-                                      dh + 2L * nbins * r + nbins, nbins, s);
+        if (!rc) rc = knn_launch(idx, nullptr, dcnt + r, 0, cap, p.ext, p.h[0], k, kws, nullptr, nullptr,
+                                 dh + 2L * nbins * r, dh + 2L * nbins * r + nbins, nbins, s);
     }
     std::vector<unsigned long long> hh(2 * (size_t)nbins * nrep);
     if (!rc && hipMemcpyAsync(hh.data(), dh, hh.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost, s) != hipSuccess)

@@ -828,52 +828,63 @@ static inline int min_image(int d, int n) {
     return d;
 }
 
-void orc_bond_order_knn(const int* ijk, int P, int n, double h, int k, int* nbr, double* out) {
+/* the k nearest neighbours of point i by an all-pairs scan and its q4 q6 q8 w4 (the
+   12-NN form asked for beside bond-order.cpp's cutoff form); bestd/bestj: k slots */
+static void knn_point(const int* ijk, int P, int n, double h, int k, int i, long* bestd, int* bestj, int* nbr,
+                      double* o) {
+    int cnt = 0;
+    for (int j = 0; j < P; j++) {
+        if (j == i) continue;
+        const long dx = min_image(ijk[3 * j] - ijk[3 * i], n);
+        const long dy = min_image(ijk[3 * j + 1] - ijk[3 * i + 1], n);
+        const long dz = min_image(ijk[3 * j + 2] - ijk[3 * i + 2], n);
+        const long d2 = dx * dx + dy * dy + dz * dz;
+        /* insert (d2, j) keeping the k smallest, ordered by (d2, j) */
+        if (cnt == k && !(d2 < bestd[k - 1] || (d2 == bestd[k - 1] && j < bestj[k - 1]))) continue;
+        int pos = cnt < k ? cnt : k - 1;
+        while (pos > 0 && (bestd[pos - 1] > d2 || (bestd[pos - 1] == d2 && bestj[pos - 1] > j))) {
+            bestd[pos] = bestd[pos - 1]; bestj[pos] = bestj[pos - 1]; pos--;
+        }
+        bestd[pos] = d2; bestj[pos] = j;
+        if (cnt < k) cnt++;
+    }
+    for (int t = 0; t < k; t++) nbr[t] = t < cnt ? bestj[t] : -1;
+    cplx q4[9] = {0}, q6[13] = {0}, q8[17] = {0};
+    for (int t = 0; t < cnt; t++) {
+        const int j = bestj[t];
+        const double dx = h * min_image(ijk[3 * j] - ijk[3 * i], n);
+        const double dy = h * min_image(ijk[3 * j + 1] - ijk[3 * i + 1], n);
+        const double dz = h * min_image(ijk[3 * j + 2] - ijk[3 * i + 2], n);
+        const double r = sqrt(dx * dx + dy * dy + dz * dz);
+        const double ct = dz / r, ph = atan2(dy, dx);
+        double re, im;
+        for (int m = -4; m <= 4; m++) { orc_ylm(4, m, ct, ph, &re, &im); q4[m + 4] += re + I * im; }
+        for (int m = -6; m <= 6; m++) { orc_ylm(6, m, ct, ph, &re, &im); q6[m + 6] += re + I * im; }
+        for (int m = -8; m <= 8; m++) { orc_ylm(8, m, ct, ph, &re, &im); q8[m + 8] += re + I * im; }
+    }
+    if (cnt == 0) { o[0] = o[1] = o[2] = o[3] = NAN; return; }
+    for (int m = 0; m < 9; m++) q4[m] /= cnt;
+    for (int m = 0; m < 13; m++) q6[m] /= cnt;
+    for (int m = 0; m < 17; m++) q8[m] /= cnt;
+    o[0] = ql_of(q4, 4); o[1] = ql_of(q6, 6); o[2] = ql_of(q8, 8); o[3] = w4_of(q4);
+}
+
+/* queries qi[0..nq) (or every point when qi is NULL, nq = P): row q of nbr/out for point qi[q] */
+void orc_bond_order_knn_points(const int* ijk, int P, int n, double h, int k, const int* qi, int nq, int* nbr,
+                               double* out) {
 #pragma omp parallel
     {
         long* bestd = (long*)malloc(sizeof(long) * (size_t)(k > 0 ? k : 1));
         int* bestj = (int*)malloc(sizeof(int) * (size_t)(k > 0 ? k : 1));
 #pragma omp for schedule(dynamic, 16)
-        for (int i = 0; i < P; i++) {
-            int cnt = 0;
-            for (int j = 0; j < P; j++) {
-                if (j == i) continue;
-                const long dx = min_image(ijk[3 * j] - ijk[3 * i], n);
-                const long dy = min_image(ijk[3 * j + 1] - ijk[3 * i + 1], n);
-                const long dz = min_image(ijk[3 * j + 2] - ijk[3 * i + 2], n);
-                const long d2 = dx * dx + dy * dy + dz * dz;
-                /* insert (d2, j) keeping the k smallest, ordered by (d2, j) */
-                if (cnt == k && !(d2 < bestd[k - 1] || (d2 == bestd[k - 1] && j < bestj[k - 1]))) continue;
-                int pos = cnt < k ? cnt : k - 1;
-                while (pos > 0 && (bestd[pos - 1] > d2 || (bestd[pos - 1] == d2 && bestj[pos - 1] > j))) {
-                    bestd[pos] = bestd[pos - 1]; bestj[pos] = bestj[pos - 1]; pos--;
-                }
-                bestd[pos] = d2; bestj[pos] = j;
-                if (cnt < k) cnt++;
-            }
-            for (int t = 0; t < k; t++) nbr[(long)k * i + t] = t < cnt ? bestj[t] : -1;
-            cplx q4[9] = {0}, q6[13] = {0}, q8[17] = {0};
-            for (int t = 0; t < cnt; t++) {
-                const int j = bestj[t];
-                const double dx = h * min_image(ijk[3 * j] - ijk[3 * i], n);
-                const double dy = h * min_image(ijk[3 * j + 1] - ijk[3 * i + 1], n);
-                const double dz = h * min_image(ijk[3 * j + 2] - ijk[3 * i + 2], n);
-                const double r = sqrt(dx * dx + dy * dy + dz * dz);
-                const double ct = dz / r, ph = atan2(dy, dx);
-                double re, im;
-                for (int m = -4; m <= 4; m++) { orc_ylm(4, m, ct, ph, &re, &im); q4[m + 4] += re + I * im; }
-                for (int m = -6; m <= 6; m++) { orc_ylm(6, m, ct, ph, &re, &im); q6[m + 6] += re + I * im; }
-                for (int m = -8; m <= 8; m++) { orc_ylm(8, m, ct, ph, &re, &im); q8[m + 8] += re + I * im; }
-            }
-            double* o = out + 4 * (long)i;
-            if (cnt == 0) { o[0] = o[1] = o[2] = o[3] = NAN; continue; }
-            for (int m = 0; m < 9; m++) q4[m] /= cnt;
-            for (int m = 0; m < 13; m++) q6[m] /= cnt;
-            for (int m = 0; m < 17; m++) q8[m] /= cnt;
-            o[0] = ql_of(q4, 4); o[1] = ql_of(q6, 6); o[2] = ql_of(q8, 8); o[3] = w4_of(q4);
-        }
+        for (int q = 0; q < nq; q++)
+            knn_point(ijk, P, n, h, k, qi ? qi[q] : q, bestd, bestj, nbr + (long)k * q, out + 4 * (long)q);
         free(bestd); free(bestj);
     }
+}
+
+void orc_bond_order_knn(const int* ijk, int P, int n, double h, int k, int* nbr, double* out) {
+    orc_bond_order_knn_points(ijk, P, n, h, k, NULL, P, nbr, out);
 }
 
 /* ------------------------------------------------------------------------ */

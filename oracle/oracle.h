@@ -91,6 +91,10 @@ void orc_bond_order_cutoff(const double* xyz, int P, double box, double rlim, in
    nbr[k*i + j] = index of the j-th nearest neighbour (integer squared distance, ties by index);
    out[4*i + {0..3}] = q4 q6 q8 w4 */
 void orc_bond_order_knn(const int* ijk, int P, int n, double h, int k, int* nbr, double* out);
+/* the same for the points qi[0..nq) only (all P points are candidates): row q of
+   nbr and out belongs to point qi[q]; qi NULL means every point (nq = P) */
+void orc_bond_order_knn_points(const int* ijk, int P, int n, double h, int k, const int* qi, int nq, int* nbr,
+                               double* out);
 /* normalised spherical harmonic Y_lm(cos(theta), phi), Condon-Shortley phase */
 void orc_ylm(int l, int m, double ct, double phi, double* re, double* im);
 /* Wigner 3j symbol (j1 j2 j3; m1 m2 m3) for integer arguments */

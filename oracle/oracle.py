@@ -64,6 +64,8 @@ def lib():
         L.orc_findpeaks.restype = ctypes.c_int64
         L.orc_bond_order_cutoff.argtypes = [_d, ctypes.c_int, ctypes.c_double, ctypes.c_double, _i, _d]
         L.orc_bond_order_knn.argtypes = [_i, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, _i, _d]
+        L.orc_bond_order_knn_points.argtypes = [_i, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, _i,
+                                                ctypes.c_int, _i, _d]
         L.orc_ylm.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, _d, _d]
         L.orc_wigner3j.argtypes = [ctypes.c_int] * 6
         L.orc_wigner3j.restype = ctypes.c_double
@@ -254,6 +256,17 @@ def bond_order_knn(ijk, n, h, k=12):
     out = np.zeros((max(P, 1), 4), dtype=np.float64)
     lib().orc_bond_order_knn(_p(ijk, _i), P, n, h, k, _p(nbr, _i), _p(out))
     return nbr[:P], out[:P]
+
+
+def bond_order_knn_points(ijk, n, h, queries, k=12):
+    """bond_order_knn for the points `queries` only (all points are candidates): rows follow `queries`"""
+    ijk = np.ascontiguousarray(ijk, dtype=np.int32)
+    qi = np.ascontiguousarray(queries, dtype=np.int32)
+    nq = qi.size
+    nbr = np.zeros((max(nq, 1), k), dtype=np.int32)
+    out = np.zeros((max(nq, 1), 4), dtype=np.float64)
+    lib().orc_bond_order_knn_points(_p(ijk, _i), ijk.shape[0], n, h, k, _p(qi, _i), nq, _p(nbr, _i), _p(out))
+    return nbr[:nq], out[:nq]
 
 
 def ylm(l, m, ct, phi):

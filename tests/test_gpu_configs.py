@@ -9,6 +9,7 @@
 """
 import os
 import sys
+import time
 
 import numpy as np
 import pytest
@@ -161,3 +162,32 @@ def test_precision_study_512cubed_500_steps_full_oracle(mcs, orc):
     del want
     pg = mcs.findpeaks(got, float(got.mean()))
     assert np.array_equal(pg, orc.findpeaks(got, float(got.mean())))
+
+
+def test_precision_study_512cubed_peaks_and_bond_order(mcs, orc):
+    # the full-size flow: 500 FP32 steps at 512^3, peaks of the result, 12-NN q4/q6 of every peak on
+    # the device; peak list against the oracle's on the same field, neighbour lists and q values of a
+    # sample of peaks against the oracle's all-pairs search over every peak
+    n = 512
+    psi = orc.init_field((n, n, n), -0.25, 0.05, 11)
+    st = mcs.Stepper((n, n, n), [H] * 3, 0.3, 0.5, dtype=np.float32)
+    st.set_psi(psi)
+    del psi
+    st.step(500)
+    got = st.get_psi()
+    del st
+    ijk = _peaks_and_knn(mcs, orc, got, n)
+    P = ijk.shape[0]
+    print(f"512^3 after 500 steps: {P} peaks", file=sys.__stderr__, flush=True)
+    assert P > 10000
+    t0 = time.perf_counter()
+    nbr, q, h4, h6 = mcs.bond_order_knn(ijk, n, H, 12)
+    print(f"12-NN bond order of {P} peaks: {1e3 * (time.perf_counter() - t0):.1f} ms including transfers",
+          file=sys.__stderr__, flush=True)
+    assert (nbr >= 0).all() and np.isfinite(q).all()
+    sample = np.random.default_rng(0).choice(P, 1500, replace=False)
+    onbr, oq = orc.bond_order_knn_points(ijk, n, H, sample, 12)
+    assert np.array_equal(nbr[sample], onbr)
+    assert np.abs(q[sample] - oq).max() < 1e-12
+    assert np.array_equal(h4, np.bincount(np.minimum((q[:, 0] * 64).astype(int), 63), minlength=64))
+    assert np.array_equal(h6, np.bincount(np.minimum((q[:, 1] * 64).astype(int), 63), minlength=64))

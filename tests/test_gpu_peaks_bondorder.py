@@ -75,6 +75,60 @@ def test_bond_order_knn_bit_exact_neighbours(mcs, orc):
     assert np.array_equal(h4, want4) and np.array_equal(h6, want6)
 
 
+def _ijk(cells, n):
+    return np.column_stack([cells % n, (cells // n) % n, cells // (n * n)]).astype(np.int32)
+
+
+def _knn_matches_oracle(mcs, orc, ijk, n, h, k):
+    nbr, out, h4, h6 = mcs.bond_order_knn(ijk, n, h, k)
+    onbr, oout = orc.bond_order_knn(ijk, n, h, k)
+    assert np.array_equal(nbr, onbr)
+    assert np.array_equal(np.isnan(out), np.isnan(oout))
+    if out.size and not np.isnan(out).all():
+        assert np.nanmax(np.abs(out - oout)) < 1e-12
+    m = ~np.isnan(oout[:, 0])
+    assert h4.sum() == m.sum() and h6.sum() == m.sum()
+    return nbr, out
+
+
+@pytest.mark.parametrize("k", [1, 8, 12, 13, 16, 17, 32])
+def test_bond_order_knn_cell_list_every_capacity(mcs, orc, k):
+    # every register capacity of the cell-list search (8, 12, 16, 32 slots) against the all-pairs oracle
+    rng = np.random.default_rng(100 + k)
+    n = 48
+    _knn_matches_oracle(mcs, orc, _ijk(rng.choice(n ** 3, 3000, replace=False), n), n, 0.7, k)
+
+
+def test_bond_order_knn_cell_list_edge_cases(mcs, orc):
+    rng = np.random.default_rng(3)
+    # one point: no neighbours (-1 and NaN); fewer points than k: every other point, in key order
+    nbr, out = _knn_matches_oracle(mcs, orc, np.array([[3, 4, 5]], np.int32), 16, 1.0, 12)
+    assert (nbr == -1).all() and np.isnan(out).all()
+    nbr, _ = _knn_matches_oracle(mcs, orc, _ijk(rng.choice(16 ** 3, 6, replace=False), 16), 16, 1.0, 12)
+    assert (nbr[:, :5] >= 0).all() and (nbr[:, 5:] == -1).all()
+    # a dense cluster straddling the periodic corner of a large, otherwise empty box
+    n = 256
+    c = rng.integers(-4, 4, size=(400, 3))
+    c = np.unique(c, axis=0)
+    _knn_matches_oracle(mcs, orc, (c % n).astype(np.int32), n, 0.5, 12)
+    # the same points given outside [0, n): coordinates are taken modulo n
+    _knn_matches_oracle(mcs, orc, c.astype(np.int32) + np.array([n, -n, 3 * n], np.int32), n, 0.5, 12)
+    # uneven density: a dense slab and a sparse remainder
+    n = 64
+    dense = rng.choice(n * n * 8, 2500, replace=False)
+    sparse = n * n * 8 + rng.choice(n * n * (n - 8), 300, replace=False)
+    _knn_matches_oracle(mcs, orc, _ijk(np.concatenate([sparse, dense]), n), n, 1.0, 12)
+    # a simple-cubic lattice: 6 nearest at a, 12 at a*sqrt(2), so the 12-NN set is decided by index ties
+    a, m = 5, 8
+    g = np.arange(m) * a
+    X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
+    lat = np.column_stack([X.ravel(), Y.ravel(), Z.ravel()]).astype(np.int32)
+    perm = rng.permutation(len(lat))
+    _knn_matches_oracle(mcs, orc, lat[perm], a * m, 1.0, 12)
+    # a box smaller than a cell in every direction and shells that wrap around it
+    _knn_matches_oracle(mcs, orc, _ijk(rng.choice(5 ** 3, 60, replace=False), 5), 5, 1.0, 12)
+
+
 def test_bond_order_fcc_lattice(mcs):
     pts = []
     for i in range(4):

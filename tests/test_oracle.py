@@ -539,3 +539,16 @@ def test_knn_neighbours_brute_force(orc):
     for i in range(len(ijk)):
         order = sorted((int(d2[i, j]), j) for j in range(len(ijk)) if j != i)[:12]
         assert list(nbr[i]) == [j for _, j in order]
+
+
+def test_knn_points_equals_full_rows(orc):
+    # the query-subset form (used to check full-size GPU runs on a sample) is the full form's rows
+    rng = np.random.default_rng(5)
+    n = 24
+    pts = rng.choice(n ** 3, 300, replace=False)
+    ijk = np.column_stack([pts % n, (pts // n) % n, pts // (n * n)]).astype(np.int32)
+    nbr, out = orc.bond_order_knn(ijk, n, 0.5, 12)
+    q = rng.choice(300, 40, replace=False)
+    snbr, sout = orc.bond_order_knn_points(ijk, n, 0.5, q, 12)
+    assert np.array_equal(snbr, nbr[q])
+    assert np.array_equal(sout, out[q])
