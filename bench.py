@@ -163,6 +163,8 @@ def main():
     ap.add_argument("--plan", default=None,
                     help="explicit FP32 plans Px,Gx,Py,Gy,Pz,Gz instead of the tuned ones (profiling without "
                          "the tuner's trial launches)")
+    ap.add_argument("--fixed", type=int, default=None,
+                    help="fixed-plan kernel mask (1 column passes, 2 x pass; default both)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a 1 GiB device-to-device copy (reference dispatch for HBM counters)")
     a = ap.parse_args()
@@ -172,6 +174,8 @@ def main():
         sys.exit(2)
 
     mcs = load_pkg()
+    if a.fixed is not None:
+        mcs.set_fixed_kernels(a.fixed)
     if a.only_ensemble:
         print(json.dumps({"ensemble_2d": ensemble_bench(mcs, "2d", 40), "ensemble_3d": ensemble_bench(mcs, "3d", 16)}))
         return
@@ -222,7 +226,7 @@ def main():
                                   f"dt=0.5, noise off", "n": n, "precision": "fp32", "parallelism": "1 GPU"},
            "steps_per_s": 1e3 / ms_step, "hbm_roofline_frac_step": round(step_frac, 4),
            "model_bytes_per_step": b_step, "pass_ms": {k: round(v, 5) for k, v in passes.items()},
-           "plans": st.plan_info(), "autotune": mcs.get_autotune(),
+           "plans": st.plan_info(), "autotune": mcs.get_autotune(), "fixed_kernels": mcs.get_fixed_kernels(),
            "roofline": roofline}
 
     if not a.no_fp64:

@@ -204,6 +204,16 @@ int mcs_stepper_pass_times(mcs_stepper* st, int nsteps, double* ms, int cap, int
  * default plans (reproducible bit for bit across processes). */
 int mcs_set_autotune(int on);
 int mcs_get_autotune(int* on);
+/* Fixed-plan kernels of the fused step (no reference counterpart).  For the
+ * passes whose bit is set (both by default), a pass whose plan is one the
+ * library holds compiled with its whole radix sequence as constants (axis
+ * lengths 128, 256, 512; FP32 and FP64) runs that kernel; it computes the same
+ * operations in the same order as the runtime-plan kernel, so results are
+ * identical either way.  A cleared bit runs the runtime-plan kernels. */
+#define MCS_FIXED_COLUMNS 1 /* the y and z column passes */
+#define MCS_FIXED_XPASS 2   /* the x pass with the update */
+int mcs_set_fixed_kernels(int mask);
+int mcs_get_fixed_kernels(int* mask);
 /* Exact checkpoint of the stepper (no reference counterpart; the simulators saved psi
  * with CPUArray::save, array_abstraction.hpp:155-180).  The state is the packed spectrum
  * U = FFT(u) of u(m,y,z) = psi(2m,y,z) + i psi(2m+1,y,z) over (nx/2) x ny x nz, complex,

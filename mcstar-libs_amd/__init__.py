@@ -108,6 +108,8 @@ _SIGS = {
     "mcs_stepper_set_state": ([_vp, ctypes.c_void_p, ctypes.c_double, ctypes.c_int64], ctypes.c_int),
     "mcs_set_autotune": ([ctypes.c_int], ctypes.c_int),
     "mcs_get_autotune": ([_ip], ctypes.c_int),
+    "mcs_set_fixed_kernels": ([ctypes.c_int], ctypes.c_int),
+    "mcs_get_fixed_kernels": ([_ip], ctypes.c_int),
     "mcs_ensemble_run": ([_vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int, _i64p, _i64p, _i64p],
                          ctypes.c_int),
     "mcs_stepper_create_planned": ([ctypes.POINTER(_vp), ctypes.c_void_p, _ip], ctypes.c_int),
@@ -687,6 +689,21 @@ def get_autotune():
     v = ctypes.c_int()
     check(lib().mcs_get_autotune(ctypes.byref(v)), "mcs_get_autotune")
     return bool(v.value)
+
+
+FIXED_COLUMNS, FIXED_XPASS = 1, 2
+
+
+def set_fixed_kernels(mask):
+    """fixed-plan kernels for the passes in mask (FIXED_COLUMNS | FIXED_XPASS, the default) where a plan
+    matches one; the other passes run the runtime-plan kernels"""
+    check(lib().mcs_set_fixed_kernels(int(mask)), "mcs_set_fixed_kernels")
+
+
+def get_fixed_kernels():
+    v = ctypes.c_int()
+    check(lib().mcs_get_fixed_kernels(ctypes.byref(v)), "mcs_get_fixed_kernels")
+    return v.value
 
 
 def sync():

@@ -282,6 +282,68 @@ MCS_DEV void reg_fft(C2<F> (&a)[P], const FftPlanDev& pl, const C2<F>* __restric
     }
 }
 
+// ------------------------------------------------------------ fixed plans
+// The register engine with the whole plan known at compile time: length L and
+// radices Rs... in pass order (sub-lengths and twiddle-table offsets follow as
+// in the runtime plan, so both read the same table and compute the same
+// values).  Index arithmetic folds to constants and the passes unroll into one
+// straight-line body, so no register array is copied between passes.
+struct RunPlan {
+    static constexpr bool fixed = false;
+    static constexpr int L = 0;
+};
+template <int L_, int... Rs> struct FixPlan {
+    static constexpr bool fixed = true;
+    static constexpr int L = L_;
+};
+
+template <typename F, int P, int DIR, bool WAVE, int L, int Ns, int TwOff, int R, int... Rest>
+MCS_DEV void fix_pass(C2<F> (&a)[P], const C2<F>* __restrict__ tw, int t, C2<F>* lds, int lbase, int lps) {
+    constexpr int S = L / P;
+    constexpr int B = P / R;
+    constexpr bool last = sizeof...(Rest) == 0;
+    static_assert(P % R == 0, "every radix of a register plan divides P");
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+        C2<F> v[R];
+#pragma unroll
+        for (int q = 0; q < R; ++q) v[q] = a[b + q * B];
+        const int j = t + b * S;
+        const int k = Ns > 1 ? j % Ns : 0;
+        if constexpr (Ns > 1) apply_tw<F, R, DIR>(v, tw + TwOff, k);
+        dft<F, R, DIR>(v);
+        if constexpr (last) {
+#pragma unroll
+            for (int q = 0; q < R; ++q) a[b + q * B] = v[q];
+        } else {
+            const int base = (j - k) * R + k;
+#pragma unroll
+            for (int q = 0; q < R; ++q) lds[lbase + (base + q * Ns) * lps] = v[q];
+        }
+    }
+    if constexpr (!last) {
+        lds_sync<WAVE>();
+#pragma unroll
+        for (int m = 0; m < P; ++m) a[m] = lds[lbase + (t + m * S) * lps];
+        lds_sync<WAVE>();
+        fix_pass<F, P, DIR, WAVE, L, Ns * R, TwOff + (Ns > 1 ? Ns * (R - 1) : 0), Rest...>(a, tw, t, lds, lbase, lps);
+    }
+}
+
+template <typename F, int P, int DIR, bool WAVE, int L, int... Rs>
+MCS_DEV void fix_fft(C2<F> (&a)[P], const C2<F>* __restrict__ tw, int t, C2<F>* lds, int lbase, int lps,
+                     FixPlan<L, Rs...>) {
+    fix_pass<F, P, DIR, WAVE, L, 1, 0, Rs...>(a, tw, t, lds, lbase, lps);
+}
+
+// the line transform of a pass body: runtime plan pl, or the fixed plan FP (pl unused)
+template <typename F, int P, int DIR, bool WAVE, typename FP>
+MCS_DEV void line_fft(C2<F> (&a)[P], const FftPlanDev& pl, const C2<F>* __restrict__ tw, int t, C2<F>* lds,
+                      int lbase, int lps) {
+    if constexpr (FP::fixed) fix_fft<F, P, DIR, WAVE>(a, tw, t, lds, lbase, lps, FP{});
+    else reg_fft<F, P, DIR, WAVE>(a, pl, tw, t, lds, lbase, lps);
+}
+
 // ------------------------------------------------------------ generic engine
 // Lines live in LDS as buf[c*Lp + pos], c in [0, G).
 template <typename F, int R, int DIR>

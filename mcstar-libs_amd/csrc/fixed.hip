@@ -1,0 +1,261 @@
+// Fixed-plan kernels of the fused step for the shapes that carry the headline
+// workloads (axis lengths 128, 256 and 512 in FP32 and FP64).
+//
+// They compute exactly what the runtime-plan kernels compute (same butterflies,
+// same twiddle table, same operation order) with the whole line plan known at
+// compile time.  Against a plan read from kernel arguments this removes the
+// register-array copies between passes and the per-butterfly index arithmetic:
+// the 512-point z pass (32 points per thread, two transforms) drops from about
+// 3600 to about 2200 vector instructions per thread.  Column tiles address
+// HBM through a uniform tile base plus a 32-bit per-thread byte offset, which
+// the launcher checks.
+//
+// A plan the table below does not hold runs the runtime-plan kernel; so does
+// every pass whose bit mcs_set_fixed_kernels has cleared.
+#include "fixed.hpp"
+
+#include <atomic>
+
+static std::atomic<int> g_fixed{MCS_FIXED_COLUMNS | MCS_FIXED_XPASS};
+
+extern "C" int mcs_set_fixed_kernels(int mask) {
+    MCS_CHECK(mask >= 0 && mask <= (MCS_FIXED_COLUMNS | MCS_FIXED_XPASS), MCS_ERR_INVALID, "unknown fixed-kernel bits");
+    g_fixed.store(mask);
+    return MCS_OK;
+}
+
+extern "C" int mcs_get_fixed_kernels(int* mask) {
+    MCS_CHECK(mask, MCS_ERR_INVALID, "null argument");
+    *mask = g_fixed.load();
+    return MCS_OK;
+}
+
+// ------------------------------------------------------------------ kernels
+// a tile of G adjacent columns of W (in place), S = L/P threads per column; blockDim.x == G * S
+template <typename F, int P, typename FP>
+MCS_DEV void fx_col_load(C2<F> (&a)[P], const char* wb, long mst, unsigned off, bool valid) {
+#pragma unroll
+    for (int m = 0; m < P; ++m)
+        a[m] = valid ? *reinterpret_cast<const C2<F>*>(wb + m * mst + off) : mk<F>(0, 0);
+}
+
+template <typename F, int P, typename FP>
+MCS_DEV void fx_col_store(const C2<F> (&a)[P], char* wb, long mst, unsigned off, bool valid) {
+    if (valid) {
+#pragma unroll
+        for (int m = 0; m < P; ++m) *reinterpret_cast<C2<F>*>(wb + m * mst + off) = a[m];
+    }
+}
+
+template <typename F, int P, int DIR, typename FP>
+__global__ void __launch_bounds__(MCS_REG_MAX_THREADS(P, F)) k_col_fx(C2<F>* W, ColGeom g, int G,
+                                                                     const C2<F>* __restrict__ tw) {
+    extern __shared__ unsigned char smem_raw[];
+    C2<F>* lds = reinterpret_cast<C2<F>*>(smem_raw);
+    constexpr int S = FP::L / P;
+    const int tilesx = (g.nxc + G - 1) / G;
+    const int tile = blockIdx.x % tilesx, outer = blockIdx.x / tilesx;
+    const int c = threadIdx.x % G, t = threadIdx.x / G;
+    const bool valid = tile * G + c < g.nxc;
+    char* wb = reinterpret_cast<char*>(W + (long)outer * g.os + (long)tile * G);
+    const long mst = (long)S * g.ls * (long)sizeof(C2<F>);
+    const unsigned off = (unsigned)((c + (long)t * g.ls) * (long)sizeof(C2<F>));
+    C2<F> a[P];
+    fx_col_load<F, P, FP>(a, wb, mst, off, valid);
+    fix_fft<F, P, DIR, false>(a, tw, t, lds, c, G, FP{});
+    fx_col_store<F, P, FP>(a, wb, mst, off, valid);
+}
+
+// the z pass of the step: inverse transform, N = psi^3 (or psi^3 - psi), forward transform
+template <typename F, int P, int SPLIT, typename FP>
+__global__ void __launch_bounds__(MCS_REG_MAX_THREADS(P, F)) k_col_nl_fx(C2<F>* W, ColGeom g, int G,
+                                                                        const C2<F>* __restrict__ tw, F scale,
+                                                                        int64_t* d_step) {
+    extern __shared__ unsigned char smem_raw[];
+    C2<F>* lds = reinterpret_cast<C2<F>*>(smem_raw);
+    constexpr int S = FP::L / P;
+    const int tilesx = (g.nxc + G - 1) / G;
+    const int tile = blockIdx.x % tilesx, outer = blockIdx.x / tilesx;
+    const int c = threadIdx.x % G, t = threadIdx.x / G;
+    const bool valid = tile * G + c < g.nxc;
+    char* wb = reinterpret_cast<char*>(W + (long)outer * g.os + (long)tile * G);
+    const long mst = (long)S * g.ls * (long)sizeof(C2<F>);
+    const unsigned off = (unsigned)((c + (long)t * g.ls) * (long)sizeof(C2<F>));
+    C2<F> a[P];
+    fx_col_load<F, P, FP>(a, wb, mst, off, valid);
+    fix_fft<F, P, +1, false>(a, tw, t, lds, c, G, FP{});
+#pragma unroll
+    for (int m = 0; m < P; ++m) {
+        const F u = a[m].x * scale, v = a[m].y * scale;
+        if (SPLIT == MCS_SPLIT_REFERENCE) a[m] = mk<F>(u * u * u - u, v * v * v - v);
+        else a[m] = mk<F>(u * u * u, v * v * v);
+    }
+    fix_fft<F, P, -1, false>(a, tw, t, lds, c, G, FP{});
+    fx_col_store<F, P, FP>(a, wb, mst, off, valid);
+    if (d_step && blockIdx.x == 0 && threadIdx.x == 0) *d_step += 1;
+}
+
+template <typename F, int P, int SPLIT, bool WAVE, typename FP>
+__global__ void __launch_bounds__(256) k_xupd_fx(C2<F>* __restrict__ W, C2<F>* __restrict__ U,
+                                                 const int2* __restrict__ pairs, int npairs, int G, XArgs<F> ar,
+                                                 const C2<F>* __restrict__ tw, const C2<F>* __restrict__ wph,
+                                                 const int64_t* __restrict__ d_step) {
+    extern __shared__ unsigned char smem_raw[];
+    const uint32_t draw = ar.noise ? (uint32_t)*d_step : 0u;
+    const FftPlanDev unused{};
+    xupd_body<F, P, SPLIT, WAVE, FP>(W, U, pairs, npairs, G, ar, unused, tw, wph, draw,
+                                     reinterpret_cast<C2<F>*>(smem_raw), blockIdx.x, threadIdx.x);
+}
+
+// ------------------------------------------------------------------ table
+// Each entry: precision, points per thread, length, radices in pass order (the
+// runtime plan builder's order for that length and P), and the kernels.
+struct ColEntry {
+    int prec, P, L, np;
+    int rad[4];
+    const void* fwd;
+    const void* inv;
+    const void* nl_ns;   // psi^3
+    const void* nl_ref;  // psi^3 - psi
+};
+struct XEntry {
+    int prec, P, L, np;
+    int rad[4];
+    const void* ns;
+    const void* ref;
+};
+
+template <typename F, int P, int L, int... Rs> static ColEntry col_entry() {
+    using FP = FixPlan<L, Rs...>;
+    ColEntry e{};
+    e.prec = sizeof(F) == 8 ? 64 : 32;
+    e.P = P;
+    e.L = L;
+    const int r[] = {Rs...};
+    e.np = (int)sizeof...(Rs);
+    for (int i = 0; i < e.np; ++i) e.rad[i] = r[i];
+    e.fwd = (const void*)k_col_fx<F, P, -1, FP>;
+    e.inv = (const void*)k_col_fx<F, P, +1, FP>;
+    e.nl_ns = (const void*)k_col_nl_fx<F, P, MCS_SPLIT_NORTHSTAR, FP>;
+    e.nl_ref = (const void*)k_col_nl_fx<F, P, MCS_SPLIT_REFERENCE, FP>;
+    return e;
+}
+
+template <typename F, int P, int L, int... Rs> static XEntry x_entry() {
+    using FP = FixPlan<L, Rs...>;
+    constexpr bool wave = 64 % (L / P) == 0;
+    XEntry e{};
+    e.prec = sizeof(F) == 8 ? 64 : 32;
+    e.P = P;
+    e.L = L;
+    const int r[] = {Rs...};
+    e.np = (int)sizeof...(Rs);
+    for (int i = 0; i < e.np; ++i) e.rad[i] = r[i];
+    e.ns = (const void*)k_xupd_fx<F, P, MCS_SPLIT_NORTHSTAR, wave, FP>;
+    e.ref = (const void*)k_xupd_fx<F, P, MCS_SPLIT_REFERENCE, wave, FP>;
+    return e;
+}
+
+static const ColEntry* col_table(int* n) {
+    static const ColEntry t[] = {
+        col_entry<float, 8, 512, 8, 8, 8>(),   col_entry<float, 16, 512, 2, 16, 16>(),
+        col_entry<float, 32, 512, 16, 32>(),   col_entry<float, 8, 256, 4, 8, 8>(),
+        col_entry<float, 16, 256, 16, 16>(),   col_entry<double, 8, 512, 8, 8, 8>(),
+        col_entry<double, 16, 512, 2, 16, 16>(), col_entry<double, 8, 256, 4, 8, 8>(),
+        col_entry<double, 16, 256, 16, 16>(),
+    };
+    *n = (int)(sizeof(t) / sizeof(t[0]));
+    return t;
+}
+
+static const XEntry* x_table(int* n) {
+    static const XEntry t[] = {
+        x_entry<float, 8, 256, 4, 8, 8>(),  x_entry<float, 16, 256, 16, 16>(), x_entry<float, 8, 128, 2, 8, 8>(),
+        x_entry<float, 16, 128, 8, 16>(),   x_entry<double, 8, 256, 4, 8, 8>(), x_entry<double, 8, 128, 2, 8, 8>(),
+    };
+    *n = (int)(sizeof(t) / sizeof(t[0]));
+    return t;
+}
+
+template <typename E> static bool plan_matches(const E& e, const LinePlan& lp) {
+    if (lp.engine != ENGINE_REG || lp.prec != e.prec || lp.P != e.P || lp.L != e.L || lp.dev.np != e.np) return false;
+    for (int i = 0; i < e.np; ++i)
+        if (lp.dev.radix[i] != e.rad[i]) return false;
+    return true;
+}
+
+static int set_lds_fx(const void* fn, size_t bytes) {
+    if (bytes > 65536) MCS_HIP_TRY(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+    return MCS_OK;
+}
+
+// ------------------------------------------------------------------ launchers
+int fixed_step_col(const LinePlan& lp, void* W, const ColGeom& g, int kind, double scale, int split,
+                   int64_t* d_step, hipStream_t s, bool* done) {
+    *done = false;
+    if (!(g_fixed.load() & MCS_FIXED_COLUMNS)) return MCS_OK;
+    int n = 0;
+    const ColEntry* t = col_table(&n);
+    const ColEntry* e = nullptr;
+    for (int i = 0; i < n && !e; ++i)
+        if (plan_matches(t[i], lp)) e = &t[i];
+    if (!e) return MCS_OK;
+    const int S = lp.L / lp.P;
+    const int G = col_lines_per_wg(lp);
+    const long esz = lp.prec == 64 ? 16 : 8;
+    // per-thread byte offsets (column + t * ls) stay below 2^32
+    if (((long)S * g.ls + G) * esz >= (1L << 32)) return MCS_OK;
+    const void* fn = kind < 0 ? e->fwd : kind > 0 ? e->inv : (split == MCS_SPLIT_REFERENCE ? e->nl_ref : e->nl_ns);
+    const size_t lds = (size_t)G * lp.L * esz;
+    int st = set_lds_fx(fn, lds);
+    if (st) return st;
+    const long tiles = (long)((g.nxc + G - 1) / G) * g.nouter;
+    void* tw = lp.tw;
+    if (kind != 0) {
+        void* args[] = {&W, (void*)&g, (void*)&G, &tw};
+        MCS_HIP_TRY(hipLaunchKernel(fn, dim3((unsigned)tiles), dim3(G * S), args, lds, s));
+    } else if (lp.prec == 64) {
+        double sc = scale;
+        void* args[] = {&W, (void*)&g, (void*)&G, &tw, &sc, &d_step};
+        MCS_HIP_TRY(hipLaunchKernel(fn, dim3((unsigned)tiles), dim3(G * S), args, lds, s));
+    } else {
+        float sc = (float)scale;
+        void* args[] = {&W, (void*)&g, (void*)&G, &tw, &sc, &d_step};
+        MCS_HIP_TRY(hipLaunchKernel(fn, dim3((unsigned)tiles), dim3(G * S), args, lds, s));
+    }
+    *done = true;
+    return MCS_OK;
+}
+
+template <typename F>
+static int fixed_xupd_t(const XEntry& e, const StepGeom& sg, const StepPhys& ph, void* W, void* U, hipStream_t s) {
+    const LinePlan& lp = sg.px;
+    XArgs<F> ar = make_xargs<F>(sg, ph);
+    const int S = lp.L / lp.P;
+    const int G = xupd_pairs_per_wg(lp);
+    const size_t lds = (size_t)G * 2 * lp.L * sizeof(C2<F>);
+    const void* fn = ph.split == MCS_SPLIT_REFERENCE ? e.ref : e.ns;
+    int st = set_lds_fx(fn, lds);
+    if (st) return st;
+    const int2* pairs = sg.pairs;
+    int npairs = sg.npairs, g = G;
+    const void* tw = lp.tw;
+    const void* wph = sg.wph;
+    const int64_t* ds = sg.d_step;
+    void* args[] = {&W, &U, (void*)&pairs, &npairs, &g, &ar, (void*)&tw, (void*)&wph, (void*)&ds};
+    MCS_HIP_TRY(hipLaunchKernel(fn, dim3((unsigned)((sg.npairs + G - 1) / G)), dim3(G * S), args, lds, s));
+    return MCS_OK;
+}
+
+int fixed_xupd(const StepGeom& sg, const StepPhys& ph, void* W, void* U, hipStream_t s, bool* done) {
+    *done = false;
+    if (!(g_fixed.load() & MCS_FIXED_XPASS)) return MCS_OK;
+    int n = 0;
+    const XEntry* t = x_table(&n);
+    for (int i = 0; i < n; ++i)
+        if (plan_matches(t[i], sg.px)) {
+            *done = true;
+            return sg.prec == 64 ? fixed_xupd_t<double>(t[i], sg, ph, W, U, s) : fixed_xupd_t<float>(t[i], sg, ph, W, U, s);
+        }
+    return MCS_OK;
+}

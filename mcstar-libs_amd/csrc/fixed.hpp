@@ -1,0 +1,12 @@
+// Fixed-plan kernels of the fused step: launch interface.
+#pragma once
+
+#include "stepper.hpp"
+
+// A column pass of the step (kind -1 forward, +1 inverse, 0 nonlinear) through
+// a fixed-plan kernel when one matches lp; *done tells whether it launched
+// (when false the caller runs the runtime-plan kernel).
+int fixed_step_col(const LinePlan& lp, void* W, const ColGeom& g, int kind, double scale, int split,
+                   int64_t* d_step, hipStream_t s, bool* done);
+// the x pass with the update, likewise
+int fixed_xupd(const StepGeom& sg, const StepPhys& ph, void* W, void* U, hipStream_t s, bool* done);

@@ -157,11 +157,12 @@ MCS_DEV void xupd_pair(C2<F> nA, C2<F> nBr, C2<F> pA, C2<F> pBr, int p, int iy, 
 // Each thread updates the mode pairs {(c, p), (m, -p)} of its own positions p once; the
 // mirror row's results move to their owners through LDS.  A self-mirrored row (c == m)
 // is processed as row c alone.
-template <typename F, int P, int SPLIT, bool WAVE = false>
+// FP: RunPlan (the line plan pl) or a FixPlan of length L = nx/2.
+template <typename F, int P, int SPLIT, bool WAVE = false, typename FP = RunPlan>
 MCS_DEV void xupd_body(C2<F>* __restrict__ W, C2<F>* __restrict__ U, const int2* __restrict__ pairs, int npairs,
                        int G, const XArgs<F>& ar, const FftPlanDev& pl, const C2<F>* __restrict__ tw,
                        const C2<F>* __restrict__ wph, uint32_t draw, C2<F>* lds, int blk, int tid) {
-    const int L = pl.L, S = L / P;
+    const int L = FP::fixed ? FP::L : pl.L, S = L / P;
     const int t = tid % S, g = tid / S;
     const int q = blk * G + g;
     const bool valid = q < npairs && g < G;
@@ -180,8 +181,8 @@ MCS_DEV void xupd_body(C2<F>* __restrict__ W, C2<F>* __restrict__ U, const int2*
         a[m] = valid ? W[rc * L + p] : mk<F>(0, 0);
         b[m] = valid ? W[rm * L + p] : mk<F>(0, 0);
     }
-    reg_fft<F, P, -1, WAVE>(a, pl, tw, t, lds, base, 1);
-    reg_fft<F, P, -1, WAVE>(b, pl, tw, t, lds, base + L, 1);
+    line_fft<F, P, -1, WAVE, FP>(a, pl, tw, t, lds, base, 1);
+    line_fft<F, P, -1, WAVE, FP>(b, pl, tw, t, lds, base + L, 1);
 #pragma unroll
     for (int m = 0; m < P; ++m) {
         const int p = t + m * S;
@@ -224,8 +225,8 @@ MCS_DEV void xupd_body(C2<F>* __restrict__ W, C2<F>* __restrict__ U, const int2*
 #pragma unroll
     for (int m = 0; m < P; ++m) b[m] = self ? a[m] : Nm[t + m * S];
     lds_sync<WAVE>();
-    reg_fft<F, P, +1, WAVE>(a, pl, tw, t, lds, base, 1);
-    reg_fft<F, P, +1, WAVE>(b, pl, tw, t, lds, base + L, 1);
+    line_fft<F, P, +1, WAVE, FP>(a, pl, tw, t, lds, base, 1);
+    line_fft<F, P, +1, WAVE, FP>(b, pl, tw, t, lds, base + L, 1);
     if (valid) {
 #pragma unroll
         for (int m = 0; m < P; ++m) {

@@ -18,6 +18,7 @@
 //   S2  y cols:  FFT_y(W)
 // 10 sweeps of N/2 complex values per step (S3 reads W and U, writes U and W).
 // 2D: S3 on x rows, S1 on y columns (6 sweeps).
+#include "fixed.hpp"
 #include "pass_bodies.hpp"
 #include "stepper.hpp"
 
@@ -149,11 +150,17 @@ static int launch_col_nl_d(const LinePlan& lp, void* W, const ColGeom& g, double
 int launch_step_col(const LinePlan& lp, void* W, const ColGeom& g, int kind, double scale, int split,
                     int64_t* d_step, hipStream_t s) {
     if (kind == 0) return launch_col_nl(lp, W, g, scale, split, d_step, s);
+    bool done = false;
+    const int st = fixed_step_col(lp, W, g, kind, scale, split, d_step, s, &done);
+    if (st || done) return st;
     return launch_col(lp, W, W, g, kind, s);
 }
 
 int launch_col_nl(const LinePlan& lp, void* W, const ColGeom& g, double scale, int split, int64_t* d_step,
                   hipStream_t s) {
+    bool done = false;
+    const int st = fixed_step_col(lp, W, g, 0, scale, split, d_step, s, &done);
+    if (st || done) return st;
     if (lp.prec == 64)
         return split == MCS_SPLIT_REFERENCE ? launch_col_nl_d<double, MCS_SPLIT_REFERENCE>(lp, W, g, scale, d_step, s)
                                             : launch_col_nl_d<double, MCS_SPLIT_NORTHSTAR>(lp, W, g, scale, d_step, s);
@@ -191,6 +198,9 @@ static int launch_xupd_d(const StepGeom& sg, void* W, void* U, const XArgs<F>& a
 }
 
 int launch_xupd(const StepGeom& sg, const StepPhys& ph, void* W, void* U, hipStream_t s) {
+    bool done = false;
+    const int st = fixed_xupd(sg, ph, W, U, s, &done);
+    if (st || done) return st;
     if (sg.prec == 64) {
         const XArgs<double> ar = make_xargs<double>(sg, ph);
         return ph.split == MCS_SPLIT_REFERENCE ? launch_xupd_d<double, MCS_SPLIT_REFERENCE>(sg, W, U, ar, s)

@@ -52,6 +52,9 @@ template <typename F> inline XArgs<F> make_xargs(const StepGeom& sg, const StepP
 int launch_xupd(const StepGeom& sg, const StepPhys& ph, void* W, void* U, hipStream_t s);
 int launch_col_nl(const LinePlan& lp, void* W, const ColGeom& g, double scale, int split, int64_t* d_step,
                   hipStream_t s);
+// a column pass of the step on W in place: kind -1 forward, +1 inverse, 0 nonlinear
+int launch_step_col(const LinePlan& lp, void* W, const ColGeom& g, int kind, double scale, int split,
+                    int64_t* d_step, hipStream_t s);
 ColGeom step_geom_y(const StepGeom& sg);
 ColGeom step_geom_z(const StepGeom& sg);
 

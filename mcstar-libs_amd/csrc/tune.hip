@@ -155,7 +155,7 @@ int step_geom_tune(StepGeom* sg, const StepPhys& ph_in, void* W, void* U, hipStr
     if (!st && sg->is3d) {
         const ColGeom gy = step_geom_y(*sg), gz = step_geom_z(*sg);
         st = tune_pass(sg->ny, prec, maxC, false, s, [&](const LinePlan& lp) {
-            return launch_col(lp, W, W, gy, -1, s);
+            return launch_step_col(lp, W, gy, -1, scale, ph.split, nullptr, s);
         }, &c.y);
         if (!st) st = tune_pass(sg->nz, prec, maxC, false, s, [&](const LinePlan& lp) {
             return launch_col_nl(lp, W, gz, scale, ph.split, nullptr, s);
