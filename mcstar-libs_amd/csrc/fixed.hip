@@ -54,7 +54,10 @@ __global__ void __launch_bounds__(MCS_REG_MAX_THREADS(P, F)) k_col_fx(C2<F>* W, 
     C2<F>* lds = reinterpret_cast<C2<F>*>(smem_raw);
     constexpr int S = FP::L / P;
     const int tilesx = (g.nxc + G - 1) / G;
-    const int tile = blockIdx.x % tilesx, outer = blockIdx.x / tilesx;
+    // each XCD takes a contiguous run of tiles (measured: y passes of 512^3 FP32 207 -> 191 us;
+    // the z pass runs faster in plain order)
+    const int blk = (int)xcd_tile(blockIdx.x, gridDim.x, true);
+    const int tile = blk % tilesx, outer = blk / tilesx;
     const int c = threadIdx.x % G, t = threadIdx.x / G;
     const bool valid = tile * G + c < g.nxc;
     char* wb = reinterpret_cast<char*>(W + (long)outer * g.os + (long)tile * G);
