@@ -9,8 +9,9 @@ HDR := $(wildcard $(PKG)/csrc/*.hpp) include/mcstar_hip.h
 OBJDIR := $(PKG)/_build
 OBJ := $(patsubst $(PKG)/csrc/%.hip,$(OBJDIR)/%.o,$(SRC))
 LIB := $(PKG)/lib/libmcstar_hip.so
-# packed-fp32 instructions are disabled for all device code
-HIPFLAGS := -O3 -std=c++17 -fPIC --offload-arch=$(ARCH) -Xclang -target-feature -Xclang -packed-fp32-ops \
+# packed-fp32 instructions are disabled for all device code; multiply-adds are fused only where
+# the source writes fma (mfma), so every kernel built from the same expressions rounds alike
+HIPFLAGS := -O3 -std=c++17 -fPIC --offload-arch=$(ARCH) -Xclang -target-feature -Xclang -packed-fp32-ops -ffp-contract=off \
             -Wall -Wno-unused-function -Wno-unused-result -I include
 
 all: lib oracle driver sim

@@ -105,6 +105,31 @@ def ensemble_bench(mcs, kind, steps):
     return out
 
 
+def noise_config_bench(mcs, steps):
+    """3D 256^3 with conserved k-space noise (eps=0.3, dt=0.25, psi0=-0.2+U[-0.1,0.1] seed 7, T=1e-3):
+    ms per step and algorithmic GB/s of every pass, FP32 and FP64"""
+    n, cfg = 256, dict(eps=0.3, psi0=-0.2, amp=0.1, seed=7, dx=np.pi / 4, dt=0.25)
+    out = {"workload": "3D PFC 256^3, eps=0.3, dt=0.25, psi0=-0.2+U[-0.1,0.1] seed 7, conserved noise T=1e-3",
+           "steps": steps, "note": "the state (2 x 67 MB in FP32) fits the 256 MB last-level cache: effective GB/s"}
+    for dtype, name in ((np.float32, "fp32"), (np.float64, "fp64")):
+        shp = (n, n, n)
+        buf = mcs.DeviceBuffer(n ** 3 * np.dtype(dtype).itemsize)
+        prec = 64 if dtype == np.float64 else 32
+        mcs.check(mcs.lib().mcs_init_field(buf.ptr, mcs._ext(shp), cfg["psi0"], cfg["amp"], cfg["seed"], prec))
+        st = mcs.Stepper(shp, [cfg["dx"]] * 3, cfg["eps"], cfg["dt"], temp=1e-3, seed=7, dtype=dtype)
+        st.set_psi_device(buf)
+        st.step(10)
+        ms = timed_steps(mcs, st, steps) / steps
+        passes = st.pass_times(10)
+        sweep = (n ** 3 // 2) * 2 * np.dtype(dtype).itemsize
+        nbytes = {"x_update": 4 * sweep, "y_inverse": 2 * sweep, "z_nonlinear": 2 * sweep, "y_forward": 2 * sweep}
+        out[name] = {"ms_per_step": ms, "cell_updates_per_s": n ** 3 / (ms / 1e3), "plans": st.plan_info(),
+                     "pass_GB_per_s": {k: round(nbytes[k] / (v / 1e3) / 1e9, 1) for k, v in passes.items()},
+                     "pass_ms": {k: round(v, 5) for k, v in passes.items()}}
+        del st, buf
+    return out
+
+
 def analysis_bench(mcs, field, h):
     """local maxima above the mean of an n^3 field, then 12-NN q4/q6/q8/w4 and 64-bin histograms of every peak"""
     n = field.shape[0]
@@ -253,6 +278,8 @@ def main():
         log("[bench] peaks and 12-NN bond order of the 500-step field")
         out["analysis"] = analysis_bench(mcs, f32, cfg["dx"])
         del f32
+    log("[bench] 256^3 with noise")
+    out["noise_256"] = noise_config_bench(mcs, 100)
     log("[bench] boundary FFT")
     out["boundary_fft"] = fft_bench(mcs, n, 5)
     if not a.no_ensemble:

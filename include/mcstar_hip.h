@@ -198,10 +198,13 @@ int mcs_stepper_pass_times(mcs_stepper* st, int nsteps, double* ms, int cap, int
 /* Autotuning of the fused step (no reference counterpart).  When on (the
  * default), mcs_stepper_create and mcs_ensemble_create time every candidate
  * register plan (points per thread, hence radix sequence) and tile shape
- * (lines or row pairs per workgroup) of each pass on the new buffers and keep
- * the fastest; choices are cached per (extents, precision) for the process, so
- * steppers and ensemble groups of one shape run the same plans.  Off: fixed
- * default plans (reproducible bit for bit across processes). */
+ * (lines or row pairs per workgroup) of each pass on the new buffers, and for
+ * steppers the fixed-plan kernel of each candidate that has one against the
+ * runtime-plan kernel, and keep the fastest; choices are cached per
+ * (extents, precision) for the process, separately for steppers and for
+ * ensemble groups (whose work-table kernels run the runtime plans).  Off:
+ * fixed default plans (reproducible bit for bit across processes, and a
+ * stepper and an ensemble replica of one shape agree bit for bit). */
 int mcs_set_autotune(int on);
 int mcs_get_autotune(int* on);
 /* Fixed-plan kernels of the fused step (no reference counterpart).  For the

@@ -12,6 +12,7 @@
 // looping baseline); MCS_ENS_GRAPH replays a captured graph of several steps
 // in which every size group is an independent branch, so that the groups'
 // kernels overlap on the GPU.
+#include "ens_kernels.hpp"
 #include "knn.hpp"
 #include "peaks.hpp"
 #include "stepper.hpp"
@@ -19,64 +20,6 @@
 #include <algorithm>
 #include <map>
 #include <vector>
-
-template <typename F> struct EnsRep {
-    C2<F>* U;
-    C2<F>* W;
-    XArgs<F> xa;
-};
-
-// ------------------------------------------------------------------ kernels
-template <typename F, int P, int SPLIT, bool WAVE>
-__global__ void __launch_bounds__(256) k_ens_xupd(const EnsRep<F>* __restrict__ reps, int nr, int units,
-                                                  const int2* __restrict__ pairs, int npairs, int G, FftPlanDev pl,
-                                                  const C2<F>* __restrict__ tw, const C2<F>* __restrict__ wph,
-                                                  const int64_t* __restrict__ d_step, int draw_off) {
-    extern __shared__ unsigned char smem_raw[];
-    C2<F>* lds = reinterpret_cast<C2<F>*>(smem_raw);
-    const uint32_t draw = (uint32_t)(*d_step + draw_off);
-    const long total = (long)nr * units;
-    for (long it = blockIdx.x; it < total; it += gridDim.x) {
-        const int r = (int)(it / units), u = (int)(it % units);
-        const EnsRep<F>& R = reps[r];
-        xupd_body<F, P, SPLIT, WAVE>(R.W, R.U, pairs, npairs, G, R.xa, pl, tw, wph, draw, lds, u, opaque_tid());
-        __syncthreads();
-    }
-}
-
-template <typename F, int P, int DIR>
-__global__ void __launch_bounds__(MCS_REG_MAX_THREADS(P, F)) k_ens_col(const EnsRep<F>* __restrict__ reps, int nr, ColGeom g, int G,
-                                                  FftPlanDev pl, const C2<F>* __restrict__ tw) {
-    extern __shared__ unsigned char smem_raw[];
-    C2<F>* lds = reinterpret_cast<C2<F>*>(smem_raw);
-    const int tilesx = (g.nxc + G - 1) / G;
-    const long units = (long)tilesx * g.nouter;
-    const long total = (long)nr * units;
-    for (long it = blockIdx.x; it < total; it += gridDim.x) {
-        const int r = (int)(it / units);
-        const long u = it % units;
-        C2<F>* W = reps[r].W;
-        col_body<F, P, DIR>(W, W, g, G, pl, tw, lds, (int)(u % tilesx), (int)(u / tilesx), opaque_tid());
-        __syncthreads();
-    }
-}
-
-template <typename F, int P, int SPLIT>
-__global__ void __launch_bounds__(MCS_REG_MAX_THREADS(P, F)) k_ens_col_nl(const EnsRep<F>* __restrict__ reps, int nr, ColGeom g, int G,
-                                                     FftPlanDev pl, const C2<F>* __restrict__ tw, F scale) {
-    extern __shared__ unsigned char smem_raw[];
-    C2<F>* lds = reinterpret_cast<C2<F>*>(smem_raw);
-    const int tilesx = (g.nxc + G - 1) / G;
-    const long units = (long)tilesx * g.nouter;
-    const long total = (long)nr * units;
-    for (long it = blockIdx.x; it < total; it += gridDim.x) {
-        const int r = (int)(it / units);
-        const long u = it % units;
-        col_nl_body<F, P, SPLIT>(reps[r].W, g, G, pl, tw, scale, lds, (int)(u % tilesx), (int)(u / tilesx),
-                                 opaque_tid());
-        __syncthreads();
-    }
-}
 
 __global__ void k_ens_advance(int64_t* d_step, int n) {
     if (blockIdx.x == 0 && threadIdx.x == 0) *d_step += n;
@@ -476,7 +419,7 @@ extern "C" int mcs_ensemble_create(mcs_ensemble** out, const mcs_replica* reps, 
     for (size_t gi = 0; gi < e->groups.size() && !rc; ++gi) {
         EnsGroup* g = e->groups[gi];
         const int r0 = g->members[0];
-        rc = step_geom_tune(&g->sg, phys_of(e->reps[r0], split, 0.0), e->W[r0], e->U[r0], mcs_stream());
+        rc = step_geom_tune(&g->sg, phys_of(e->reps[r0], split, 0.0), e->W[r0], e->U[r0], mcs_stream(), true);
     }
     if (!rc && hipMalloc((void**)&e->d_step, sizeof(int64_t)) != hipSuccess) rc = MCS_ERR_NOMEM;
     if (!rc && hipMemset(e->d_step, 0, sizeof(int64_t)) != hipSuccess) rc = MCS_ERR_HIP;

@@ -1,5 +1,9 @@
-// Fixed-plan kernels of the fused step for the shapes that carry the headline
-// workloads (axis lengths 128, 256 and 512 in FP32 and FP64).
+// Fixed-plan kernels of the single-grid fused step for the shapes that carry
+// the headline workloads: axis lengths 128, 256 and 512 in FP32 and FP64, and
+// the lengths of the 3D ensemble's sizes (32 to 192, with 12- and 20-point
+// plans for 3*2^a and 5*2^a) in FP32.  The ensemble's persistent work-table
+// kernels keep the runtime plans: with the fixed bodies (up to 428 VGPRs in
+// the 20-point x pass) the 3D ensemble step was slower (0.69 against 0.61 ms).
 //
 // They compute exactly what the runtime-plan kernels compute (same butterflies,
 // same twiddle table, same operation order) with the whole line plan known at
@@ -11,7 +15,8 @@
 // the launcher checks.
 //
 // A plan the table below does not hold runs the runtime-plan kernel; so does
-// every pass whose bit mcs_set_fixed_kernels has cleared.
+// every pass whose bit mcs_set_fixed_kernels has cleared, and every plan for
+// which the tuner timed the runtime-plan kernel faster (LinePlan::nofx).
 #include "fixed.hpp"
 
 #include <atomic>
@@ -90,7 +95,7 @@ __global__ void __launch_bounds__(MCS_REG_MAX_THREADS(P, F)) k_col_nl_fx(C2<F>* 
 #pragma unroll
     for (int m = 0; m < P; ++m) {
         const F u = a[m].x * scale, v = a[m].y * scale;
-        if (SPLIT == MCS_SPLIT_REFERENCE) a[m] = mk<F>(u * u * u - u, v * v * v - v);
+        if (SPLIT == MCS_SPLIT_REFERENCE) a[m] = mk<F>(mfma(u * u, u, -u), mfma(v * v, v, -v));
         else a[m] = mk<F>(u * u * u, v * v * v);
     }
     fix_fft<F, P, -1, false>(a, tw, t, lds, c, G, FP{});
@@ -161,11 +166,17 @@ template <typename F, int P, int L, int... Rs> static XEntry x_entry() {
 
 static const ColEntry* col_table(int* n) {
     static const ColEntry t[] = {
+        // single grids of 256^3 and 512^3
         col_entry<float, 8, 512, 8, 8, 8>(),   col_entry<float, 16, 512, 2, 16, 16>(),
         col_entry<float, 32, 512, 16, 32>(),   col_entry<float, 8, 256, 4, 8, 8>(),
         col_entry<float, 16, 256, 16, 16>(),   col_entry<double, 8, 512, 8, 8, 8>(),
         col_entry<double, 16, 512, 2, 16, 16>(), col_entry<double, 8, 256, 4, 8, 8>(),
         col_entry<double, 16, 256, 16, 16>(),
+        // the 3D ensemble's sizes {64, 96, 128, 160, 192}^3 (3*2^a and 5*2^a: 12 and 20 points)
+        col_entry<float, 8, 64, 8, 8>(),       col_entry<float, 16, 64, 4, 16>(),
+        col_entry<float, 12, 96, 3, 2, 4, 4>(), col_entry<float, 8, 128, 2, 8, 8>(),
+        col_entry<float, 16, 128, 8, 16>(),    col_entry<float, 20, 160, 5, 2, 4, 4>(),
+        col_entry<float, 12, 192, 3, 4, 4, 4>(),
     };
     *n = (int)(sizeof(t) / sizeof(t[0]));
     return t;
@@ -175,6 +186,9 @@ static const XEntry* x_table(int* n) {
     static const XEntry t[] = {
         x_entry<float, 8, 256, 4, 8, 8>(),  x_entry<float, 16, 256, 16, 16>(), x_entry<float, 8, 128, 2, 8, 8>(),
         x_entry<float, 16, 128, 8, 16>(),   x_entry<double, 8, 256, 4, 8, 8>(), x_entry<double, 8, 128, 2, 8, 8>(),
+        // x lengths nx/2 of the 3D ensemble
+        x_entry<float, 8, 32, 4, 8>(),      x_entry<float, 12, 48, 3, 4, 4>(), x_entry<float, 8, 64, 8, 8>(),
+        x_entry<float, 20, 80, 5, 4, 4>(),  x_entry<float, 12, 96, 3, 2, 4, 4>(),
     };
     *n = (int)(sizeof(t) / sizeof(t[0]));
     return t;
@@ -196,7 +210,7 @@ static int set_lds_fx(const void* fn, size_t bytes) {
 int fixed_step_col(const LinePlan& lp, void* W, const ColGeom& g, int kind, double scale, int split,
                    int64_t* d_step, hipStream_t s, bool* done) {
     *done = false;
-    if (!(g_fixed.load() & MCS_FIXED_COLUMNS)) return MCS_OK;
+    if (!(g_fixed.load() & MCS_FIXED_COLUMNS) || lp.nofx) return MCS_OK;
     int n = 0;
     const ColEntry* t = col_table(&n);
     const ColEntry* e = nullptr;
@@ -252,7 +266,7 @@ static int fixed_xupd_t(const XEntry& e, const StepGeom& sg, const StepPhys& ph,
 
 int fixed_xupd(const StepGeom& sg, const StepPhys& ph, void* W, void* U, hipStream_t s, bool* done) {
     *done = false;
-    if (!(g_fixed.load() & MCS_FIXED_XPASS)) return MCS_OK;
+    if (!(g_fixed.load() & MCS_FIXED_XPASS) || sg.px.nofx) return MCS_OK;
     int n = 0;
     const XEntry* t = x_table(&n);
     for (int i = 0; i < n; ++i)
@@ -261,4 +275,19 @@ int fixed_xupd(const StepGeom& sg, const StepPhys& ph, void* W, void* U, hipStre
             return sg.prec == 64 ? fixed_xupd_t<double>(t[i], sg, ph, W, U, s) : fixed_xupd_t<float>(t[i], sg, ph, W, U, s);
         }
     return MCS_OK;
+}
+
+bool fixed_available(const LinePlan& lp, bool xpass) {
+    if (!(g_fixed.load() & (xpass ? MCS_FIXED_XPASS : MCS_FIXED_COLUMNS))) return false;
+    int n = 0;
+    if (xpass) {
+        const XEntry* t = x_table(&n);
+        for (int i = 0; i < n; ++i)
+            if (plan_matches(t[i], lp)) return true;
+    } else {
+        const ColEntry* t = col_table(&n);
+        for (int i = 0; i < n; ++i)
+            if (plan_matches(t[i], lp)) return true;
+    }
+    return false;
 }

@@ -10,3 +10,5 @@ int fixed_step_col(const LinePlan& lp, void* W, const ColGeom& g, int kind, doub
                    int64_t* d_step, hipStream_t s, bool* done);
 // the x pass with the update, likewise
 int fixed_xupd(const StepGeom& sg, const StepPhys& ph, void* W, void* U, hipStream_t s, bool* done);
+// whether a fixed-plan kernel exists for lp (x pass or column passes) and its bit is set
+bool fixed_available(const LinePlan& lp, bool xpass);

@@ -18,11 +18,25 @@ template <> struct cplx_of<double> { using type = double2; };
 template <typename F> using C2 = typename cplx_of<F>::type;
 
 template <typename F> MCS_HD C2<F> mk(F x, F y) { C2<F> r; r.x = x; r.y = y; return r; }
+// fused multiply-add written out: device code is built without implicit contraction, so
+// these are the only fused operations and every kernel that shares an expression rounds it alike
+MCS_HD float mfma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+MCS_HD double mfma(double a, double b, double c) { return __builtin_fma(a, b, c); }
 template <typename T> MCS_HD T cadd(T a, T b) { T r; r.x = a.x + b.x; r.y = a.y + b.y; return r; }
 template <typename T> MCS_HD T csub(T a, T b) { T r; r.x = a.x - b.x; r.y = a.y - b.y; return r; }
-template <typename T> MCS_HD T cmul(T a, T b) { T r; r.x = a.x * b.x - a.y * b.y; r.y = a.x * b.y + a.y * b.x; return r; }
+template <typename T> MCS_HD T cmul(T a, T b) {
+    T r;
+    r.x = mfma(a.x, b.x, -(a.y * b.y));
+    r.y = mfma(a.x, b.y, a.y * b.x);
+    return r;
+}
 // a * conj(b)
-template <typename T> MCS_HD T cmulc(T a, T b) { T r; r.x = a.x * b.x + a.y * b.y; r.y = a.y * b.x - a.x * b.y; return r; }
+template <typename T> MCS_HD T cmulc(T a, T b) {
+    T r;
+    r.x = mfma(a.x, b.x, a.y * b.y);
+    r.y = mfma(a.y, b.x, -(a.x * b.y));
+    return r;
+}
 template <typename T> MCS_HD T cconj(T a) { T r; r.x = a.x; r.y = -a.y; return r; }
 template <typename T, typename S> MCS_HD T cscale(T a, S s) { T r; r.x = a.x * s; r.y = a.y * s; return r; }
 // multiply by +i (s = +1) or -i (s = -1)

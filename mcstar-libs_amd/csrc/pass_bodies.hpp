@@ -22,10 +22,10 @@ MCS_DEV int opaque_tid() {
 
 // ------------------------------------------------------------------ column pass
 // tile of G adjacent x columns (thread c = tid % G, t = tid / G), L = pl.L points each
-template <typename F, int P, int DIR>
+template <typename F, int P, int DIR, typename FP = RunPlan>
 MCS_DEV void col_body(const C2<F>* src, C2<F>* dst, const ColGeom& g, int G, const FftPlanDev& pl,
                       const C2<F>* __restrict__ tw, C2<F>* lds, int tile, int outer, int tid) {
-    const int S = pl.L / P;
+    const int S = (FP::fixed ? FP::L : pl.L) / P;
     const int c = tid % G, t = tid / G;
     const int x = tile * G + c;
     const bool valid = x < g.nxc && t < S;
@@ -33,19 +33,20 @@ MCS_DEV void col_body(const C2<F>* src, C2<F>* dst, const ColGeom& g, int G, con
     C2<F> a[P];
 #pragma unroll
     for (int m = 0; m < P; ++m) a[m] = valid ? src[base + (long)(t + m * S) * g.ls] : mk<F>(0, 0);
-    reg_fft<F, P, DIR>(a, pl, tw, t < S ? t : 0, lds, c, G);
+    line_fft<F, P, DIR, false, FP>(a, pl, tw, t < S ? t : 0, lds, c, G);
     if (valid) {
 #pragma unroll
         for (int m = 0; m < P; ++m) dst[base + (long)(t + m * S) * g.ls] = a[m];
     }
 }
 
-// column pass with the nonlinearity between the inverse and forward transforms:
+// column pass with the nonlinearity between the inverse and forward transforms
+// (FP: RunPlan, the line plan pl, or a FixPlan):
 // u = scale * value holds psi at (2m, 2m+1); N = psi^3 (or psi^3 - psi)
-template <typename F, int P, int SPLIT>
+template <typename F, int P, int SPLIT, typename FP = RunPlan>
 MCS_DEV void col_nl_body(C2<F>* W, const ColGeom& g, int G, const FftPlanDev& pl, const C2<F>* __restrict__ tw,
                          F scale, C2<F>* lds, int tile, int outer, int tid) {
-    const int S = pl.L / P;
+    const int S = (FP::fixed ? FP::L : pl.L) / P;
     const int c = tid % G, t = tid / G;
     const int x = tile * G + c;
     const bool valid = x < g.nxc && t < S;
@@ -54,14 +55,14 @@ MCS_DEV void col_nl_body(C2<F>* W, const ColGeom& g, int G, const FftPlanDev& pl
     C2<F> a[P];
 #pragma unroll
     for (int m = 0; m < P; ++m) a[m] = valid ? W[base + (long)(t + m * S) * g.ls] : mk<F>(0, 0);
-    reg_fft<F, P, +1>(a, pl, tw, tt, lds, c, G);
+    line_fft<F, P, +1, false, FP>(a, pl, tw, tt, lds, c, G);
 #pragma unroll
     for (int m = 0; m < P; ++m) {
         const F u = a[m].x * scale, v = a[m].y * scale;
-        if (SPLIT == MCS_SPLIT_REFERENCE) a[m] = mk<F>(u * u * u - u, v * v * v - v);
+        if (SPLIT == MCS_SPLIT_REFERENCE) a[m] = mk<F>(mfma(u * u, u, -u), mfma(v * v, v, -v));
         else a[m] = mk<F>(u * u * u, v * v * v);
     }
-    reg_fft<F, P, -1>(a, pl, tw, tt, lds, c, G);
+    line_fft<F, P, -1, false, FP>(a, pl, tw, tt, lds, c, G);
     if (valid) {
 #pragma unroll
         for (int m = 0; m < P; ++m) W[base + (long)(t + m * S) * g.ls] = a[m];
@@ -115,7 +116,7 @@ MCS_DEV void xupd_pair(C2<F> nA, C2<F> nBr, C2<F> pA, C2<F> pBr, int p, int iy, 
     const C2<F> n1 = cadd(eN, woN), n2 = csub(eN, woN);
     const C2<F> p1 = cadd(eP, woP), p2 = csub(eP, woP);
     const F ky = F(ksigned(iy, ar.ny)) * ar.dky, kz = F(ksigned(iz, ar.nz)) * ar.dkz;
-    const F kyz = ky * ky + kz * kz;
+    const F kyz = mfma(ky, ky, kz * kz);
     const F kx1 = F(p) * ar.dkx;
     const F kx2 = F(ar.L - p) * ar.dkx;  // |k_x| of index p + nx/2 (the Nyquist index for p = 0)
     C2<F> xi1 = mk<F>(0, 0), xi2 = mk<F>(0, 0);
@@ -136,8 +137,8 @@ MCS_DEV void xupd_pair(C2<F> nA, C2<F> nBr, C2<F> pA, C2<F> pBr, int p, int iy, 
             }
         }
     }
-    const C2<F> q1 = pfc_mode_update<F>(p1, n1, xi1, kx1 * kx1 + kyz, ar.dt, ar.lin0, ar.amp);
-    const C2<F> q2 = pfc_mode_update<F>(p2, n2, xi2, kx2 * kx2 + kyz, ar.dt, ar.lin0, ar.amp);
+    const C2<F> q1 = pfc_mode_update<F>(p1, n1, xi1, mfma(kx1, kx1, kyz), ar.dt, ar.lin0, ar.amp);
+    const C2<F> q2 = pfc_mode_update<F>(p2, n2, xi2, mfma(kx2, kx2, kyz), ar.dt, ar.lin0, ar.amp);
     if (p == 0 && iy == 0 && iz == 0) {
         // k = 0: the mean density is carried in FP64 and re-imposed exactly
         const double m = ar.mass;

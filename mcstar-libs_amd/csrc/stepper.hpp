@@ -59,8 +59,9 @@ ColGeom step_geom_y(const StepGeom& sg);
 ColGeom step_geom_z(const StepGeom& sg);
 
 // autotuner: picks the register plan and tile shape of every pass for this shape and
-// precision (cached per process); a no-op when tuning is switched off.  Overwrites W and U.
-int step_geom_tune(StepGeom* sg, const StepPhys& ph, void* W, void* U, hipStream_t s);
+// precision (cached per process and kernel family: ensemble groups time the runtime-plan
+// kernels only); a no-op when tuning is switched off.  Overwrites W and U.
+int step_geom_tune(StepGeom* sg, const StepPhys& ph, void* W, void* U, hipStream_t s, bool ensemble = false);
 // {P_x, pairs per workgroup, P_y, lines per workgroup, P_z, lines per workgroup}
 void step_geom_plan_info(const StepGeom& sg, int info[6]);
 // installs an explicit plan in that layout after checking it against the candidate lists

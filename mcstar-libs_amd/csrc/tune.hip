@@ -2,11 +2,16 @@
 // the caller's own buffers, every candidate register plan (points per thread P,
 // hence the radix sequence and the number of LDS exchanges) and tile shape
 // (lines or row pairs per workgroup, hence the global segment width and the
-// LDS footprint) of each pass, and keeps the fastest.  Choices are cached per
-// (nx, ny, nz, precision) for the life of the process, so that a stepper and an
-// ensemble group of the same shape run identical plans, and therefore
-// bit-identical arithmetic.  Tuning runs with the noise off; the pass kernels
+// LDS footprint) of each pass, and keeps the fastest.  Where a fixed-plan
+// kernel exists for a candidate, it is timed against the runtime-plan kernel of
+// the same plan and the faster one is kept.  Choices are cached per
+// (nx, ny, nz, precision, kernel family) for the life of the process: steppers
+// of one shape share plans, and so do ensemble groups of one shape, which tune
+// on the runtime-plan kernels their work-table kernels run.  A stepper and an
+// ensemble replica agree bit for bit when they run the same plan (the default
+// plans with tuning off, or an explicit plan).  Tuning runs with the noise off; the pass kernels
 // overwrite the buffers, so it runs before a field is loaded.
+#include "fixed.hpp"
 #include "stepper.hpp"
 
 #include <algorithm>
@@ -20,13 +25,14 @@ namespace {
 struct PassChoice {
     int P = 0, G = 0;
     double ms = 0;
+    bool nofx = false;  // runtime-plan kernel (where a fixed-plan one exists and was slower)
 };
 struct StepChoice {
     PassChoice x, y, z;
 };
 
 std::mutex g_mu;
-std::map<std::tuple<int, int, int, int>, StepChoice> g_cache;
+std::map<std::tuple<int, int, int, int, int>, StepChoice> g_cache;
 int g_enabled = 1;
 
 // mean milliseconds of one launch: one warm-up, then enough launches for about a millisecond
@@ -85,19 +91,25 @@ int rebuild(LinePlan* lp, int L, int prec, int P) {
     return MCS_OK;
 }
 
-// best (P, G) of one pass; `run` launches the pass with the candidate plan
+// best (P, G) of one pass; `run` launches the pass with the candidate plan; with_fixed: also time
+// the fixed-plan kernel of each candidate that has one
 template <typename Run>
-int tune_pass(int L, int prec, int maxP, bool xpass, hipStream_t s, Run&& run, PassChoice* best) {
+int tune_pass(int L, int prec, int maxP, bool xpass, bool with_fixed, hipStream_t s, Run&& run, PassChoice* best) {
     best->ms = -1;
     const std::vector<int> opts = line_plan_reg_options(L, prec);
     for (int P : opts) {
         if (P > maxP && P != opts[0]) continue;  // the default plan is always a candidate
         LinePlan cand;
         if (line_plan_build_reg(&cand, L, prec, P) != MCS_OK) continue;
+        const bool fx = with_fixed && fixed_available(cand, xpass);
         for (int G : xpass ? x_tiles(L, P, prec) : col_tiles(L, P, prec)) {
             cand.G = G;
-            const double ms = time_launch(s, [&]() { return run(cand); });
-            if (ms > 0 && (best->ms < 0 || ms < best->ms)) *best = PassChoice{P, G, ms};
+            // the fixed-plan kernel (when one exists) and the runtime-plan kernel of this plan
+            for (int nofx = fx ? 0 : 1; nofx <= 1; ++nofx) {
+                cand.nofx = nofx != 0;
+                const double ms = time_launch(s, [&]() { return run(cand); });
+                if (ms > 0 && (best->ms < 0 || ms < best->ms)) *best = PassChoice{P, G, ms, cand.nofx};
+            }
         }
         line_plan_free(&cand);
     }
@@ -113,7 +125,12 @@ int apply(StepGeom* sg, const StepChoice& c) {
     if (st) return st;
     sg->px.G = c.x.G;
     sg->py.G = c.y.G;
-    if (sg->is3d) sg->pz.G = c.z.G;
+    sg->px.nofx = c.x.nofx;
+    sg->py.nofx = c.y.nofx;
+    if (sg->is3d) {
+        sg->pz.G = c.z.G;
+        sg->pz.nofx = c.z.nofx;
+    }
     return MCS_OK;
 }
 
@@ -132,8 +149,8 @@ extern "C" int mcs_get_autotune(int* on) {
     return MCS_OK;
 }
 
-int step_geom_tune(StepGeom* sg, const StepPhys& ph_in, void* W, void* U, hipStream_t s) {
-    const auto key = std::make_tuple(sg->nx, sg->ny, sg->nz, sg->prec);
+int step_geom_tune(StepGeom* sg, const StepPhys& ph_in, void* W, void* U, hipStream_t s, bool ensemble) {
+    const auto key = std::make_tuple(sg->nx, sg->ny, sg->nz, sg->prec, ensemble ? 1 : 0);
     {
         std::lock_guard<std::mutex> l(g_mu);
         if (!g_enabled) return MCS_OK;
@@ -147,22 +164,23 @@ int step_geom_tune(StepGeom* sg, const StepPhys& ph_in, void* W, void* U, hipStr
     const int maxX = prec == 64 ? 8 : 16, maxC = prec == 64 ? 16 : 32;
     const double scale = 2.0 / ((double)sg->nx * sg->ny * sg->nz);
     StepChoice c;
-    int st = tune_pass(sg->L, prec, maxX, true, s, [&](const LinePlan& lp) {
+    const bool wf = !ensemble;  // the ensemble's work-table kernels run the runtime plans
+    int st = tune_pass(sg->L, prec, maxX, true, wf, s, [&](const LinePlan& lp) {
         StepGeom g = *sg;
         g.px = lp;
         return launch_xupd(g, ph, W, U, s);
     }, &c.x);
     if (!st && sg->is3d) {
         const ColGeom gy = step_geom_y(*sg), gz = step_geom_z(*sg);
-        st = tune_pass(sg->ny, prec, maxC, false, s, [&](const LinePlan& lp) {
+        st = tune_pass(sg->ny, prec, maxC, false, wf, s, [&](const LinePlan& lp) {
             return launch_step_col(lp, W, gy, -1, scale, ph.split, nullptr, s);
         }, &c.y);
-        if (!st) st = tune_pass(sg->nz, prec, maxC, false, s, [&](const LinePlan& lp) {
+        if (!st) st = tune_pass(sg->nz, prec, maxC, false, wf, s, [&](const LinePlan& lp) {
             return launch_col_nl(lp, W, gz, scale, ph.split, nullptr, s);
         }, &c.z);
     } else if (!st) {
         const ColGeom gy = step_geom_y(*sg);
-        st = tune_pass(sg->ny, prec, maxC, false, s, [&](const LinePlan& lp) {
+        st = tune_pass(sg->ny, prec, maxC, false, wf, s, [&](const LinePlan& lp) {
             return launch_col_nl(lp, W, gy, scale, ph.split, nullptr, s);
         }, &c.y);
     }

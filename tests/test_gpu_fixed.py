@@ -1,11 +1,9 @@
-"""The fixed-plan kernels of the fused step compute what the runtime-plan kernels
-compute, on plans the fixed table holds (pinned) and on the tuned plans, with
-and without noise, in both splits, FP32 and FP64, 2D and 3D.  The operations
-are the same; the compiler fuses multiply-adds per basic block, and the fixed
-plans are one straight block where the runtime plans branch per pass, so
-individual values may round differently: the states agree to a few units of
-the last place (relative L2 within 1e-6 in FP32 and 1e-14 in FP64 after 12
-steps), and the mass and step counters exactly."""
+"""The fixed-plan kernels of the fused step compute exactly what the runtime-plan
+kernels compute: the same state bit for bit after several steps, on plans the
+fixed table holds (pinned) and on the tuned plans, with and without noise, in
+both splits, FP32 and FP64, 2D and 3D.  Device code is built without implicit
+multiply-add contraction (every fused operation is written out), so kernels
+built from the same expressions round alike whatever their control flow."""
 import numpy as np
 import pytest
 
@@ -41,17 +39,14 @@ def _run(mcs, psi, shp, dtype, temp, split, plan, mask, steps):
 
 
 @pytest.mark.parametrize("shp,dtype,temp,split,plan", CASES)
-def test_fixed_kernels_match_runtime_plans(mcs, shp, dtype, temp, split, plan):
+def test_fixed_kernels_bit_identical_to_runtime_plans(mcs, shp, dtype, temp, split, plan):
     psi = mcs.init_field(shp, -0.25, 0.05, 11, dtype)
     a = _run(mcs, psi, shp, dtype, temp, split, plan, mcs.FIXED_COLUMNS | mcs.FIXED_XPASS, 12)
     b = _run(mcs, psi, shp, dtype, temp, split, plan, 0, 12)
     assert a[3] == b[3]  # the same plans ran
     assert a[1] == b[1] and a[2] == b[2] == 12
     assert np.isfinite(a[0]).all()
-    x, y = a[0].astype(np.complex128), b[0].astype(np.complex128)
-    rel = np.linalg.norm(x - y) / np.linalg.norm(y)
-    print(f"{shp} {np.dtype(dtype).name} T={temp} split={split} plan={plan}: rel-L2 fixed vs runtime {rel:.2e}")
-    assert rel < (1e-6 if dtype == np.float32 else 1e-14)
+    assert np.array_equal(a[0].view(np.uint8), b[0].view(np.uint8))
 
 
 def test_fixed_kernel_mask(mcs):
