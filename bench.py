@@ -160,13 +160,25 @@ def analysis_bench(mcs, st, field, h):
     t2 = time.perf_counter()
     nbr, q, h4, h6 = mcs.bond_order_knn(ijk, n, h, 12)
     t3 = time.perf_counter()
+    # the reference's form: neighbours within rlim of the peak positions i*h in a box of n*h; rlim 1.05 a for
+    # a bcc lattice of the peaks' density (first and second shells, 14 neighbours)
+    xyz = ijk.astype(np.float64) * h
+    box = n * h
+    rlim = 1.05 * (2.0 * box ** 3 / max(pk.size, 1)) ** (1.0 / 3.0)
+    t4 = time.perf_counter()
+    nb, out9 = mcs.bond_order_cutoff(xyz, box, rlim)
+    t5 = time.perf_counter()
     return {"workload": f"peaks >= mean of the {n}^3 FP32 field after 500 steps, then 12 nearest neighbours, "
                         f"q4 q6 q8 w4 and 64-bin q4/q6 histograms of every peak (cell list, FP64)",
             "peaks": int(pk.size), "findpeaks_ms": round(1e3 * (t1 - t0), 2), "bond_order_ms": round(1e3 * (t3 - t2), 2),
             "timing": "wall clock of each call: the field is on the device; the bond order uploads the peak "
                       "coordinates and downloads neighbours, q values and histograms",
             "q4_mean": float(q[:, 0].mean()), "q6_mean": float(q[:, 1].mean()),
-            "q6_mode_bin": int(np.argmax(h6))}
+            "q6_mode_bin": int(np.argmax(h6)),
+            "cutoff_form": {"rlim": round(rlim, 4), "bond_order_ms": round(1e3 * (t5 - t4), 2),
+                            "mean_neighbours": float(nb.mean()),
+                            "note": "q4 q6 q8 w4, neighbour-averaged forms and xi of every peak (the reference's "
+                                    "bond_order_analysis), cell list"}}
 
 
 def fft_bench(mcs, n, reps):

@@ -9,9 +9,12 @@
 // (267-287).  Y_lm terms with m < 0 vanish within 1e-3 of cos(theta) = -1 and
 // with m > 0 within 1e-3 of +1 (123-133).
 //
-// Cutoff mode: one thread per point scans all points in index order, so the
-// sums run in the reference's order (distances with contraction off, so
-// neighbour sets match the CPU bit for bit).
+// Cutoff mode: one thread per point visits its neighbours in ascending index
+// order, so the sums run in the reference's order (distances with contraction
+// off, so neighbour sets match the CPU bit for bit).  The neighbours come from
+// a cell list when every point lies in the box, the box holds at least 3 cells
+// of width rlim per axis and no point has more than 64 neighbours; otherwise
+// from a scan of all points.
 //
 // k-nearest mode: integer squared distances on the periodic grid, ties broken
 // by the smaller index, exact Y_lm, q4 q6 q8 w4 and histograms of q4, q6.  The
@@ -111,15 +114,27 @@ MCS_DEV bool bond(const double* xyz, int i, int j, double box, double rlim, doub
 }
 
 static constexpr int QS = 80;  // doubles per point: 39 complex + pad
+static constexpr int CMAXN = 64;  // neighbours per point the cutoff cell list holds
 
+// The neighbours of point i, in ascending index order: every j (all-pairs form) or the
+// sorted list nl[i*CMAXN ..] of cnt[i] candidates within rlim (cell-list form); the bond
+// test is repeated for both, so both forms visit the same j in the same order.
+template <bool LIST>
+MCS_DEV int nbr_count(int P, const int* nc, int i) { return LIST ? nc[i] : P; }
+template <bool LIST>
+MCS_DEV int nbr_at(const int* nl, int i, int u) { return LIST ? nl[(long)CMAXN * i + u] : u; }
+
+template <bool LIST>
 __global__ void __launch_bounds__(128) k_bo_qlm(const double* xyz, int P, double box, double rlim, double* q,
-                                                int* nb, int* last) {
+                                                int* nb, int* last, const int* nl, const int* nc) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= P) return;
     double acc[78];
     for (int t = 0; t < 78; ++t) acc[t] = 0;
     int cnt = 0, lj = -1;
-    for (int j = 0; j < P; ++j) {
+    const int nn = nbr_count<LIST>(P, nc, i);
+    for (int u = 0; u < nn; ++u) {
+        const int j = nbr_at<LIST>(nl, i, u);
         if (j == i) continue;
         double ct, ph;
         if (bond(xyz, i, j, box, rlim, &ct, &ph)) { add_bond(acc, ct, ph, true); ++cnt; lj = j; }
@@ -130,8 +145,10 @@ __global__ void __launch_bounds__(128) k_bo_qlm(const double* xyz, int P, double
     last[i] = lj;
 }
 
+template <bool LIST>
 __global__ void __launch_bounds__(128) k_bo_stats(const double* xyz, int P, double box, double rlim, const double* q,
-                                                  const int* nb, const int* last, Wig4 W, double* out) {
+                                                  const int* nb, const int* last, Wig4 W, double* out, const int* nl,
+                                                  const int* nc) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= P) return;
     double* o = out + 9L * i;
@@ -151,7 +168,9 @@ __global__ void __launch_bounds__(128) k_bo_stats(const double* xyz, int P, doub
     for (int m = 0; m < 13; ++m) den_i += qi[18 + 2 * m] * qi[18 + 2 * m] + qi[19 + 2 * m] * qi[19 + 2 * m];
     den_i = sqrt(den_i);
     int xi = 0;
-    for (int j = 0; j < P; ++j) {
+    const int nn = nbr_count<LIST>(P, nc, i);
+    for (int u = 0; u < nn; ++u) {
+        const int j = nbr_at<LIST>(nl, i, u);
         if (j == i) continue;
         double ct, ph;
         if (!bond(xyz, i, j, box, rlim, &ct, &ph)) continue;
@@ -175,6 +194,67 @@ __global__ void __launch_bounds__(128) k_bo_stats(const double* xyz, int P, doub
     for (int t = 0; t < 18; ++t) b4[t] = ql[t] / (double)cnt;
     o[7] = w4_dev(b4, W);
     o[8] = (double)xi;
+}
+
+// ---------------------------------------------------------------- cutoff cell list
+// Points in [0, box)^3 are binned into g^3 cells of width box/g >= rlim (1 + 1e-6), g >= 3: a
+// neighbour within rlim under the minimum image lies in one of the 27 cells around a point's
+// own.  Each point gathers the candidates of those cells that pass the bond test, sorts
+// them by index and keeps at most CMAXN (a larger count sets the overflow flag and the
+// caller falls back to the all-pairs form).
+MCS_DEV int cut_cell(double x, double box, int g) {
+    int c = (int)(x * g / box);
+    return c < 0 ? 0 : (c >= g ? g - 1 : c);
+}
+
+__global__ void __launch_bounds__(256) k_cut_count(const double* xyz, int P, double box, int g, int* cnt) {
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < P; i += gridDim.x * 256) {
+        const int c = (cut_cell(xyz[3 * i + 2], box, g) * g + cut_cell(xyz[3 * i + 1], box, g)) * g +
+                      cut_cell(xyz[3 * i], box, g);
+        atomicAdd(&cnt[c], 1);
+    }
+}
+
+__global__ void __launch_bounds__(256) k_cut_scatter(const double* xyz, int P, double box, int g, const int* start,
+                                                     int* cnt, int* sorted) {
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < P; i += gridDim.x * 256) {
+        const int c = (cut_cell(xyz[3 * i + 2], box, g) * g + cut_cell(xyz[3 * i + 1], box, g)) * g +
+                      cut_cell(xyz[3 * i], box, g);
+        sorted[start[c] + atomicSub(&cnt[c], 1) - 1] = i;
+    }
+}
+
+__global__ void __launch_bounds__(128) k_cut_nbrs(const double* xyz, int P, double box, double rlim, int g,
+                                                  const int* start, const int* sorted, int* nl, int* nc,
+                                                  int* overflow) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P) return;
+    const int cx = cut_cell(xyz[3 * i], box, g), cy = cut_cell(xyz[3 * i + 1], box, g),
+              cz = cut_cell(xyz[3 * i + 2], box, g);
+    int* li = nl + (long)CMAXN * i;
+    int n = 0;
+    for (int dz = -1; dz <= 1; ++dz)
+        for (int dy = -1; dy <= 1; ++dy)
+            for (int dx = -1; dx <= 1; ++dx) {
+                const int c = ((cz + dz + g) % g * g + (cy + dy + g) % g) * g + (cx + dx + g) % g;
+                for (int p = start[c]; p < start[c + 1]; ++p) {
+                    const int j = sorted[p];
+                    double ct, ph;
+                    if (j == i || !bond(xyz, i, j, box, rlim, &ct, &ph)) continue;
+                    if (n == CMAXN) {
+                        atomicOr(overflow, 1);
+                        nc[i] = n;
+                        return;
+                    }
+                    int pos = n++;  // insertion in ascending index order
+                    while (pos > 0 && li[pos - 1] > j) {
+                        li[pos] = li[pos - 1];
+                        --pos;
+                    }
+                    li[pos] = j;
+                }
+            }
+    nc[i] = n;
 }
 
 static constexpr int KMAX = 32;
@@ -545,13 +625,66 @@ int knn_launch(const int64_t* d_idx, const int* d_ijk, const int64_t* d_count, i
     return MCS_OK;
 }
 
+// Sorted neighbour lists of the cutoff form through the cell list, when it applies: every
+// point in [0, box)^3, at least 3 cells of width >= rlim per axis, and no point with more
+// than CMAXN neighbours.  *listed tells whether the lists (*dnl, *dnc, allocated here) hold.
+static int cutoff_lists(const double* dx, const double* xyz, int P, double box, double rlim, hipStream_t s,
+                        bool* listed, int** dnl, int** dnc) {
+    *listed = false;
+    if (!(box > 0) || !(rlim > 0) || P < 256) return MCS_OK;
+    for (long i = 0; i < 3L * P; ++i)
+        if (!(xyz[i] >= 0.0 && xyz[i] < box)) return MCS_OK;
+    const double gw = std::floor(box / (rlim * (1.0 + 1e-6)));
+    if (!(gw >= 3.0)) return MCS_OK;
+    const int g = (int)std::min(gw, 256.0);
+    const long ncell = (long)g * g * g;
+    int *cnt = nullptr, *start = nullptr, *sorted = nullptr, *ovf = nullptr;
+    int st = MCS_OK;
+    if (hipMalloc((void**)&cnt, (ncell + 1) * sizeof(int)) != hipSuccess ||
+        hipMalloc((void**)&start, (ncell + 1) * sizeof(int)) != hipSuccess ||
+        hipMalloc((void**)&sorted, (size_t)P * sizeof(int)) != hipSuccess ||
+        hipMalloc((void**)&ovf, sizeof(int)) != hipSuccess ||
+        hipMalloc((void**)dnl, (size_t)CMAXN * P * sizeof(int)) != hipSuccess ||
+        hipMalloc((void**)dnc, (size_t)P * sizeof(int)) != hipSuccess) {
+        mcs_set_last_error(__FILE__, __LINE__, "bond-order cell list allocation failed");
+        st = MCS_ERR_NOMEM;
+    }
+    int overflow = 0;
+    if (!st) {
+        KnnGrid kg{};
+        kg.ncell = (int)ncell;
+        KnnGrid* dkg = nullptr;
+        const unsigned gp = (unsigned)std::min<long>(4096, (P + 255) / 256);
+        const unsigned gz = (unsigned)std::min<long>(1024, (ncell + 256) / 256);
+        if (hipMalloc((void**)&dkg, sizeof(KnnGrid)) != hipSuccess) st = MCS_ERR_NOMEM;
+        if (!st && hipMemcpyAsync(dkg, &kg, sizeof(KnnGrid), hipMemcpyHostToDevice, s) != hipSuccess) st = MCS_ERR_HIP;
+        if (!st && hipMemsetAsync(ovf, 0, sizeof(int), s) != hipSuccess) st = MCS_ERR_HIP;
+        if (!st) {
+            hipLaunchKernelGGL(k_knn_zero, dim3(gz), dim3(256), 0, s, dkg, cnt);
+            hipLaunchKernelGGL(k_cut_count, dim3(gp), dim3(256), 0, s, dx, P, box, g, cnt);
+            hipLaunchKernelGGL(k_knn_scan, dim3(1), dim3(1024), 0, s, dkg, cnt, start);
+            hipLaunchKernelGGL(k_cut_scatter, dim3(gp), dim3(256), 0, s, dx, P, box, g, start, cnt, sorted);
+            hipLaunchKernelGGL(k_cut_nbrs, dim3((unsigned)((P + 127) / 128)), dim3(128), 0, s, dx, P, box, rlim, g, start,
+                               sorted, *dnl, *dnc, ovf);
+            if (hipGetLastError() != hipSuccess) st = MCS_ERR_HIP;
+        }
+        if (!st && hipMemcpyAsync(&overflow, ovf, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess) st = MCS_ERR_HIP;
+        if (!st && hipStreamSynchronize(s) != hipSuccess) st = MCS_ERR_HIP;
+        if (dkg) (void)hipFree(dkg);
+        if (st == MCS_ERR_HIP) mcs_set_last_error(__FILE__, __LINE__, "HIP failure in the bond-order cell list");
+    }
+    (void)hipFree(cnt); (void)hipFree(start); (void)hipFree(sorted); (void)hipFree(ovf);
+    if (!st && !overflow) *listed = true;
+    return st;
+}
+
 extern "C" int mcs_bond_order_cutoff(const double* xyz, int npts, double box, double rlim, int* nb, double* out9) {
     MCS_CHECK(npts >= 0, MCS_ERR_INVALID, "negative point count");
     if (npts == 0) return MCS_OK;
     MCS_CHECK(xyz && nb && out9, MCS_ERR_INVALID, "null argument");
     hipStream_t s = mcs_stream();
     double *dx = nullptr, *dq = nullptr, *dout = nullptr;
-    int *dnb = nullptr, *dlast = nullptr;
+    int *dnb = nullptr, *dlast = nullptr, *dnl = nullptr, *dnc = nullptr;
     int st = MCS_OK;
     if (hipMalloc((void**)&dx, 3L * npts * sizeof(double)) != hipSuccess ||
         hipMalloc((void**)&dq, (long)QS * npts * sizeof(double)) != hipSuccess ||
@@ -565,8 +698,18 @@ extern "C" int mcs_bond_order_cutoff(const double* xyz, int npts, double box, do
         const Wig4 W = wig4();
         const unsigned g = (unsigned)((npts + 127) / 128);
         if (hipMemcpyAsync(dx, xyz, 3L * npts * sizeof(double), hipMemcpyHostToDevice, s) != hipSuccess) st = MCS_ERR_HIP;
-        hipLaunchKernelGGL(k_bo_qlm, dim3(g), dim3(128), 0, s, dx, npts, box, rlim, dq, dnb, dlast);
-        hipLaunchKernelGGL(k_bo_stats, dim3(g), dim3(128), 0, s, dx, npts, box, rlim, dq, dnb, dlast, W, dout);
+        bool listed = false;
+        if (!st) st = cutoff_lists(dx, xyz, npts, box, rlim, s, &listed, &dnl, &dnc);
+        if (!st && listed) {
+            hipLaunchKernelGGL(k_bo_qlm<true>, dim3(g), dim3(128), 0, s, dx, npts, box, rlim, dq, dnb, dlast, dnl, dnc);
+            hipLaunchKernelGGL(k_bo_stats<true>, dim3(g), dim3(128), 0, s, dx, npts, box, rlim, dq, dnb, dlast, W, dout,
+                               dnl, dnc);
+        } else if (!st) {
+            hipLaunchKernelGGL(k_bo_qlm<false>, dim3(g), dim3(128), 0, s, dx, npts, box, rlim, dq, dnb, dlast,
+                               (const int*)nullptr, (const int*)nullptr);
+            hipLaunchKernelGGL(k_bo_stats<false>, dim3(g), dim3(128), 0, s, dx, npts, box, rlim, dq, dnb, dlast, W, dout,
+                               (const int*)nullptr, (const int*)nullptr);
+        }
         if (!st && hipGetLastError() != hipSuccess) st = MCS_ERR_HIP;
         if (!st && hipMemcpyAsync(nb, dnb, npts * sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess) st = MCS_ERR_HIP;
         if (!st && hipMemcpyAsync(out9, dout, 9L * npts * sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess)
@@ -575,6 +718,7 @@ extern "C" int mcs_bond_order_cutoff(const double* xyz, int npts, double box, do
         if (st) mcs_set_last_error(__FILE__, __LINE__, "HIP failure in bond order");
     }
     (void)hipFree(dx); (void)hipFree(dq); (void)hipFree(dout); (void)hipFree(dnb); (void)hipFree(dlast);
+    (void)hipFree(dnl); (void)hipFree(dnc);
     return st;
 }
 

@@ -139,3 +139,47 @@ def test_bond_order_fcc_lattice(mcs):
     nbr, out, h4, h6 = mcs.bond_order_knn(np.array(pts), 8, 1.0, 12)
     assert np.allclose(out[:, 0], 0.19094, atol=5e-5) and np.allclose(out[:, 1], 0.57452, atol=5e-5)
     assert h4[int(0.19094 * 64)] == len(pts) and h6[int(0.57452 * 64)] == len(pts)
+
+
+def _cutoff_matches_oracle(mcs, orc, pts, box, rlim):
+    nb, out = mcs.bond_order_cutoff(pts, box, rlim)
+    onb, oout = orc.bond_order_cutoff(pts, box, rlim)
+    assert np.array_equal(nb, onb)
+    assert np.array_equal(np.isnan(out), np.isnan(oout))
+    if not np.isnan(oout).all():
+        assert np.nanmax(np.abs(out - oout)) < 1e-12
+    return nb, out
+
+
+def test_bond_order_cutoff_cell_list(mcs, orc):
+    # inputs that take the cell list (>= 256 points in the box, >= 3 cells of width rlim per axis)
+    rng = np.random.default_rng(21)
+    box = 40.0
+    _cutoff_matches_oracle(mcs, orc, rng.uniform(0, box, size=(3000, 3)), box, 3.0)
+    # a simple cubic lattice with rlim equal to the spacing: neighbours at exactly rlim count (r <= rlim)
+    a, m = 2.5, 10
+    g = np.arange(m) * a
+    X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
+    lat = np.column_stack([X.ravel(), Y.ravel(), Z.ravel()])[rng.permutation(m ** 3)]
+    nb, _ = _cutoff_matches_oracle(mcs, orc, lat, a * m, a)
+    assert (nb == 6).all()
+    # fcc with rlim between the first and second shells: 12 neighbours, q6 of the closed form
+    fcc = []
+    for i in range(7):
+        for j in range(7):
+            for k in range(7):
+                for (x, y, z) in [(0, 0, 0), (0.5, 0.5, 0), (0.5, 0, 0.5), (0, 0.5, 0.5)]:
+                    fcc.append((i + x, j + y, k + z))
+    fcc = np.array(fcc) * 2.0
+    nb, out = _cutoff_matches_oracle(mcs, orc, fcc, 14.0, 1.6)
+    assert (nb == 12).all() and np.allclose(out[:, 1], 0.57452, atol=5e-5)
+    # points on cell faces and at the periodic boundary
+    edge = rng.uniform(0, box, size=(600, 3))
+    edge[:200, 0] = np.round(edge[:200, 0] / (box / 13)) * (box / 13) % box
+    edge[200:300, 1] = 0.0
+    edge[300:400, 2] = np.nextafter(box, 0)
+    _cutoff_matches_oracle(mcs, orc, edge, box, 3.0)
+    # a dense cluster (more than 64 neighbours) falls back to the all-pairs scan
+    dense = np.concatenate([rng.uniform(0, box, size=(400, 3)), 20 + rng.uniform(0, 1.5, size=(200, 3))])
+    nb, _ = _cutoff_matches_oracle(mcs, orc, dense, box, 3.0)
+    assert nb.max() > 64
