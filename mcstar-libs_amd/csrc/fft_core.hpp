@@ -336,6 +336,13 @@ MCS_DEV void fix_fft(C2<F> (&a)[P], const C2<F>* __restrict__ tw, int t, C2<F>* 
     fix_pass<F, P, DIR, WAVE, L, 1, 0, Rs...>(a, tw, t, lds, lbase, lps);
 }
 
+// entries of a fixed plan's twiddle table (Ns (R - 1) per pass with Ns > 1)
+template <int Ns, int R, int... Rest> constexpr int fix_tw_count_from() {
+    if constexpr (sizeof...(Rest) == 0) return Ns > 1 ? Ns * (R - 1) : 0;
+    else return (Ns > 1 ? Ns * (R - 1) : 0) + fix_tw_count_from<Ns * R, Rest...>();
+}
+template <int L, int... Rs> constexpr int fix_tw_count(FixPlan<L, Rs...>) { return fix_tw_count_from<1, Rs...>(); }
+
 // the line transform of a pass body: runtime plan pl, or the fixed plan FP (pl unused)
 template <typename F, int P, int DIR, bool WAVE, typename FP>
 MCS_DEV void line_fft(C2<F> (&a)[P], const FftPlanDev& pl, const C2<F>* __restrict__ tw, int t, C2<F>* lds,

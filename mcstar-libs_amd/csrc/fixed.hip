@@ -68,9 +68,15 @@ __global__ void __launch_bounds__(MCS_REG_MAX_THREADS(P, F)) k_col_fx(C2<F>* W, 
     char* wb = reinterpret_cast<char*>(W + (long)outer * g.os + (long)tile * G);
     const long mst = (long)S * g.ls * (long)sizeof(C2<F>);
     const unsigned off = (unsigned)((c + (long)t * g.ls) * (long)sizeof(C2<F>));
+    // the twiddle table in LDS behind the exchange area: a pass reads its twiddles in the
+    // middle of the transform, where an L2 round trip stalls the wave
+    constexpr int NTW = fix_tw_count(FP{});
+    C2<F>* twl = lds + G * FP::L;
+    for (int i = threadIdx.x; i < NTW; i += blockDim.x) twl[i] = tw[i];
+    __syncthreads();
     C2<F> a[P];
     fx_col_load<F, P, FP>(a, wb, mst, off, valid);
-    fix_fft<F, P, DIR, false>(a, tw, t, lds, c, G, FP{});
+    fix_fft<F, P, DIR, false>(a, twl, t, lds, c, G, FP{});
     fx_col_store<F, P, FP>(a, wb, mst, off, valid);
 }
 
@@ -89,16 +95,21 @@ __global__ void __launch_bounds__(MCS_REG_MAX_THREADS(P, F)) k_col_nl_fx(C2<F>* 
     char* wb = reinterpret_cast<char*>(W + (long)outer * g.os + (long)tile * G);
     const long mst = (long)S * g.ls * (long)sizeof(C2<F>);
     const unsigned off = (unsigned)((c + (long)t * g.ls) * (long)sizeof(C2<F>));
+    // the twiddle table in LDS behind the exchange area, read by both transforms
+    constexpr int NTW = fix_tw_count(FP{});
+    C2<F>* twl = lds + G * FP::L;
+    for (int i = threadIdx.x; i < NTW; i += blockDim.x) twl[i] = tw[i];
+    __syncthreads();
     C2<F> a[P];
     fx_col_load<F, P, FP>(a, wb, mst, off, valid);
-    fix_fft<F, P, +1, false>(a, tw, t, lds, c, G, FP{});
+    fix_fft<F, P, +1, false>(a, twl, t, lds, c, G, FP{});
 #pragma unroll
     for (int m = 0; m < P; ++m) {
         const F u = a[m].x * scale, v = a[m].y * scale;
         if (SPLIT == MCS_SPLIT_REFERENCE) a[m] = mk<F>(mfma(u * u, u, -u), mfma(v * v, v, -v));
         else a[m] = mk<F>(u * u * u, v * v * v);
     }
-    fix_fft<F, P, -1, false>(a, tw, t, lds, c, G, FP{});
+    fix_fft<F, P, -1, false>(a, twl, t, lds, c, G, FP{});
     fx_col_store<F, P, FP>(a, wb, mst, off, valid);
     if (d_step && blockIdx.x == 0 && threadIdx.x == 0) *d_step += 1;
 }
@@ -111,6 +122,8 @@ __global__ void __launch_bounds__(256) k_xupd_fx(C2<F>* __restrict__ W, C2<F>* _
     extern __shared__ unsigned char smem_raw[];
     const uint32_t draw = ar.noise ? (uint32_t)*d_step : 0u;
     const FftPlanDev unused{};
+    // (twiddles staged in LDS, also with a persistent grid that copies them once per
+    // workgroup, did not make this pass faster: it keeps the global tables)
     xupd_body<F, P, SPLIT, WAVE, FP>(W, U, pairs, npairs, G, ar, unused, tw, wph, draw,
                                      reinterpret_cast<C2<F>*>(smem_raw), blockIdx.x, threadIdx.x);
 }
@@ -223,7 +236,8 @@ int fixed_step_col(const LinePlan& lp, void* W, const ColGeom& g, int kind, doub
     // per-thread byte offsets (column + t * ls) stay below 2^32
     if (((long)S * g.ls + G) * esz >= (1L << 32)) return MCS_OK;
     const void* fn = kind < 0 ? e->fwd : kind > 0 ? e->inv : (split == MCS_SPLIT_REFERENCE ? e->nl_ref : e->nl_ns);
-    const size_t lds = (size_t)G * lp.L * esz;
+    // the exchange area and the twiddle table
+    const size_t lds = ((size_t)G * lp.L + lp.twcount) * esz;
     int st = set_lds_fx(fn, lds);
     if (st) return st;
     const long tiles = (long)((g.nxc + G - 1) / G) * g.nouter;
