@@ -216,6 +216,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--no-noise-config", action="store_true", help="skip the 256^3 noise-on line")
     ap.add_argument("--only-ensemble", action="store_true", help="run only the ensemble lines (profiling)")
     ap.add_argument("--only-fft", action="store_true", help="run only the boundary FFT lines (profiling)")
     ap.add_argument("--plan", default=None,
@@ -311,8 +312,9 @@ def main():
         log("[bench] peaks and 12-NN bond order of the 500-step field")
         out["analysis"] = analysis_bench(mcs, st, f32, cfg["dx"])
         del f32
-    log("[bench] 256^3 with noise")
-    out["noise_256"] = noise_config_bench(mcs, 100)
+    if not a.no_noise_config:
+        log("[bench] 256^3 with noise")
+        out["noise_256"] = noise_config_bench(mcs, 100)
     log("[bench] boundary FFT")
     out["boundary_fft"] = fft_bench(mcs, n, 5)
     if not a.no_ensemble:
