@@ -13,19 +13,20 @@ template <typename F> struct EnsRep {
 };
 
 // ------------------------------------------------------------------ kernels
-template <typename F, int P, int SPLIT, bool WAVE, typename FP = RunPlan>
+template <typename F, int P, int SPLIT, bool WAVE, typename FP = RunPlan, bool NOISE = true>
 __global__ void __launch_bounds__(256) k_ens_xupd(const EnsRep<F>* __restrict__ reps, int nr, int units,
                                                   const int2* __restrict__ pairs, int npairs, int G, FftPlanDev pl,
                                                   const C2<F>* __restrict__ tw, const C2<F>* __restrict__ wph,
                                                   const int64_t* __restrict__ d_step, int draw_off) {
     extern __shared__ unsigned char smem_raw[];
     C2<F>* lds = reinterpret_cast<C2<F>*>(smem_raw);
-    const uint32_t draw = (uint32_t)(*d_step + draw_off);
+    const uint32_t draw = NOISE ? (uint32_t)(*d_step + draw_off) : 0u;
     const long total = (long)nr * units;
     for (long it = blockIdx.x; it < total; it += gridDim.x) {
         const int r = (int)(it / units), u = (int)(it % units);
         const EnsRep<F>& R = reps[r];
-        xupd_body<F, P, SPLIT, WAVE, FP>(R.W, R.U, pairs, npairs, G, R.xa, pl, tw, wph, draw, lds, u, opaque_tid());
+        xupd_body<F, P, SPLIT, WAVE, FP, NOISE>(R.W, R.U, pairs, npairs, G, R.xa, pl, tw, wph, draw, lds, u,
+                                                opaque_tid());
         __syncthreads();
     }
 }

@@ -113,7 +113,12 @@ static int ens_xupd_t(mcs_ensemble* e, EnsGroup* g, int first, int nr, int draw_
     const int S = lp.L / P;
     const size_t lds = (size_t)G * 2 * lp.L * sizeof(C2<F>);
     const int units = (g->sg.npairs + G - 1) / G;
-    auto fn = (64 % S == 0) ? k_ens_xupd<F, P, SPLIT, true> : k_ens_xupd<F, P, SPLIT, false>;
+    bool noise = false;  // the noise-free kernels carry no noise draw
+    for (int i = first; i < first + nr; ++i) noise = noise || e->reps[g->members[i]].temp > 0;
+    auto fn = noise ? ((64 % S == 0) ? k_ens_xupd<F, P, SPLIT, true, RunPlan, true>
+                                     : k_ens_xupd<F, P, SPLIT, false, RunPlan, true>)
+                    : ((64 % S == 0) ? k_ens_xupd<F, P, SPLIT, true, RunPlan, false>
+                                     : k_ens_xupd<F, P, SPLIT, false, RunPlan, false>);
     const unsigned grid = persistent_grid(e, G * S, lds, (long)nr * units);
     hipLaunchKernelGGL(fn, dim3(grid), dim3(G * S), lds, s, (const EnsRep<F>*)g->d_reps + first, nr, units,
                        g->sg.pairs, g->sg.npairs, G, lp.dev, (const C2<F>*)lp.tw, (const C2<F>*)g->sg.wph,

@@ -114,18 +114,18 @@ __global__ void __launch_bounds__(MCS_REG_MAX_THREADS(P, F)) k_col_nl_fx(C2<F>* 
     if (d_step && blockIdx.x == 0 && threadIdx.x == 0) *d_step += 1;
 }
 
-template <typename F, int P, int SPLIT, bool WAVE, typename FP>
+template <typename F, int P, int SPLIT, bool WAVE, typename FP, bool NOISE>
 __global__ void __launch_bounds__(256) k_xupd_fx(C2<F>* __restrict__ W, C2<F>* __restrict__ U,
                                                  const int2* __restrict__ pairs, int npairs, int G, XArgs<F> ar,
                                                  const C2<F>* __restrict__ tw, const C2<F>* __restrict__ wph,
                                                  const int64_t* __restrict__ d_step) {
     extern __shared__ unsigned char smem_raw[];
-    const uint32_t draw = ar.noise ? (uint32_t)*d_step : 0u;
+    const uint32_t draw = NOISE && ar.noise ? (uint32_t)*d_step : 0u;
     const FftPlanDev unused{};
     // (twiddles staged in LDS, also with a persistent grid that copies them once per
     // workgroup, did not make this pass faster: it keeps the global tables)
-    xupd_body<F, P, SPLIT, WAVE, FP>(W, U, pairs, npairs, G, ar, unused, tw, wph, draw,
-                                     reinterpret_cast<C2<F>*>(smem_raw), blockIdx.x, threadIdx.x);
+    xupd_body<F, P, SPLIT, WAVE, FP, NOISE>(W, U, pairs, npairs, G, ar, unused, tw, wph, draw,
+                                            reinterpret_cast<C2<F>*>(smem_raw), blockIdx.x, threadIdx.x);
 }
 
 // ------------------------------------------------------------------ table
@@ -142,8 +142,10 @@ struct ColEntry {
 struct XEntry {
     int prec, P, L, np;
     int rad[4];
-    const void* ns;
-    const void* ref;
+    const void* ns;   // psi^3 split, with the noise draw
+    const void* ref;  // psi^3 - psi split, with the noise draw
+    const void* ns_quiet;  // the same without it (noise off)
+    const void* ref_quiet;
 };
 
 template <typename F, int P, int L, int... Rs> static ColEntry col_entry() {
@@ -172,8 +174,10 @@ template <typename F, int P, int L, int... Rs> static XEntry x_entry() {
     const int r[] = {Rs...};
     e.np = (int)sizeof...(Rs);
     for (int i = 0; i < e.np; ++i) e.rad[i] = r[i];
-    e.ns = (const void*)k_xupd_fx<F, P, MCS_SPLIT_NORTHSTAR, wave, FP>;
-    e.ref = (const void*)k_xupd_fx<F, P, MCS_SPLIT_REFERENCE, wave, FP>;
+    e.ns = (const void*)k_xupd_fx<F, P, MCS_SPLIT_NORTHSTAR, wave, FP, true>;
+    e.ref = (const void*)k_xupd_fx<F, P, MCS_SPLIT_REFERENCE, wave, FP, true>;
+    e.ns_quiet = (const void*)k_xupd_fx<F, P, MCS_SPLIT_NORTHSTAR, wave, FP, false>;
+    e.ref_quiet = (const void*)k_xupd_fx<F, P, MCS_SPLIT_REFERENCE, wave, FP, false>;
     return e;
 }
 
@@ -265,7 +269,8 @@ static int fixed_xupd_t(const XEntry& e, const StepGeom& sg, const StepPhys& ph,
     const int S = lp.L / lp.P;
     const int G = xupd_pairs_per_wg(lp);
     const size_t lds = (size_t)G * 2 * lp.L * sizeof(C2<F>);
-    const void* fn = ph.split == MCS_SPLIT_REFERENCE ? e.ref : e.ns;
+    const void* fn = ar.noise ? (ph.split == MCS_SPLIT_REFERENCE ? e.ref : e.ns)
+                              : (ph.split == MCS_SPLIT_REFERENCE ? e.ref_quiet : e.ns_quiet);
     int st = set_lds_fx(fn, lds);
     if (st) return st;
     const int2* pairs = sg.pairs;

@@ -105,7 +105,9 @@ MCS_DEV int ksigned(int i, int n) { return i < n / 2 + 1 ? i : i - n; }
 //   Psi1 = E + w^p O (full mode p), Psi2 = E - w^p O (full mode p + nx/2)
 // both modes are updated, then E' = (Psi1' + Psi2')/2, O' = (Psi1' - Psi2') conj(w^p) / 2 and
 //   U'(c, p) = E' + i O',  U'(m, -p) = conj(E') + i conj(O').
-template <typename F>
+// NOISE = false compiles the noise draw out (ar.noise must then be 0): the draw's call keeps
+// many registers live across the unrolled update (the 512-point FP32 x pass: 125 VGPRs against 191)
+template <typename F, bool NOISE = true>
 MCS_DEV void xupd_pair(C2<F> nA, C2<F> nBr, C2<F> pA, C2<F> pBr, int p, int iy, int iz, int my, int mz, C2<F> w,
                        const XArgs<F>& ar, uint32_t draw, C2<F>* uc, C2<F>* um) {
     const C2<F> nB = cconj(nBr), pB = cconj(pBr);
@@ -120,7 +122,7 @@ MCS_DEV void xupd_pair(C2<F> nA, C2<F> nBr, C2<F> pA, C2<F> pBr, int p, int iy, 
     const F kx1 = F(p) * ar.dkx;
     const F kx2 = F(ar.L - p) * ar.dkx;  // |k_x| of index p + nx/2 (the Nyquist index for p = 0)
     C2<F> xi1 = mk<F>(0, 0), xi2 = mk<F>(0, 0);
-    if (ar.noise) {
+    if (NOISE && ar.noise) {
         // one Philox block gives both modes; a self-mirrored row takes mode p + L as the
         // conjugate of its partner L - p (a second block) and real modes at p = 0
         const long rc = (long)iz * ar.ny + iy;
@@ -158,8 +160,8 @@ MCS_DEV void xupd_pair(C2<F> nA, C2<F> nBr, C2<F> pA, C2<F> pBr, int p, int iy, 
 // Each thread updates the mode pairs {(c, p), (m, -p)} of its own positions p once; the
 // mirror row's results move to their owners through LDS.  A self-mirrored row (c == m)
 // is processed as row c alone.
-// FP: RunPlan (the line plan pl) or a FixPlan of length L = nx/2.
-template <typename F, int P, int SPLIT, bool WAVE = false, typename FP = RunPlan>
+// FP: RunPlan (the line plan pl) or a FixPlan of length L = nx/2; NOISE as for xupd_pair.
+template <typename F, int P, int SPLIT, bool WAVE = false, typename FP = RunPlan, bool NOISE = true>
 MCS_DEV void xupd_body(C2<F>* __restrict__ W, C2<F>* __restrict__ U, const int2* __restrict__ pairs, int npairs,
                        int G, const XArgs<F>& ar, const FftPlanDev& pl, const C2<F>* __restrict__ tw,
                        const C2<F>* __restrict__ wph, uint32_t draw, C2<F>* lds, int blk, int tid) {
@@ -205,7 +207,7 @@ MCS_DEV void xupd_body(C2<F>* __restrict__ W, C2<F>* __restrict__ U, const int2*
         const int p = t + m * S;
         const int pp = p == 0 ? 0 : L - p;
         C2<F> uc, um;
-        xupd_pair<F>(Nc[p], Nm[pp], a[m], b[m], p, iyc, izc, iym, izm, wph[p], ar, draw, &uc, &um);
+        xupd_pair<F, NOISE>(Nc[p], Nm[pp], a[m], b[m], p, iyc, izc, iym, izm, wph[p], ar, draw, &uc, &um);
         a[m] = uc;
         b[m] = um;
     }

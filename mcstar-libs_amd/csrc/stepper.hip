@@ -44,15 +44,15 @@ __global__ void __launch_bounds__(MCS_REG_MAX_THREADS(P, F)) k_col_nl(C2<F>* W, 
     if (d_step && blockIdx.x == 0 && threadIdx.x == 0) *d_step += 1;
 }
 
-template <typename F, int P, int SPLIT, bool WAVE>
+template <typename F, int P, int SPLIT, bool WAVE, bool NOISE>
 __global__ void __launch_bounds__(256) k_xupd(C2<F>* __restrict__ W, C2<F>* __restrict__ U, const int2* __restrict__ pairs,
                                               int npairs, int G, XArgs<F> ar, FftPlanDev pl,
                                               const C2<F>* __restrict__ tw, const C2<F>* __restrict__ wph,
                                               const int64_t* __restrict__ d_step) {
     extern __shared__ unsigned char smem_raw[];
-    const uint32_t draw = ar.noise ? (uint32_t)*d_step : 0u;
-    xupd_body<F, P, SPLIT, WAVE>(W, U, pairs, npairs, G, ar, pl, tw, wph, draw,
-                                 reinterpret_cast<C2<F>*>(smem_raw), blockIdx.x, threadIdx.x);
+    const uint32_t draw = NOISE && ar.noise ? (uint32_t)*d_step : 0u;
+    xupd_body<F, P, SPLIT, WAVE, RunPlan, NOISE>(W, U, pairs, npairs, G, ar, pl, tw, wph, draw,
+                                                 reinterpret_cast<C2<F>*>(smem_raw), blockIdx.x, threadIdx.x);
 }
 
 // ------------------------------------------------------------------ helpers
@@ -175,7 +175,9 @@ static int launch_xupd_t(const StepGeom& sg, void* W, void* U, const XArgs<F>& a
     const size_t esz = sizeof(C2<F>);
     const int G = xupd_pairs_per_wg(lp);
     const size_t lds = (size_t)G * 2 * lp.L * esz;
-    auto fn = (64 % S == 0) ? k_xupd<F, P, SPLIT, true> : k_xupd<F, P, SPLIT, false>;
+    // the noise-free kernels carry no noise draw
+    auto fn = ar.noise ? ((64 % S == 0) ? k_xupd<F, P, SPLIT, true, true> : k_xupd<F, P, SPLIT, false, true>)
+                       : ((64 % S == 0) ? k_xupd<F, P, SPLIT, true, false> : k_xupd<F, P, SPLIT, false, false>);
     int st = set_lds_attr((const void*)fn, lds);
     if (st) return st;
     hipLaunchKernelGGL(fn, dim3((unsigned)((sg.npairs + G - 1) / G)), dim3(G * S), lds, s, (C2<F>*)W, (C2<F>*)U,

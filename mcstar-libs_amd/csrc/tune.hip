@@ -9,8 +9,9 @@
 // of one shape share plans, and so do ensemble groups of one shape, which tune
 // on the runtime-plan kernels their work-table kernels run.  A stepper and an
 // ensemble replica agree bit for bit when they run the same plan (the default
-// plans with tuning off, or an explicit plan).  Tuning runs with the noise off; the pass kernels
-// overwrite the buffers, so it runs before a field is loaded.
+// plans with tuning off, or an explicit plan).  Noisy and noise-free steps tune
+// separately; the pass kernels overwrite the buffers, so tuning runs before a
+// field is loaded.
 #include "fixed.hpp"
 #include "stepper.hpp"
 
@@ -150,7 +151,10 @@ extern "C" int mcs_get_autotune(int* on) {
 }
 
 int step_geom_tune(StepGeom* sg, const StepPhys& ph_in, void* W, void* U, hipStream_t s, bool ensemble) {
-    const auto key = std::make_tuple(sg->nx, sg->ny, sg->nz, sg->prec, ensemble ? 1 : 0);
+    // noisy and noise-free steps run different x-pass kernels (the draw costs registers), so
+    // each tunes on its own
+    const int noisy = ph_in.temp > 0 ? 1 : 0;
+    const auto key = std::make_tuple(sg->nx, sg->ny, sg->nz, sg->prec, (ensemble ? 2 : 0) + noisy);
     {
         std::lock_guard<std::mutex> l(g_mu);
         if (!g_enabled) return MCS_OK;
@@ -158,7 +162,6 @@ int step_geom_tune(StepGeom* sg, const StepPhys& ph_in, void* W, void* U, hipStr
         if (it != g_cache.end()) return apply(sg, it->second);
     }
     StepPhys ph = ph_in;
-    ph.temp = 0;  // noise off: the draw counter is not read
     ph.mass = 0;
     const int prec = sg->prec;
     const int maxX = prec == 64 ? 8 : 16, maxC = prec == 64 ? 16 : 32;
