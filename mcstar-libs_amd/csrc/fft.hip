@@ -8,6 +8,7 @@
 // an even nx packs the real row as nx/2 complex values and splits the
 // spectrum with one extra twiddle, so it costs one FFT of half length.
 #include "fft_kernels.hpp"
+#include "fixed.hpp"
 
 #include <vector>
 
@@ -287,6 +288,9 @@ static int launch_col_t(const LinePlan& lp, const void* src, void* dst, const Co
 }
 
 int launch_col(const LinePlan& lp, const void* src, void* dst, const ColGeom& g, int dir, hipStream_t s) {
+    bool done = false;
+    const int st = fixed_col(lp, src, dst, g, dir, s, &done);
+    if (st || done) return st;
     if (lp.prec == 64) return dir < 0 ? launch_col_t<double, -1>(lp, src, dst, g, s) : launch_col_t<double, +1>(lp, src, dst, g, s);
     return dir < 0 ? launch_col_t<float, -1>(lp, src, dst, g, s) : launch_col_t<float, +1>(lp, src, dst, g, s);
 }

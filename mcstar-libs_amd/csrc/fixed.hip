@@ -52,20 +52,23 @@ MCS_DEV void fx_col_store(const C2<F> (&a)[P], char* wb, long mst, unsigned off,
     }
 }
 
+// a column pass (stepper y passes, state import/export, boundary FFT); src and dst may alias
+// (a workgroup reads its whole tile before it writes).  With swz each XCD takes a contiguous
+// run of tiles.
 template <typename F, int P, int DIR, typename FP>
-__global__ void __launch_bounds__(MCS_REG_MAX_THREADS(P, F)) k_col_fx(C2<F>* W, ColGeom g, int G,
-                                                                     const C2<F>* __restrict__ tw) {
+__global__ void __launch_bounds__(MCS_REG_MAX_THREADS(P, F)) k_col_fx(const C2<F>* src, C2<F>* dst, ColGeom g,
+                                                                     int G, const C2<F>* __restrict__ tw, int swz) {
     extern __shared__ unsigned char smem_raw[];
     C2<F>* lds = reinterpret_cast<C2<F>*>(smem_raw);
     constexpr int S = FP::L / P;
     const int tilesx = (g.nxc + G - 1) / G;
-    // each XCD takes a contiguous run of tiles (measured: y passes of 512^3 FP32 207 -> 191 us;
-    // the z pass runs faster in plain order)
-    const int blk = (int)xcd_tile(blockIdx.x, gridDim.x, true);
+    const int blk = (int)xcd_tile(blockIdx.x, gridDim.x, swz != 0);
     const int tile = blk % tilesx, outer = blk / tilesx;
     const int c = threadIdx.x % G, t = threadIdx.x / G;
     const bool valid = tile * G + c < g.nxc;
-    char* wb = reinterpret_cast<char*>(W + (long)outer * g.os + (long)tile * G);
+    const long tb = (long)outer * g.os + (long)tile * G;
+    const char* sb = reinterpret_cast<const char*>(src + tb);
+    char* db = reinterpret_cast<char*>(dst + tb);
     const long mst = (long)S * g.ls * (long)sizeof(C2<F>);
     const unsigned off = (unsigned)((c + (long)t * g.ls) * (long)sizeof(C2<F>));
     // the twiddle table in LDS behind the exchange area: a pass reads its twiddles in the
@@ -75,9 +78,9 @@ __global__ void __launch_bounds__(MCS_REG_MAX_THREADS(P, F)) k_col_fx(C2<F>* W, 
     for (int i = threadIdx.x; i < NTW; i += blockDim.x) twl[i] = tw[i];
     __syncthreads();
     C2<F> a[P];
-    fx_col_load<F, P, FP>(a, wb, mst, off, valid);
+    fx_col_load<F, P, FP>(a, sb, mst, off, valid);
     fix_fft<F, P, DIR, false>(a, twl, t, lds, c, G, FP{});
-    fx_col_store<F, P, FP>(a, wb, mst, off, valid);
+    fx_col_store<F, P, FP>(a, db, mst, off, valid);
 }
 
 // the z pass of the step: inverse transform, N = psi^3 (or psi^3 - psi), forward transform
@@ -224,32 +227,60 @@ static int set_lds_fx(const void* fn, size_t bytes) {
 }
 
 // ------------------------------------------------------------------ launchers
-int fixed_step_col(const LinePlan& lp, void* W, const ColGeom& g, int kind, double scale, int split,
-                   int64_t* d_step, hipStream_t s, bool* done) {
-    *done = false;
-    if (!(g_fixed.load() & MCS_FIXED_COLUMNS) || lp.nofx) return MCS_OK;
+static const ColEntry* col_match(const LinePlan& lp, const ColGeom& g) {
+    if (!(g_fixed.load() & MCS_FIXED_COLUMNS) || lp.nofx) return nullptr;
     int n = 0;
     const ColEntry* t = col_table(&n);
     const ColEntry* e = nullptr;
     for (int i = 0; i < n && !e; ++i)
         if (plan_matches(t[i], lp)) e = &t[i];
+    if (!e) return nullptr;
+    const int S = lp.L / lp.P;
+    const long esz = lp.prec == 64 ? 16 : 8;
+    // per-thread byte offsets (column + t * ls) stay below 2^32
+    if (((long)S * g.ls + col_lines_per_wg(lp)) * esz >= (1L << 32)) return nullptr;
+    return e;
+}
+
+int fixed_col(const LinePlan& lp, const void* src, void* dst, const ColGeom& g, int dir, hipStream_t s, bool* done) {
+    *done = false;
+    const ColEntry* e = col_match(lp, g);
     if (!e) return MCS_OK;
     const int S = lp.L / lp.P;
     const int G = col_lines_per_wg(lp);
     const long esz = lp.prec == 64 ? 16 : 8;
-    // per-thread byte offsets (column + t * ls) stay below 2^32
-    if (((long)S * g.ls + G) * esz >= (1L << 32)) return MCS_OK;
-    const void* fn = kind < 0 ? e->fwd : kind > 0 ? e->inv : (split == MCS_SPLIT_REFERENCE ? e->nl_ref : e->nl_ns);
-    // the exchange area and the twiddle table
-    const size_t lds = ((size_t)G * lp.L + lp.twcount) * esz;
+    // each XCD takes a contiguous run of tiles where neighbouring tiles share cache lines (rows
+    // not a whole number of 128-byte tiles), and along short strides (y: 512^3 FP32 y passes
+    // 207 -> 191 us); the long z stride runs faster in plain order
+    const int swz = (g.nxc % G != 0 || (g.nxc * esz) % 128 != 0 || g.ls * esz <= 65536) ? 1 : 0;
+    const void* fn = dir < 0 ? e->fwd : e->inv;
+    const size_t lds = ((size_t)G * lp.L + lp.twcount) * esz;  // the exchange area and the twiddle table
+    int st = set_lds_fx(fn, lds);
+    if (st) return st;
+    const long tiles = (long)((g.nxc + G - 1) / G) * g.nouter;
+    const void* tw = lp.tw;
+    void* args[] = {(void*)&src, &dst, (void*)&g, (void*)&G, (void*)&tw, (void*)&swz};
+    MCS_HIP_TRY(hipLaunchKernel(fn, dim3((unsigned)tiles), dim3(G * S), args, lds, s));
+    *done = true;
+    return MCS_OK;
+}
+
+int fixed_step_col(const LinePlan& lp, void* W, const ColGeom& g, int kind, double scale, int split,
+                   int64_t* d_step, hipStream_t s, bool* done) {
+    if (kind != 0) return fixed_col(lp, W, W, g, kind, s, done);
+    *done = false;
+    const ColEntry* e = col_match(lp, g);
+    if (!e) return MCS_OK;
+    const int S = lp.L / lp.P;
+    const int G = col_lines_per_wg(lp);
+    const long esz = lp.prec == 64 ? 16 : 8;
+    const void* fn = split == MCS_SPLIT_REFERENCE ? e->nl_ref : e->nl_ns;
+    const size_t lds = ((size_t)G * lp.L + lp.twcount) * esz;  // the exchange area and the twiddle table
     int st = set_lds_fx(fn, lds);
     if (st) return st;
     const long tiles = (long)((g.nxc + G - 1) / G) * g.nouter;
     void* tw = lp.tw;
-    if (kind != 0) {
-        void* args[] = {&W, (void*)&g, (void*)&G, &tw};
-        MCS_HIP_TRY(hipLaunchKernel(fn, dim3((unsigned)tiles), dim3(G * S), args, lds, s));
-    } else if (lp.prec == 64) {
+    if (lp.prec == 64) {
         double sc = scale;
         void* args[] = {&W, (void*)&g, (void*)&G, &tw, &sc, &d_step};
         MCS_HIP_TRY(hipLaunchKernel(fn, dim3((unsigned)tiles), dim3(G * S), args, lds, s));

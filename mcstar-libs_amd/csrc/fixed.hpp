@@ -8,6 +8,8 @@
 // (when false the caller runs the runtime-plan kernel).
 int fixed_step_col(const LinePlan& lp, void* W, const ColGeom& g, int kind, double scale, int split,
                    int64_t* d_step, hipStream_t s, bool* done);
+// a column pass src -> dst (may alias) through a fixed-plan kernel when one matches lp
+int fixed_col(const LinePlan& lp, const void* src, void* dst, const ColGeom& g, int dir, hipStream_t s, bool* done);
 // the x pass with the update, likewise
 int fixed_xupd(const StepGeom& sg, const StepPhys& ph, void* W, void* U, hipStream_t s, bool* done);
 // whether a fixed-plan kernel exists for lp (x pass or column passes) and its bit is set
