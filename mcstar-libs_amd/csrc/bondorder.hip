@@ -603,6 +603,8 @@ int knn_launch(const int64_t* d_idx, const int* d_ijk, const int64_t* d_count, i
     MCS_CHECK(cap >= 0 && cap <= (int64_t)0x7fffff00, MCS_ERR_INVALID, "too many points for bond order");
     MCS_CHECK((d_idx != nullptr) != (d_ijk != nullptr), MCS_ERR_INVALID, "pass either indices or coordinates");
     MCS_CHECK(!(d_h4 || d_h6) || (d_h4 && d_h6 && nbins > 0), MCS_ERR_INVALID, "bad histogram args");
+    const bool full = d_nbr != nullptr;
+    MCS_CHECK(!full || d_out4, MCS_ERR_INVALID, "neighbour output needs the q output");
     if (cap == 0) return MCS_OK;
     const KnnWs w = knn_carve(ws, cap);
     const Wig4 W = wig4();
@@ -615,8 +617,6 @@ int knn_launch(const int64_t* d_idx, const int* d_ijk, const int64_t* d_count, i
     hipLaunchKernelGGL(k_knn_scan, dim3(1), dim3(1024), 0, s, w.grid, w.cnt, w.start);
     hipLaunchKernelGGL(k_knn_scatter, dim3(gp), dim3(256), 0, s, w.grid, w.pts, w.start, w.cnt, w.sorted);
     const unsigned gq = (unsigned)((cap + 127) / 128);
-    const bool full = d_nbr != nullptr;
-    MCS_CHECK(!full || d_out4, MCS_ERR_INVALID, "neighbour output needs the q output");
     if (k <= 8) knn_query_launch<8>(full, gq, s, w, k, h, W, d_nbr, d_out4, d_h4, d_h6, nbins);
     else if (k <= 12) knn_query_launch<12>(full, gq, s, w, k, h, W, d_nbr, d_out4, d_h4, d_h6, nbins);
     else if (k <= 16) knn_query_launch<16>(full, gq, s, w, k, h, W, d_nbr, d_out4, d_h4, d_h6, nbins);
