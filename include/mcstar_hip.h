@@ -207,14 +207,15 @@ int mcs_stepper_pass_times(mcs_stepper* st, int nsteps, double* ms, int cap, int
  * stepper and an ensemble replica of one shape agree bit for bit). */
 int mcs_set_autotune(int on);
 int mcs_get_autotune(int* on);
-/* Fixed-plan kernels of the fused step (no reference counterpart).  For the
- * passes whose bit is set (both by default), a pass whose plan is one the
- * library holds compiled with its whole radix sequence as constants (axis
- * lengths 128, 256, 512; FP32 and FP64) runs that kernel; it computes the same
- * operations in the same order as the runtime-plan kernel, so results are
- * identical either way.  A cleared bit runs the runtime-plan kernels. */
-#define MCS_FIXED_COLUMNS 1 /* the y and z column passes */
-#define MCS_FIXED_XPASS 2   /* the x pass with the update */
+/* Fixed-plan kernels of the fused step and the boundary FFT (no reference
+ * counterpart).  For the passes whose bit is set (both by default), a pass whose
+ * plan is one the library holds compiled with its whole radix sequence as
+ * constants (axis lengths 128, 256, 512; FP32 and FP64) runs that kernel; it
+ * computes the same operations in the same order as the runtime-plan kernel, so
+ * results are identical either way.  A cleared bit runs the runtime-plan
+ * kernels. */
+#define MCS_FIXED_COLUMNS 1 /* column passes: the step's y and z, the boundary FFT's y and z, state import/export */
+#define MCS_FIXED_XPASS 2   /* x passes: the step's x pass with the update, the boundary FFT's row passes */
 int mcs_set_fixed_kernels(int mask);
 int mcs_get_fixed_kernels(int* mask);
 /* Exact checkpoint of the stepper (no reference counterpart; the simulators saved psi

@@ -9,6 +9,7 @@
 // spectrum with one extra twiddle, so it costs one FFT of half length.
 #include "fft_kernels.hpp"
 #include "fixed.hpp"
+#include "row_kernels.hpp"
 
 #include <vector>
 
@@ -160,75 +161,6 @@ __global__ void __launch_bounds__(256) k_row_x(const void* srcv, void* dstv, lon
     }
 }
 
-// x pass of the boundary transform for even nx on the register engine: G rows of
-// L = nx/2 packed points per workgroup, S = L/P threads per row.
-//   R2C: packed FFT in registers, then X_k = E_k + w^k O_k with the mirror Z_{L-k}
-//        read back through LDS; k = L (Nyquist) comes from k = 0 with w^L = -1
-//   C2R: Z_k = (X_k + X*_{L-k}) + i (X_k - X*_{L-k}) w^-k straight from global
-//        memory (the mirror element lies in the same row), inverse FFT in registers
-template <typename F, int P, bool WAVE>
-__global__ void __launch_bounds__(256) k_row_r2c_reg(const C2<F>* __restrict__ src, C2<F>* __restrict__ dst,
-                                                     long nrows, int G, FftPlanDev pl,
-                                                     const C2<F>* __restrict__ tw, const C2<F>* __restrict__ wph) {
-    extern __shared__ unsigned char smem_raw[];
-    C2<F>* lds = reinterpret_cast<C2<F>*>(smem_raw);
-    const int L = pl.L, S = L / P;
-    const int t = threadIdx.x % S, g = threadIdx.x / S;
-    const long row = (long)blockIdx.x * G + g;
-    const bool valid = row < nrows && g < G;
-    const int base = (g < G ? g : 0) * L;
-    C2<F> a[P];
-#pragma unroll
-    for (int m = 0; m < P; ++m) a[m] = valid ? src[row * L + t + m * S] : mk<F>(0, 0);
-    reg_fft<F, P, -1, WAVE>(a, pl, tw, t, lds, base, 1);
-    lds_sync<WAVE>();
-#pragma unroll
-    for (int m = 0; m < P; ++m) lds[base + t + m * S] = a[m];
-    lds_sync<WAVE>();
-    if (!valid) return;
-    C2<F>* out = dst + row * (L + 1);
-#pragma unroll
-    for (int m = 0; m < P; ++m) {
-        const int k = t + m * S;
-        const C2<F> zk = a[m];
-        const C2<F> zp = cconj(lds[base + (k == 0 ? 0 : L - k)]);
-        const C2<F> e = cscale(cadd(zk, zp), F(0.5));
-        const C2<F> o = cscale(cmuli<-1>(csub(zk, zp)), F(0.5));
-        out[k] = cadd(e, cmul(wph[k], o));
-        if (k == 0) out[L] = csub(e, o);
-    }
-}
-
-template <typename F, int P, bool WAVE>
-__global__ void __launch_bounds__(256) k_row_c2r_reg(const C2<F>* __restrict__ src, C2<F>* __restrict__ dst,
-                                                     long nrows, int G, FftPlanDev pl,
-                                                     const C2<F>* __restrict__ tw, const C2<F>* __restrict__ wph) {
-    extern __shared__ unsigned char smem_raw[];
-    C2<F>* lds = reinterpret_cast<C2<F>*>(smem_raw);
-    const int L = pl.L, S = L / P;
-    const int t = threadIdx.x % S, g = threadIdx.x / S;
-    const long row = (long)blockIdx.x * G + g;
-    const bool valid = row < nrows && g < G;
-    const int base = (g < G ? g : 0) * L;
-    const C2<F>* in = src + (valid ? row : 0) * (L + 1);
-    C2<F> a[P];
-#pragma unroll
-    for (int m = 0; m < P; ++m) {
-        const int k = t + m * S;
-        C2<F> xk = valid ? in[k] : mk<F>(0, 0);
-        C2<F> xp = valid ? in[L - k] : mk<F>(0, 0);
-        if (k == 0) { xk.y = F(0); xp.y = F(0); }
-        const C2<F> sm = cadd(xk, cconj(xp));
-        const C2<F> d = cmulc(csub(xk, cconj(xp)), wph[k]);
-        a[m] = cadd(sm, cmuli<+1>(d));
-    }
-    reg_fft<F, P, +1, WAVE>(a, pl, tw, t, lds, base, 1);
-    if (valid) {
-#pragma unroll
-        for (int m = 0; m < P; ++m) dst[row * L + t + m * S] = a[m];
-    }
-}
-
 // ------------------------------------------------------------------ launchers
 static int set_lds(const void* fn, size_t bytes) {
     if (bytes > 65536) MCS_HIP_TRY(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
@@ -320,6 +252,9 @@ static int launch_row_reg_t(int mode, const LinePlan& lp, const void* wph, const
     const size_t esz = sizeof(C2<F>);
     while (G > 1 && (size_t)G * lp.L * esz > 65536) --G;
     const size_t lds = (size_t)G * lp.L * esz;
+    bool done = false;
+    const int fst = fixed_row(mode, lp, wph, src, dst, nrows, G, lds, s, &done);
+    if (fst || done) return fst;
     const bool wave = 64 % S == 0;
     auto fn = mode == 0 ? (wave ? k_row_r2c_reg<F, P, true> : k_row_r2c_reg<F, P, false>)
                         : (wave ? k_row_c2r_reg<F, P, true> : k_row_c2r_reg<F, P, false>);

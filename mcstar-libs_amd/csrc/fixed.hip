@@ -1,7 +1,7 @@
-// Fixed-plan kernels of the single-grid fused step for the shapes that carry
-// the headline workloads: axis lengths 128, 256 and 512 in FP32 and FP64, and
-// the lengths of the 3D ensemble's sizes (32 to 192, with 12- and 20-point
-// plans for 3*2^a and 5*2^a) in FP32.  The ensemble's persistent work-table
+// Fixed-plan kernels of the single-grid fused step and of the boundary FFT for
+// the shapes that carry the headline workloads: axis lengths 128, 256 and 512
+// in FP32 and FP64, and the lengths of the 3D ensemble's sizes (32 to 192, with
+// 12- and 20-point plans for 3*2^a and 5*2^a) in FP32.  The ensemble's persistent work-table
 // kernels keep the runtime plans: with the fixed bodies (up to 428 VGPRs in
 // the 20-point x pass) the 3D ensemble step was slower (0.69 against 0.61 ms).
 //
@@ -18,6 +18,7 @@
 // every pass whose bit mcs_set_fixed_kernels has cleared, and every plan for
 // which the tuner timed the runtime-plan kernel faster (LinePlan::nofx).
 #include "fixed.hpp"
+#include "row_kernels.hpp"
 
 #include <atomic>
 
@@ -324,6 +325,63 @@ int fixed_xupd(const StepGeom& sg, const StepPhys& ph, void* W, void* U, hipStre
             *done = true;
             return sg.prec == 64 ? fixed_xupd_t<double>(t[i], sg, ph, W, U, s) : fixed_xupd_t<float>(t[i], sg, ph, W, U, s);
         }
+    return MCS_OK;
+}
+
+// ------------------------------------------------------------------ boundary row passes
+// The x (row) passes of the boundary transform, R2C and C2R, for the register plans the
+// boundary FFT builds for nx/2 = 128, 256 and 512 (the MCS_FIXED_XPASS bit).
+struct RowEntry {
+    int prec, P, L, np;
+    int rad[4];
+    const void* r2c;
+    const void* c2r;
+};
+
+template <typename F, int P, int L, int... Rs> static RowEntry row_entry() {
+    using FP = FixPlan<L, Rs...>;
+    constexpr bool wave = 64 % (L / P) == 0;
+    RowEntry e{};
+    e.prec = sizeof(F) == 8 ? 64 : 32;
+    e.P = P;
+    e.L = L;
+    const int r[] = {Rs...};
+    e.np = (int)sizeof...(Rs);
+    for (int i = 0; i < e.np; ++i) e.rad[i] = r[i];
+    e.r2c = (const void*)k_row_r2c_reg<F, P, wave, FP>;
+    e.c2r = (const void*)k_row_c2r_reg<F, P, wave, FP>;
+    return e;
+}
+
+static const RowEntry* row_table(int* n) {
+    static const RowEntry t[] = {
+        row_entry<float, 16, 128, 8, 16>(),   row_entry<float, 16, 256, 16, 16>(),
+        row_entry<float, 32, 512, 16, 32>(),  row_entry<double, 8, 128, 2, 8, 8>(),
+        row_entry<double, 8, 256, 4, 8, 8>(), row_entry<double, 8, 512, 8, 8, 8>(),
+    };
+    *n = (int)(sizeof(t) / sizeof(t[0]));
+    return t;
+}
+
+int fixed_row(int mode, const LinePlan& lp, const void* wph, const void* src, void* dst, long nrows, int G,
+              size_t lds, hipStream_t s, bool* done) {
+    *done = false;
+    if (!(g_fixed.load() & MCS_FIXED_XPASS) || lp.nofx) return MCS_OK;
+    int n = 0;
+    const RowEntry* t = row_table(&n);
+    const RowEntry* e = nullptr;
+    for (int i = 0; i < n && !e; ++i)
+        if (plan_matches(t[i], lp)) e = &t[i];
+    if (!e) return MCS_OK;
+    const void* fn = mode == 0 ? e->r2c : e->c2r;
+    int st = set_lds_fx(fn, lds);
+    if (st) return st;
+    FftPlanDev pl = lp.dev;
+    const void* tw = lp.tw;
+    int g = G;
+    void* args[] = {(void*)&src, &dst, &nrows, &g, &pl, (void*)&tw, (void*)&wph};
+    MCS_HIP_TRY(hipLaunchKernel(fn, dim3((unsigned)((nrows + G - 1) / G)), dim3(G * (lp.L / lp.P)), args, lds, s));
+    *done = true;
     return MCS_OK;
 }
 

@@ -10,6 +10,10 @@ int fixed_step_col(const LinePlan& lp, void* W, const ColGeom& g, int kind, doub
                    int64_t* d_step, hipStream_t s, bool* done);
 // a column pass src -> dst (may alias) through a fixed-plan kernel when one matches lp
 int fixed_col(const LinePlan& lp, const void* src, void* dst, const ColGeom& g, int dir, hipStream_t s, bool* done);
+// a row pass of the boundary transform (mode 0 R2C, else C2R) with G rows per workgroup and lds
+// bytes of LDS, likewise
+int fixed_row(int mode, const LinePlan& lp, const void* wph, const void* src, void* dst, long nrows, int G,
+              size_t lds, hipStream_t s, bool* done);
 // the x pass with the update, likewise
 int fixed_xupd(const StepGeom& sg, const StepPhys& ph, void* W, void* U, hipStream_t s, bool* done);
 // whether a fixed-plan kernel exists for lp (x pass or column passes) and its bit is set
