@@ -219,6 +219,8 @@ def main():
     ap.add_argument("--no-noise-config", action="store_true", help="skip the 256^3 noise-on line")
     ap.add_argument("--only-ensemble", action="store_true", help="run only the ensemble lines (profiling)")
     ap.add_argument("--only-fft", action="store_true", help="run only the boundary FFT lines (profiling)")
+    ap.add_argument("--no-fft", action="store_true",
+                    help="skip the boundary FFT line (profiling the step alone: its column passes share kernels)")
     ap.add_argument("--plan", default=None,
                     help="explicit FP32 plans Px,Gx,Py,Gy,Pz,Gz instead of the tuned ones (profiling without "
                          "the tuner's trial launches)")
@@ -235,17 +237,17 @@ def main():
     mcs = load_pkg()
     if a.fixed is not None:
         mcs.set_fixed_kernels(a.fixed)
+    if a.copy_calibration:
+        nb = 1 << 30
+        src, dst = mcs.DeviceBuffer(nb).zero(), mcs.DeviceBuffer(nb)
+        mcs.check(mcs.lib().mcs_memcpy(dst.ptr, src.ptr, nb, mcs.D2D), "mcs_memcpy")
+        del src, dst
     if a.only_ensemble:
         print(json.dumps({"ensemble_2d": ensemble_bench(mcs, "2d", 40), "ensemble_3d": ensemble_bench(mcs, "3d", 16)}))
         return
     if a.only_fft:
         print(json.dumps({"boundary_fft_512": fft_bench(mcs, a.n, 5)}))
         return
-    if a.copy_calibration:
-        nb = 1 << 30
-        src, dst = mcs.DeviceBuffer(nb).zero(), mcs.DeviceBuffer(nb)
-        mcs.check(mcs.lib().mcs_memcpy(dst.ptr, src.ptr, nb, mcs.D2D), "mcs_memcpy")
-        del src, dst
     n = a.n
     cfg = dict(eps=0.3, psi0=-0.25, amp=0.05, seed=11, dx=np.pi / 4, dt=0.5)
     N = n ** 3
@@ -316,7 +318,8 @@ def main():
         log("[bench] 256^3 with noise")
         out["noise_256"] = noise_config_bench(mcs, 100)
     log("[bench] boundary FFT")
-    out["boundary_fft"] = fft_bench(mcs, n, 5)
+    if not a.no_fft:
+        out["boundary_fft"] = fft_bench(mcs, n, 5)
     if not a.no_ensemble:
         log("[bench] ensembles")
         out["ensemble_2d"] = ensemble_bench(mcs, "2d", 40)

@@ -23,7 +23,7 @@ __global__ void __launch_bounds__(MCS_REG_MAX_THREADS(P, F)) k_col_reg(const C2<
     C2<F>* lds = reinterpret_cast<C2<F>*>(smem_raw);
     const int S = pl.L / P;
     const int tilesx = (g.nxc + G - 1) / G;
-    const int blk = (int)xcd_tile(blockIdx.x, gridDim.x, swz != 0);
+    const int blk = (int)xcd_tile(blockIdx.x, gridDim.x, swz, tilesx, g.nouter);
     const int tile = blk % tilesx, outer = blk / tilesx;
     const int c = threadIdx.x % G, t = threadIdx.x / G;
     const int x = tile * G + c;
@@ -48,7 +48,7 @@ __global__ void __launch_bounds__(256) k_col_gen(const C2<F>* src, C2<F>* dst, C
     C2<F>* b1 = b0 + (long)G * Lp;
     const int L = pl.L;
     const int tilesx = (g.nxc + G - 1) / G;
-    const int blk = (int)xcd_tile(blockIdx.x, gridDim.x, swz != 0);
+    const int blk = (int)xcd_tile(blockIdx.x, gridDim.x, swz, tilesx, g.nouter);
     const int tile = blk % tilesx, outer = blk / tilesx;
     const long base = (long)outer * g.os + (long)tile * G;
     for (int idx = threadIdx.x; idx < G * L; idx += blockDim.x) {
@@ -162,6 +162,15 @@ __global__ void __launch_bounds__(256) k_row_x(const void* srcv, void* dstv, lon
 }
 
 // ------------------------------------------------------------------ launchers
+// Measured on 512^3 FP32: the step's y passes take 207 us in order 0 and 191 us in
+// order 1; the boundary transform's z passes over 257-wide rows cost 0.79 ms per R2C in
+// order 1 and 0.72 ms in order 2; the step's z pass (256-wide rows, no shared lines) is
+// fastest in order 0.
+int col_tile_order(const ColGeom& g, int G, long esz) {
+    if (g.ls * esz <= 65536) return 1;
+    return (g.nxc % G != 0 || (g.nxc * esz) % 128 != 0) ? 2 : 0;
+}
+
 static int set_lds(const void* fn, size_t bytes) {
     if (bytes > 65536) MCS_HIP_TRY(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
     return MCS_OK;
@@ -177,8 +186,7 @@ static int launch_col_reg_t(const LinePlan& lp, const void* src, void* dst, cons
     int st = set_lds((const void*)fn, lds);
     if (st) return st;
     const long tiles = (long)((g.nxc + G - 1) / G) * g.nouter;
-    // swizzle when tiles share cache lines: rows not a whole number of 128-byte tiles
-    const int swz = (g.nxc % G != 0 || (g.nxc * esz) % 128 != 0) ? 1 : 0;
+    const int swz = col_tile_order(g, G, esz);
     hipLaunchKernelGGL(fn, dim3((unsigned)tiles), dim3(G * S), lds, s, (const C2<F>*)src, (C2<F>*)dst, g, G, lp.dev,
                        (const C2<F>*)lp.tw, swz);
     MCS_HIP_TRY(hipGetLastError());
@@ -196,7 +204,7 @@ static int launch_col_gen_t(const LinePlan& lp, const void* src, void* dst, cons
     int st = set_lds((const void*)fn, lds);
     if (st) return st;
     const long tiles = (long)((g.nxc + G - 1) / G) * g.nouter;
-    const int swz = (g.nxc % G != 0 || (g.nxc * esz) % 128 != 0) ? 1 : 0;
+    const int swz = col_tile_order(g, G, esz);
     hipLaunchKernelGGL(fn, dim3((unsigned)tiles), dim3(256), lds, s, (const C2<F>*)src, (C2<F>*)dst, g, G, Lp, lp.dev,
                        (const C2<F>*)lp.tw, swz);
     MCS_HIP_TRY(hipGetLastError());

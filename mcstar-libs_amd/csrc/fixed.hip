@@ -54,8 +54,7 @@ MCS_DEV void fx_col_store(const C2<F> (&a)[P], char* wb, long mst, unsigned off,
 }
 
 // a column pass (stepper y passes, state import/export, boundary FFT); src and dst may alias
-// (a workgroup reads its whole tile before it writes).  With swz each XCD takes a contiguous
-// run of tiles.
+// (a workgroup reads its whole tile before it writes).  swz is the tile order of xcd_tile.
 template <typename F, int P, int DIR, typename FP>
 __global__ void __launch_bounds__(MCS_REG_MAX_THREADS(P, F)) k_col_fx(const C2<F>* src, C2<F>* dst, ColGeom g,
                                                                      int G, const C2<F>* __restrict__ tw, int swz) {
@@ -63,7 +62,7 @@ __global__ void __launch_bounds__(MCS_REG_MAX_THREADS(P, F)) k_col_fx(const C2<F
     C2<F>* lds = reinterpret_cast<C2<F>*>(smem_raw);
     constexpr int S = FP::L / P;
     const int tilesx = (g.nxc + G - 1) / G;
-    const int blk = (int)xcd_tile(blockIdx.x, gridDim.x, swz != 0);
+    const int blk = (int)xcd_tile(blockIdx.x, gridDim.x, swz, tilesx, g.nouter);
     const int tile = blk % tilesx, outer = blk / tilesx;
     const int c = threadIdx.x % G, t = threadIdx.x / G;
     const bool valid = tile * G + c < g.nxc;
@@ -250,10 +249,7 @@ int fixed_col(const LinePlan& lp, const void* src, void* dst, const ColGeom& g, 
     const int S = lp.L / lp.P;
     const int G = col_lines_per_wg(lp);
     const long esz = lp.prec == 64 ? 16 : 8;
-    // each XCD takes a contiguous run of tiles where neighbouring tiles share cache lines (rows
-    // not a whole number of 128-byte tiles), and along short strides (y: 512^3 FP32 y passes
-    // 207 -> 191 us); the long z stride runs faster in plain order
-    const int swz = (g.nxc % G != 0 || (g.nxc * esz) % 128 != 0 || g.ls * esz <= 65536) ? 1 : 0;
+    const int swz = col_tile_order(g, G, esz);
     const void* fn = dir < 0 ? e->fwd : e->inv;
     const size_t lds = ((size_t)G * lp.L + lp.twcount) * esz;  // the exchange area and the twiddle table
     int st = set_lds_fx(fn, lds);
