@@ -8,8 +8,8 @@
 // L = nx/2 packed points per workgroup, S = L/P threads per row.
 //   R2C: packed FFT in registers, then X_k = E_k + w^k O_k with the mirror Z_{L-k}
 //        read back through LDS; k = L (Nyquist) comes from k = 0 with w^L = -1
-//   C2R: Z_k = (X_k + X*_{L-k}) + i (X_k - X*_{L-k}) w^-k straight from global
-//        memory (the mirror element lies in the same row), inverse FFT in registers
+//   C2R: Z_k = (X_k + X*_{L-k}) + i (X_k - X*_{L-k}) w^-k with the mirror element
+//        read back through LDS, inverse FFT in registers
 template <typename F, int P, bool WAVE, typename FP = RunPlan>
 __global__ void __launch_bounds__(256) k_row_r2c_reg(const C2<F>* __restrict__ src, C2<F>* __restrict__ dst,
                                                      long nrows, int G, FftPlanDev pl,
@@ -55,17 +55,26 @@ __global__ void __launch_bounds__(256) k_row_c2r_reg(const C2<F>* __restrict__ s
     const bool valid = row < nrows && g < G;
     const int base = (g < G ? g : 0) * L;
     const C2<F>* in = src + (valid ? row : 0) * (L + 1);
+    // the row is read once; the mirror X_{L-k} comes back through LDS (X_L, the
+    // mirror of k = 0, is read by the row's first thread)
     C2<F> a[P];
+#pragma unroll
+    for (int m = 0; m < P; ++m) a[m] = valid ? in[t + m * S] : mk<F>(0, 0);
+    const C2<F> xnyq = valid && t == 0 ? in[L] : mk<F>(0, 0);
+#pragma unroll
+    for (int m = 0; m < P; ++m) lds[base + t + m * S] = a[m];
+    lds_sync<WAVE>();
 #pragma unroll
     for (int m = 0; m < P; ++m) {
         const int k = t + m * S;
-        C2<F> xk = valid ? in[k] : mk<F>(0, 0);
-        C2<F> xp = valid ? in[L - k] : mk<F>(0, 0);
+        C2<F> xk = a[m];
+        C2<F> xp = k == 0 ? xnyq : lds[base + L - k];
         if (k == 0) { xk.y = F(0); xp.y = F(0); }
         const C2<F> sm = cadd(xk, cconj(xp));
         const C2<F> d = cmulc(csub(xk, cconj(xp)), wph[k]);
         a[m] = cadd(sm, cmuli<+1>(d));
     }
+    lds_sync<WAVE>();  // every mirror is read before the transform's exchanges reuse the LDS
     line_fft<F, P, +1, WAVE, FP>(a, pl, tw, t, lds, base, 1);
     if (valid) {
 #pragma unroll
