@@ -78,10 +78,13 @@ def measured_traffic(pass_name, n):
     path = os.path.join(REPO, "profiles", "pfc_step_512cubed_fp32_hbm_traffic.md")
     if not os.path.exists(path):
         return None
+    col = None
     for line in open(path):
         cols = [c.strip() for c in line.split("|")]
-        if len(cols) > 7 and cols[1] == pass_name:
-            return float(cols[5].split()[0]) * 1e9
+        if "HBM bytes (scaled)" in cols:
+            col = cols.index("HBM bytes (scaled)")
+        elif col is not None and len(cols) > col and cols[1] == pass_name:
+            return float(cols[col].split()[0]) * 1e9
     return None
 
 
