@@ -51,7 +51,7 @@ template <typename F, int DIR> MCS_DEV void dft3(C2<F>* v) {
     const F s = F(DIR) * F(0.86602540378443864676);
     const C2<F> t = cadd(v[1], v[2]);
     const C2<F> u = csub(v[1], v[2]);
-    const C2<F> m = mk<F>(mfma(c, t.x, v[0].x), mfma(c, t.y, v[0].y));
+    const C2<F> m = mk<F>(fmad(c, t.x, v[0].x), fmad(c, t.y, v[0].y));
     const C2<F> iu = mk<F>(-s * u.y, s * u.x);  // i*s*u
     v[0] = cadd(v[0], t);
     v[1] = cadd(m, iu);
@@ -74,10 +74,10 @@ template <typename F, int DIR> MCS_DEV void dft5(C2<F>* v) {
     const C2<F> t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]);
     const C2<F> u1 = csub(v[1], v[4]), u2 = csub(v[2], v[3]);
     const C2<F> a0 = v[0];
-    const C2<F> m1 = mk<F>(mfma(c2, t2.x, mfma(c1, t1.x, a0.x)), mfma(c2, t2.y, mfma(c1, t1.y, a0.y)));
-    const C2<F> m2 = mk<F>(mfma(c1, t2.x, mfma(c2, t1.x, a0.x)), mfma(c1, t2.y, mfma(c2, t1.y, a0.y)));
-    const C2<F> n1 = mk<F>(mfma(s1, u1.x, s2 * u2.x), mfma(s1, u1.y, s2 * u2.y));
-    const C2<F> n2 = mk<F>(mfma(s2, u1.x, -(s1 * u2.x)), mfma(s2, u1.y, -(s1 * u2.y)));
+    const C2<F> m1 = mk<F>(fmad(c2, t2.x, fmad(c1, t1.x, a0.x)), fmad(c2, t2.y, fmad(c1, t1.y, a0.y)));
+    const C2<F> m2 = mk<F>(fmad(c1, t2.x, fmad(c2, t1.x, a0.x)), fmad(c1, t2.y, fmad(c2, t1.y, a0.y)));
+    const C2<F> n1 = mk<F>(fmad(s1, u1.x, s2 * u2.x), fmad(s1, u1.y, s2 * u2.y));
+    const C2<F> n2 = mk<F>(fmad(s2, u1.x, -(s1 * u2.x)), fmad(s2, u1.y, -(s1 * u2.y)));
     const C2<F> in1 = mk<F>(-n1.y, n1.x), in2 = mk<F>(-n2.y, n2.x);
     v[0] = mk<F>(a0.x + t1.x + t2.x, a0.y + t1.y + t2.y);
     v[1] = cadd(m1, in1);

@@ -109,7 +109,7 @@ __global__ void __launch_bounds__(MCS_REG_MAX_THREADS(P, F)) k_col_nl_fx(C2<F>* 
 #pragma unroll
     for (int m = 0; m < P; ++m) {
         const F u = a[m].x * scale, v = a[m].y * scale;
-        if (SPLIT == MCS_SPLIT_REFERENCE) a[m] = mk<F>(mfma(u * u, u, -u), mfma(v * v, v, -v));
+        if (SPLIT == MCS_SPLIT_REFERENCE) a[m] = mk<F>(fmad(u * u, u, -u), fmad(v * v, v, -v));
         else a[m] = mk<F>(u * u * u, v * v * v);
     }
     fix_fft<F, P, -1, false>(a, twl, t, lds, c, G, FP{});

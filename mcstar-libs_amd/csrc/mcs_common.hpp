@@ -20,21 +20,21 @@ template <typename F> using C2 = typename cplx_of<F>::type;
 template <typename F> MCS_HD C2<F> mk(F x, F y) { C2<F> r; r.x = x; r.y = y; return r; }
 // fused multiply-add written out: device code is built without implicit contraction, so
 // these are the only fused operations and every kernel that shares an expression rounds it alike
-MCS_HD float mfma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
-MCS_HD double mfma(double a, double b, double c) { return __builtin_fma(a, b, c); }
+MCS_HD float fmad(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+MCS_HD double fmad(double a, double b, double c) { return __builtin_fma(a, b, c); }
 template <typename T> MCS_HD T cadd(T a, T b) { T r; r.x = a.x + b.x; r.y = a.y + b.y; return r; }
 template <typename T> MCS_HD T csub(T a, T b) { T r; r.x = a.x - b.x; r.y = a.y - b.y; return r; }
 template <typename T> MCS_HD T cmul(T a, T b) {
     T r;
-    r.x = mfma(a.x, b.x, -(a.y * b.y));
-    r.y = mfma(a.x, b.y, a.y * b.x);
+    r.x = fmad(a.x, b.x, -(a.y * b.y));
+    r.y = fmad(a.x, b.y, a.y * b.x);
     return r;
 }
 // a * conj(b)
 template <typename T> MCS_HD T cmulc(T a, T b) {
     T r;
-    r.x = mfma(a.x, b.x, a.y * b.y);
-    r.y = mfma(a.y, b.x, -(a.x * b.y));
+    r.x = fmad(a.x, b.x, a.y * b.y);
+    r.y = fmad(a.y, b.x, -(a.x * b.y));
     return r;
 }
 template <typename T> MCS_HD T cconj(T a) { T r; r.x = a.x; r.y = -a.y; return r; }

@@ -59,7 +59,7 @@ MCS_DEV void col_nl_body(C2<F>* W, const ColGeom& g, int G, const FftPlanDev& pl
 #pragma unroll
     for (int m = 0; m < P; ++m) {
         const F u = a[m].x * scale, v = a[m].y * scale;
-        if (SPLIT == MCS_SPLIT_REFERENCE) a[m] = mk<F>(mfma(u * u, u, -u), mfma(v * v, v, -v));
+        if (SPLIT == MCS_SPLIT_REFERENCE) a[m] = mk<F>(fmad(u * u, u, -u), fmad(v * v, v, -v));
         else a[m] = mk<F>(u * u * u, v * v * v);
     }
     line_fft<F, P, -1, false, FP>(a, pl, tw, tt, lds, c, G);
@@ -118,7 +118,7 @@ MCS_DEV void xupd_pair(C2<F> nA, C2<F> nBr, C2<F> pA, C2<F> pBr, int p, int iy, 
     const C2<F> n1 = cadd(eN, woN), n2 = csub(eN, woN);
     const C2<F> p1 = cadd(eP, woP), p2 = csub(eP, woP);
     const F ky = F(ksigned(iy, ar.ny)) * ar.dky, kz = F(ksigned(iz, ar.nz)) * ar.dkz;
-    const F kyz = mfma(ky, ky, kz * kz);
+    const F kyz = fmad(ky, ky, kz * kz);
     const F kx1 = F(p) * ar.dkx;
     const F kx2 = F(ar.L - p) * ar.dkx;  // |k_x| of index p + nx/2 (the Nyquist index for p = 0)
     C2<F> xi1 = mk<F>(0, 0), xi2 = mk<F>(0, 0);
@@ -139,8 +139,8 @@ MCS_DEV void xupd_pair(C2<F> nA, C2<F> nBr, C2<F> pA, C2<F> pBr, int p, int iy, 
             }
         }
     }
-    const C2<F> q1 = pfc_mode_update<F>(p1, n1, xi1, mfma(kx1, kx1, kyz), ar.dt, ar.lin0, ar.amp);
-    const C2<F> q2 = pfc_mode_update<F>(p2, n2, xi2, mfma(kx2, kx2, kyz), ar.dt, ar.lin0, ar.amp);
+    const C2<F> q1 = pfc_mode_update<F>(p1, n1, xi1, fmad(kx1, kx1, kyz), ar.dt, ar.lin0, ar.amp);
+    const C2<F> q2 = pfc_mode_update<F>(p2, n2, xi2, fmad(kx2, kx2, kyz), ar.dt, ar.lin0, ar.amp);
     if (p == 0 && iy == 0 && iz == 0) {
         // k = 0: the mean density is carried in FP64 and re-imposed exactly
         const double m = ar.mass;

@@ -19,7 +19,7 @@ template <typename F>
 MCS_DEV C2<F> pfc_mode_update(C2<F> psi, C2<F> nl, C2<F> xi, F ksq, F dt, F lin0, F amp) {
     const F om = F(1) - ksq;
     const F a = dt * ksq;
-    const F den = mfma(a, mfma(om, om, lin0), F(1));
+    const F den = fmad(a, fmad(om, om, lin0), F(1));
     F inv, na;
     if constexpr (sizeof(F) == 4) {
         inv = __builtin_amdgcn_rcpf(den);
@@ -29,7 +29,7 @@ MCS_DEV C2<F> pfc_mode_update(C2<F> psi, C2<F> nl, C2<F> xi, F ksq, F dt, F lin0
         na = amp != F(0) ? amp * sqrt(ksq) : F(0);
     }
     C2<F> r;
-    r.x = mfma(na, xi.x, mfma(-a, nl.x, psi.x)) * inv;
-    r.y = mfma(na, xi.y, mfma(-a, nl.y, psi.y)) * inv;
+    r.x = fmad(na, xi.x, fmad(-a, nl.x, psi.x)) * inv;
+    r.y = fmad(na, xi.y, fmad(-a, nl.y, psi.y)) * inv;
     return r;
 }
