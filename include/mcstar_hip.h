@@ -201,8 +201,9 @@ int mcs_stepper_pass_times(mcs_stepper* st, int nsteps, double* ms, int cap, int
  * (lines or row pairs per workgroup) of each pass on the new buffers, and for
  * steppers the fixed-plan kernel of each candidate that has one against the
  * runtime-plan kernel, and keep the fastest; choices are cached per
- * (extents, precision) for the process, separately for steppers and for
- * ensemble groups (whose work-table kernels run the runtime plans).  Off:
+ * (extents, precision) for the process, separately for steppers (with and
+ * without noise) and for ensemble groups, which time their own work-table
+ * kernels over all the group's members (cached per member count too).  Off:
  * fixed default plans (reproducible bit for bit across processes, and a
  * stepper and an ensemble replica of one shape agree bit for bit). */
 int mcs_set_autotune(int on);

@@ -205,6 +205,26 @@ static int ens_step_group(mcs_ensemble* e, EnsGroup* g, int first, int nr, int d
     return st;
 }
 
+// plans and tile shapes of a size group, timed on the group's own work-table kernels over all
+// its members (the trial launches overwrite their states, which are loaded afterwards); cached
+// per shape, member count and noise
+template <typename F> static int ens_tune_group(mcs_ensemble* e, EnsGroup* g, hipStream_t s) {
+    const int n = (int)g->members.size();
+    bool noisy = false;
+    for (int r : g->members) noisy = noisy || e->reps[r].temp > 0;
+    const ColGeom gy = step_geom_y(g->sg), gz = step_geom_z(g->sg);
+    const bool is3d = g->sg.is3d;
+    return step_geom_tune_runs(
+        &g->sg, 2 + (noisy ? 1 : 0) + 4 * n, false,
+        [&](const LinePlan& lp) {
+            EnsGroup t = *g;
+            t.sg.px = lp;
+            return ens_xupd<F>(e, &t, 0, n, 0, s);
+        },
+        [&](const LinePlan& lp) { return is3d ? ens_col<F>(e, g, lp, gy, -1, 0, n, s) : ens_col_nl<F>(e, g, lp, gy, 0, n, s); },
+        [&](const LinePlan& lp) { return ens_col_nl<F>(e, g, lp, gz, 0, n, s); }, s);
+}
+
 static int ens_advance(mcs_ensemble* e, int nsteps, hipStream_t s) {
     hipLaunchKernelGGL(k_ens_advance, dim3(1), dim3(64), 0, s, e->d_step, nsteps);
     MCS_HIP_TRY(hipGetLastError());
@@ -420,15 +440,12 @@ extern "C" int mcs_ensemble_create(mcs_ensemble** out, const mcs_replica* reps, 
             rc = MCS_ERR_NOMEM;
         }
     }
-    // plans and tile shapes per size group, timed on the group's first state buffers
-    for (size_t gi = 0; gi < e->groups.size() && !rc; ++gi) {
-        EnsGroup* g = e->groups[gi];
-        const int r0 = g->members[0];
-        rc = step_geom_tune(&g->sg, phys_of(e->reps[r0], split, 0.0), e->W[r0], e->U[r0], mcs_stream(), true);
-    }
     if (!rc && hipMalloc((void**)&e->d_step, sizeof(int64_t)) != hipSuccess) rc = MCS_ERR_NOMEM;
     if (!rc && hipMemset(e->d_step, 0, sizeof(int64_t)) != hipSuccess) rc = MCS_ERR_HIP;
     if (!rc) rc = upload_reps(e);
+    for (size_t gi = 0; gi < e->groups.size() && !rc; ++gi)
+        rc = e->prec == 64 ? ens_tune_group<double>(e, e->groups[gi], mcs_stream())
+                           : ens_tune_group<float>(e, e->groups[gi], mcs_stream());
     if (rc) { mcs_ensemble_destroy(e); return rc; }
     *out = e;
     return MCS_OK;

@@ -4,6 +4,7 @@
 #include "pass_bodies.hpp"
 
 #include <cmath>
+#include <functional>
 
 struct StepGeom {
     int nx = 0, ny = 0, nz = 0, L = 0, prec = 32;
@@ -58,10 +59,17 @@ int launch_step_col(const LinePlan& lp, void* W, const ColGeom& g, int kind, dou
 ColGeom step_geom_y(const StepGeom& sg);
 ColGeom step_geom_z(const StepGeom& sg);
 
-// autotuner: picks the register plan and tile shape of every pass for this shape and
-// precision (cached per process and kernel family: ensemble groups time the runtime-plan
-// kernels only); a no-op when tuning is switched off.  Overwrites W and U.
-int step_geom_tune(StepGeom* sg, const StepPhys& ph, void* W, void* U, hipStream_t s, bool ensemble = false);
+// autotuner of a single-grid stepper: picks the register plan and tile shape of every pass
+// for this shape and precision by timing the stepper's own pass kernels, fixed-plan ones
+// included (cached per process); a no-op when tuning is switched off.  Overwrites W and U.
+int step_geom_tune(StepGeom* sg, const StepPhys& ph, void* W, void* U, hipStream_t s);
+// the autotuner over caller-supplied pass launchers: each runs one pass with a candidate plan
+// (run_y the y pass of a 3D step, or the nonlinear y pass of a 2D one; run_z the z pass);
+// fixed: also time the fixed-plan kernel of a candidate where one exists.  Choices are cached
+// under (extents, precision, tag); the caller's kernels overwrite their buffers.
+using PassRun = std::function<int(const LinePlan&)>;
+int step_geom_tune_runs(StepGeom* sg, int tag, bool fixed, const PassRun& run_x, const PassRun& run_y,
+                        const PassRun& run_z, hipStream_t s);
 // {P_x, pairs per workgroup, P_y, lines per workgroup, P_z, lines per workgroup}
 void step_geom_plan_info(const StepGeom& sg, int info[6]);
 // installs an explicit plan in that layout after checking it against the candidate lists

@@ -5,13 +5,12 @@
 // LDS footprint) of each pass, and keeps the fastest.  Where a fixed-plan
 // kernel exists for a candidate, it is timed against the runtime-plan kernel of
 // the same plan and the faster one is kept.  Choices are cached per
-// (nx, ny, nz, precision, kernel family) for the life of the process: steppers
-// of one shape share plans, and so do ensemble groups of one shape, which tune
-// on the runtime-plan kernels their work-table kernels run.  A stepper and an
-// ensemble replica agree bit for bit when they run the same plan (the default
-// plans with tuning off, or an explicit plan).  Noisy and noise-free steps tune
-// separately; the pass kernels overwrite the buffers, so tuning runs before a
-// field is loaded.
+// (nx, ny, nz, precision, tag) for the life of the process: steppers of one
+// shape share plans (tagged noisy or not), and ensemble groups of one shape and
+// member count share theirs, timed on the group's own work-table kernels.  A
+// stepper and an ensemble replica agree bit for bit when they run the same plan
+// (the default plans with tuning off, or an explicit plan).  The pass kernels
+// overwrite the buffers, so tuning runs before a field is loaded.
 #include "fixed.hpp"
 #include "stepper.hpp"
 
@@ -150,43 +149,21 @@ extern "C" int mcs_get_autotune(int* on) {
     return MCS_OK;
 }
 
-int step_geom_tune(StepGeom* sg, const StepPhys& ph_in, void* W, void* U, hipStream_t s, bool ensemble) {
-    // noisy and noise-free steps run different x-pass kernels (the draw costs registers), so
-    // each tunes on its own
-    const int noisy = ph_in.temp > 0 ? 1 : 0;
-    const auto key = std::make_tuple(sg->nx, sg->ny, sg->nz, sg->prec, (ensemble ? 2 : 0) + noisy);
+int step_geom_tune_runs(StepGeom* sg, int tag, bool fixed, const PassRun& run_x, const PassRun& run_y,
+                        const PassRun& run_z, hipStream_t s) {
+    const auto key = std::make_tuple(sg->nx, sg->ny, sg->nz, sg->prec, tag);
     {
         std::lock_guard<std::mutex> l(g_mu);
         if (!g_enabled) return MCS_OK;
         auto it = g_cache.find(key);
         if (it != g_cache.end()) return apply(sg, it->second);
     }
-    StepPhys ph = ph_in;
-    ph.mass = 0;
     const int prec = sg->prec;
     const int maxX = prec == 64 ? 8 : 16, maxC = prec == 64 ? 16 : 32;
-    const double scale = 2.0 / ((double)sg->nx * sg->ny * sg->nz);
     StepChoice c;
-    const bool wf = !ensemble;  // the ensemble's work-table kernels run the runtime plans
-    int st = tune_pass(sg->L, prec, maxX, true, wf, s, [&](const LinePlan& lp) {
-        StepGeom g = *sg;
-        g.px = lp;
-        return launch_xupd(g, ph, W, U, s);
-    }, &c.x);
-    if (!st && sg->is3d) {
-        const ColGeom gy = step_geom_y(*sg), gz = step_geom_z(*sg);
-        st = tune_pass(sg->ny, prec, maxC, false, wf, s, [&](const LinePlan& lp) {
-            return launch_step_col(lp, W, gy, -1, scale, ph.split, nullptr, s);
-        }, &c.y);
-        if (!st) st = tune_pass(sg->nz, prec, maxC, false, wf, s, [&](const LinePlan& lp) {
-            return launch_col_nl(lp, W, gz, scale, ph.split, nullptr, s);
-        }, &c.z);
-    } else if (!st) {
-        const ColGeom gy = step_geom_y(*sg);
-        st = tune_pass(sg->ny, prec, maxC, false, wf, s, [&](const LinePlan& lp) {
-            return launch_col_nl(lp, W, gy, scale, ph.split, nullptr, s);
-        }, &c.y);
-    }
+    int st = tune_pass(sg->L, prec, maxX, true, fixed, s, run_x, &c.x);
+    if (!st) st = tune_pass(sg->ny, prec, maxC, false, fixed, s, run_y, &c.y);
+    if (!st && sg->is3d) st = tune_pass(sg->nz, prec, maxC, false, fixed, s, run_z, &c.z);
     if (st) return st;
     MCS_HIP_TRY(hipStreamSynchronize(s));
     {
@@ -196,6 +173,29 @@ int step_geom_tune(StepGeom* sg, const StepPhys& ph_in, void* W, void* U, hipStr
         else g_cache.emplace(key, c);
     }
     return apply(sg, c);
+}
+
+int step_geom_tune(StepGeom* sg, const StepPhys& ph_in, void* W, void* U, hipStream_t s) {
+    StepPhys ph = ph_in;
+    ph.mass = 0;
+    const double scale = 2.0 / ((double)sg->nx * sg->ny * sg->nz);
+    const ColGeom gy = step_geom_y(*sg), gz = step_geom_z(*sg);
+    const bool is3d = sg->is3d;
+    // noisy and noise-free steps run different x-pass kernels (the draw costs registers), so
+    // each tunes on its own
+    const int noisy = ph.temp > 0 ? 1 : 0;
+    return step_geom_tune_runs(
+        sg, noisy, true,
+        [&](const LinePlan& lp) {
+            StepGeom g = *sg;
+            g.px = lp;
+            return launch_xupd(g, ph, W, U, s);
+        },
+        [&](const LinePlan& lp) {
+            return is3d ? launch_step_col(lp, W, gy, -1, scale, ph.split, nullptr, s)
+                        : launch_col_nl(lp, W, gy, scale, ph.split, nullptr, s);
+        },
+        [&](const LinePlan& lp) { return launch_col_nl(lp, W, gz, scale, ph.split, nullptr, s); }, s);
 }
 
 int step_geom_apply_plan(StepGeom* sg, const int plan[6]) {
