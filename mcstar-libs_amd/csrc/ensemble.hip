@@ -13,6 +13,7 @@
 // in which every size group is an independent branch, so that the groups'
 // kernels overlap on the GPU.
 #include "ens_kernels.hpp"
+#include "fixed.hpp"
 #include "knn.hpp"
 #include "peaks.hpp"
 #include "stepper.hpp"
@@ -134,11 +135,16 @@ static int ens_col_t(mcs_ensemble* e, EnsGroup* g, const LinePlan& lp, const Col
     const int S = lp.L / P;
     const size_t lds = (size_t)G * lp.L * sizeof(C2<F>);
     const long total = (long)nr * ((cg.nxc + G - 1) / G) * cg.nouter;
-    auto fn = k_ens_col<F, P, DIR>;
+    const void* fx = fixed_ens_col_kernel(lp, DIR, 0);
+    const void* fn = fx ? fx : (const void*)k_ens_col<F, P, DIR>;
     const unsigned grid = persistent_grid(e, G * S, lds, total);
-    hipLaunchKernelGGL(fn, dim3(grid), dim3(G * S), lds, s, (const EnsRep<F>*)g->d_reps + first, nr, cg, G, lp.dev,
-                       (const C2<F>*)lp.tw);
-    MCS_HIP_TRY(hipGetLastError());
+    const EnsRep<F>* reps = (const EnsRep<F>*)g->d_reps + first;
+    FftPlanDev pl = lp.dev;
+    const C2<F>* tw = (const C2<F>*)lp.tw;
+    ColGeom geo = cg;
+    int gg = G, n = nr;
+    void* args[] = {&reps, &n, &geo, &gg, &pl, &tw};
+    MCS_HIP_TRY(hipLaunchKernel(fn, dim3(grid), dim3(G * S), args, lds, s));
     return MCS_OK;
 }
 
@@ -149,12 +155,17 @@ static int ens_col_nl_t(mcs_ensemble* e, EnsGroup* g, const LinePlan& lp, const 
     const int S = lp.L / P;
     const size_t lds = (size_t)G * lp.L * sizeof(C2<F>);
     const long total = (long)nr * ((cg.nxc + G - 1) / G) * cg.nouter;
-    auto fn = k_ens_col_nl<F, P, SPLIT>;
+    const void* fx = fixed_ens_col_kernel(lp, 0, SPLIT);
+    const void* fn = fx ? fx : (const void*)k_ens_col_nl<F, P, SPLIT>;
     const unsigned grid = persistent_grid(e, G * S, lds, total);
-    const double scale = 2.0 / ((double)g->sg.nx * g->sg.ny * g->sg.nz);
-    hipLaunchKernelGGL(fn, dim3(grid), dim3(G * S), lds, s, (const EnsRep<F>*)g->d_reps + first, nr, cg, G, lp.dev,
-                       (const C2<F>*)lp.tw, (F)scale);
-    MCS_HIP_TRY(hipGetLastError());
+    const EnsRep<F>* reps = (const EnsRep<F>*)g->d_reps + first;
+    FftPlanDev pl = lp.dev;
+    const C2<F>* tw = (const C2<F>*)lp.tw;
+    ColGeom geo = cg;
+    int gg = G, n = nr;
+    F scale = (F)(2.0 / ((double)g->sg.nx * g->sg.ny * g->sg.nz));
+    void* args[] = {&reps, &n, &geo, &gg, &pl, &tw, &scale};
+    MCS_HIP_TRY(hipLaunchKernel(fn, dim3(grid), dim3(G * S), args, lds, s));
     return MCS_OK;
 }
 
@@ -215,7 +226,7 @@ template <typename F> static int ens_tune_group(mcs_ensemble* e, EnsGroup* g, hi
     const ColGeom gy = step_geom_y(g->sg), gz = step_geom_z(g->sg);
     const bool is3d = g->sg.is3d;
     return step_geom_tune_runs(
-        &g->sg, 2 + (noisy ? 1 : 0) + 4 * n, false,
+        &g->sg, 2 + (noisy ? 1 : 0) + 4 * n, MCS_FIXED_COLUMNS,
         [&](const LinePlan& lp) {
             EnsGroup t = *g;
             t.sg.px = lp;

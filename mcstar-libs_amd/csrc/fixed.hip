@@ -18,6 +18,7 @@
 // every pass whose bit mcs_set_fixed_kernels has cleared, and every plan for
 // which the tuner timed the runtime-plan kernel faster (LinePlan::nofx).
 #include "fixed.hpp"
+#include "ens_kernels.hpp"
 #include "row_kernels.hpp"
 
 #include <atomic>
@@ -322,6 +323,64 @@ int fixed_xupd(const StepGeom& sg, const StepPhys& ph, void* W, void* U, hipStre
             return sg.prec == 64 ? fixed_xupd_t<double>(t[i], sg, ph, W, U, s) : fixed_xupd_t<float>(t[i], sg, ph, W, U, s);
         }
     return MCS_OK;
+}
+
+// ------------------------------------------------------------------ ensemble column passes
+// The work-table column kernels of the batched ensembles with the plans of the column table
+// above.  Their x pass keeps the runtime plan: its fixed bodies held up to 428 VGPRs (the
+// 20-point pass) and ran slower.
+struct EnsColEntry {
+    int prec, P, L, np;
+    int rad[4];
+    const void* fwd;
+    const void* inv;
+    const void* nl_ns;
+    const void* nl_ref;
+};
+
+template <typename F, int P, int L, int... Rs> static EnsColEntry ens_col_entry() {
+    using FP = FixPlan<L, Rs...>;
+    EnsColEntry e{};
+    e.prec = sizeof(F) == 8 ? 64 : 32;
+    e.P = P;
+    e.L = L;
+    const int r[] = {Rs...};
+    e.np = (int)sizeof...(Rs);
+    for (int i = 0; i < e.np; ++i) e.rad[i] = r[i];
+    e.fwd = (const void*)k_ens_col<F, P, -1, FP>;
+    e.inv = (const void*)k_ens_col<F, P, +1, FP>;
+    e.nl_ns = (const void*)k_ens_col_nl<F, P, MCS_SPLIT_NORTHSTAR, FP>;
+    e.nl_ref = (const void*)k_ens_col_nl<F, P, MCS_SPLIT_REFERENCE, FP>;
+    return e;
+}
+
+static const EnsColEntry* ens_col_table(int* n) {
+    static const EnsColEntry t[] = {
+        ens_col_entry<float, 8, 512, 8, 8, 8>(),   ens_col_entry<float, 16, 512, 2, 16, 16>(),
+        ens_col_entry<float, 32, 512, 16, 32>(),   ens_col_entry<float, 8, 256, 4, 8, 8>(),
+        ens_col_entry<float, 16, 256, 16, 16>(),   ens_col_entry<double, 8, 512, 8, 8, 8>(),
+        ens_col_entry<double, 16, 512, 2, 16, 16>(), ens_col_entry<double, 8, 256, 4, 8, 8>(),
+        ens_col_entry<double, 16, 256, 16, 16>(),
+        ens_col_entry<float, 8, 64, 8, 8>(),       ens_col_entry<float, 16, 64, 4, 16>(),
+        ens_col_entry<float, 12, 96, 3, 2, 4, 4>(), ens_col_entry<float, 8, 128, 2, 8, 8>(),
+        ens_col_entry<float, 16, 128, 8, 16>(),    ens_col_entry<float, 20, 160, 5, 2, 4, 4>(),
+        ens_col_entry<float, 12, 192, 3, 4, 4, 4>(),
+    };
+    *n = (int)(sizeof(t) / sizeof(t[0]));
+    return t;
+}
+
+const void* fixed_ens_col_kernel(const LinePlan& lp, int kind, int split) {
+    if (!(g_fixed.load() & MCS_FIXED_COLUMNS) || lp.nofx) return nullptr;
+    int n = 0;
+    const EnsColEntry* t = ens_col_table(&n);
+    for (int i = 0; i < n; ++i)
+        if (plan_matches(t[i], lp)) {
+            if (kind < 0) return t[i].fwd;
+            if (kind > 0) return t[i].inv;
+            return split == MCS_SPLIT_REFERENCE ? t[i].nl_ref : t[i].nl_ns;
+        }
+    return nullptr;
 }
 
 // ------------------------------------------------------------------ boundary row passes

@@ -16,5 +16,8 @@ int fixed_row(int mode, const LinePlan& lp, const void* wph, const void* src, vo
               size_t lds, hipStream_t s, bool* done);
 // the x pass with the update, likewise
 int fixed_xupd(const StepGeom& sg, const StepPhys& ph, void* W, void* U, hipStream_t s, bool* done);
+// the fixed-plan work-table column kernel of an ensemble pass (kind -1 forward, +1 inverse,
+// 0 nonlinear in the given split) for lp, or null: same arguments as the runtime-plan kernel
+const void* fixed_ens_col_kernel(const LinePlan& lp, int kind, int split);
 // whether a fixed-plan kernel exists for lp (x pass or column passes) and its bit is set
 bool fixed_available(const LinePlan& lp, bool xpass);

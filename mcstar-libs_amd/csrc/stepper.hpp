@@ -65,10 +65,11 @@ ColGeom step_geom_z(const StepGeom& sg);
 int step_geom_tune(StepGeom* sg, const StepPhys& ph, void* W, void* U, hipStream_t s);
 // the autotuner over caller-supplied pass launchers: each runs one pass with a candidate plan
 // (run_y the y pass of a 3D step, or the nonlinear y pass of a 2D one; run_z the z pass);
-// fixed: also time the fixed-plan kernel of a candidate where one exists.  Choices are cached
-// under (extents, precision, tag); the caller's kernels overwrite their buffers.
+// fixed (MCS_FIXED_* bits): for those pass kinds, also time the fixed-plan kernel of a candidate
+// where one exists (the launcher runs it unless lp.nofx).  Choices are cached under
+// (extents, precision, tag); the caller's kernels overwrite their buffers.
 using PassRun = std::function<int(const LinePlan&)>;
-int step_geom_tune_runs(StepGeom* sg, int tag, bool fixed, const PassRun& run_x, const PassRun& run_y,
+int step_geom_tune_runs(StepGeom* sg, int tag, int fixed, const PassRun& run_x, const PassRun& run_y,
                         const PassRun& run_z, hipStream_t s);
 // {P_x, pairs per workgroup, P_y, lines per workgroup, P_z, lines per workgroup}
 void step_geom_plan_info(const StepGeom& sg, int info[6]);

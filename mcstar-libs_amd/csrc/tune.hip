@@ -149,7 +149,7 @@ extern "C" int mcs_get_autotune(int* on) {
     return MCS_OK;
 }
 
-int step_geom_tune_runs(StepGeom* sg, int tag, bool fixed, const PassRun& run_x, const PassRun& run_y,
+int step_geom_tune_runs(StepGeom* sg, int tag, int fixed, const PassRun& run_x, const PassRun& run_y,
                         const PassRun& run_z, hipStream_t s) {
     const auto key = std::make_tuple(sg->nx, sg->ny, sg->nz, sg->prec, tag);
     {
@@ -161,9 +161,10 @@ int step_geom_tune_runs(StepGeom* sg, int tag, bool fixed, const PassRun& run_x,
     const int prec = sg->prec;
     const int maxX = prec == 64 ? 8 : 16, maxC = prec == 64 ? 16 : 32;
     StepChoice c;
-    int st = tune_pass(sg->L, prec, maxX, true, fixed, s, run_x, &c.x);
-    if (!st) st = tune_pass(sg->ny, prec, maxC, false, fixed, s, run_y, &c.y);
-    if (!st && sg->is3d) st = tune_pass(sg->nz, prec, maxC, false, fixed, s, run_z, &c.z);
+    const bool fx = (fixed & MCS_FIXED_XPASS) != 0, fc = (fixed & MCS_FIXED_COLUMNS) != 0;
+    int st = tune_pass(sg->L, prec, maxX, true, fx, s, run_x, &c.x);
+    if (!st) st = tune_pass(sg->ny, prec, maxC, false, fc, s, run_y, &c.y);
+    if (!st && sg->is3d) st = tune_pass(sg->nz, prec, maxC, false, fc, s, run_z, &c.z);
     if (st) return st;
     MCS_HIP_TRY(hipStreamSynchronize(s));
     {
@@ -185,7 +186,7 @@ int step_geom_tune(StepGeom* sg, const StepPhys& ph_in, void* W, void* U, hipStr
     // each tunes on its own
     const int noisy = ph.temp > 0 ? 1 : 0;
     return step_geom_tune_runs(
-        sg, noisy, true,
+        sg, noisy, MCS_FIXED_COLUMNS | MCS_FIXED_XPASS,
         [&](const LinePlan& lp) {
             StepGeom g = *sg;
             g.px = lp;
