@@ -1,9 +1,10 @@
 // Fixed-plan kernels of the single-grid fused step and of the boundary FFT for
 // the shapes that carry the headline workloads: axis lengths 128, 256 and 512
 // in FP32 and FP64, and the lengths of the 3D ensemble's sizes (32 to 192, with
-// 12- and 20-point plans for 3*2^a and 5*2^a) in FP32.  The ensemble's persistent work-table
-// kernels keep the runtime plans: with the fixed bodies (up to 428 VGPRs in
-// the 20-point x pass) the 3D ensemble step was slower (0.69 against 0.61 ms).
+// 12- and 20-point plans for 3*2^a and 5*2^a) in FP32, with the ensembles'
+// work-table column kernels for the same plans.  The ensembles' x pass keeps the
+// runtime plans: with fixed bodies (up to 428 VGPRs in the 20-point x pass) the
+// 3D ensemble step was slower (0.69 against 0.61 ms).
 //
 // They compute exactly what the runtime-plan kernels compute (same butterflies,
 // same twiddle table, same operation order) with the whole line plan known at
