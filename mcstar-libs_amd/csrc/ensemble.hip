@@ -173,6 +173,8 @@ static int ens_col_nl_t(mcs_ensemble* e, EnsGroup* g, const LinePlan& lp, const 
     switch (Pv) {                                                                           \
         case 4: { constexpr int PP = 4; return CALL; }                                      \
         case 8: { constexpr int PP = 8; return CALL; }                                      \
+        case 6: { constexpr int PP = 6; return CALL; }                                      \
+        case 10: { constexpr int PP = 10; return CALL; }                                    \
         case 12: { constexpr int PP = 12; return CALL; }                                    \
         case 20: { constexpr int PP = 20; return CALL; }                                    \
         case 16: { constexpr int PP = 16; return CALL; }                                    \

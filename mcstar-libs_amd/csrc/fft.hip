@@ -217,6 +217,8 @@ static int launch_col_t(const LinePlan& lp, const void* src, void* dst, const Co
         switch (lp.P) {
             case 4: return launch_col_reg_t<F, 4, DIR>(lp, src, dst, g, s);
             case 8: return launch_col_reg_t<F, 8, DIR>(lp, src, dst, g, s);
+            case 6: return launch_col_reg_t<F, 6, DIR>(lp, src, dst, g, s);
+            case 10: return launch_col_reg_t<F, 10, DIR>(lp, src, dst, g, s);
             case 12: return launch_col_reg_t<F, 12, DIR>(lp, src, dst, g, s);
             case 20: return launch_col_reg_t<F, 20, DIR>(lp, src, dst, g, s);
             case 16: return launch_col_reg_t<F, 16, DIR>(lp, src, dst, g, s);

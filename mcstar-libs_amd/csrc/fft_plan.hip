@@ -83,6 +83,19 @@ static bool reg_plan_pow2(int L, int P, std::vector<int>* rad) {
     return true;
 }
 
+// 3*2^a with 6 points per thread or 5*2^a with 10: the radix-3 or radix-5 pass, then radix-2
+// passes (twice the threads of the 12- and 20-point plans, for small grids)
+static bool reg_plan_small(int L, int P, std::vector<int>* rad) {
+    int a, b, c;
+    if (L < 6 || factor235(L, a, b, c) != 1 || a < 1 || a + 1 > MCS_MAX_PASSES) return false;
+    const bool three = P == 6 && b == 1 && c == 0, five = P == 10 && b == 0 && c == 1;
+    if (!(three || five) || L / P > 1024) return false;
+    std::vector<int> r{three ? 3 : 5};
+    for (int i = 0; i < a; ++i) r.push_back(2);
+    if (rad) *rad = r;
+    return true;
+}
+
 std::vector<int> line_plan_reg_options(int L, int prec) {
     std::vector<int> out;
     const int def = reg_plan(L, prec, nullptr);
@@ -93,6 +106,8 @@ std::vector<int> line_plan_reg_options(int L, int prec) {
         if (P == def || (prec == 64 && P > 16) || (size_t)L * e > MCS_LDS_MAX) continue;
         if (reg_plan_pow2(L, P, nullptr)) out.push_back(P);
     }
+    for (int P : {6, 10})
+        if (P != def && (size_t)L * e <= MCS_LDS_MAX && reg_plan_small(L, P, nullptr)) out.push_back(P);
     return out;
 }
 
@@ -191,7 +206,8 @@ int line_plan_build_reg(LinePlan* lp, int L, int prec, int P) {
     if (P == reg_plan(L, prec, nullptr)) {
         reg_plan(L, prec, &rad);
     } else {
-        MCS_CHECK(reg_plan_pow2(L, P, &rad), MCS_ERR_UNSUPPORTED, "no register plan with this many points per thread");
+        MCS_CHECK(reg_plan_pow2(L, P, &rad) || reg_plan_small(L, P, &rad), MCS_ERR_UNSUPPORTED,
+                  "no register plan with this many points per thread");
     }
     const size_t e = prec == 64 ? 16 : 8;
     MCS_CHECK((size_t)L * e <= MCS_LDS_MAX, MCS_ERR_UNSUPPORTED, "FFT length too large for LDS");

@@ -101,6 +101,8 @@ static int launch_row_reg_d(const LinePlan& lp, const void* src, void* dst, long
     switch (lp.P) {
         case 4: return launch_row_reg_t<F, 4, DIR>(lp, src, dst, nrows, s);
         case 8: return launch_row_reg_t<F, 8, DIR>(lp, src, dst, nrows, s);
+        case 6: return launch_row_reg_t<F, 6, DIR>(lp, src, dst, nrows, s);
+        case 10: return launch_row_reg_t<F, 10, DIR>(lp, src, dst, nrows, s);
         case 12: return launch_row_reg_t<F, 12, DIR>(lp, src, dst, nrows, s);
         case 20: return launch_row_reg_t<F, 20, DIR>(lp, src, dst, nrows, s);
         case 16: return launch_row_reg_t<F, 16, DIR>(lp, src, dst, nrows, s);
@@ -138,6 +140,8 @@ static int launch_col_nl_d(const LinePlan& lp, void* W, const ColGeom& g, double
     switch (lp.P) {
         case 4: return launch_col_nl_t<F, 4, SPLIT>(lp, W, g, scale, d_step, s);
         case 8: return launch_col_nl_t<F, 8, SPLIT>(lp, W, g, scale, d_step, s);
+        case 6: return launch_col_nl_t<F, 6, SPLIT>(lp, W, g, scale, d_step, s);
+        case 10: return launch_col_nl_t<F, 10, SPLIT>(lp, W, g, scale, d_step, s);
         case 12: return launch_col_nl_t<F, 12, SPLIT>(lp, W, g, scale, d_step, s);
         case 20: return launch_col_nl_t<F, 20, SPLIT>(lp, W, g, scale, d_step, s);
         case 16: return launch_col_nl_t<F, 16, SPLIT>(lp, W, g, scale, d_step, s);
@@ -191,6 +195,8 @@ static int launch_xupd_d(const StepGeom& sg, void* W, void* U, const XArgs<F>& a
     switch (sg.px.P) {
         case 4: return launch_xupd_t<F, 4, SPLIT>(sg, W, U, ar, s);
         case 8: return launch_xupd_t<F, 8, SPLIT>(sg, W, U, ar, s);
+        case 6: return launch_xupd_t<F, 6, SPLIT>(sg, W, U, ar, s);
+        case 10: return launch_xupd_t<F, 10, SPLIT>(sg, W, U, ar, s);
         case 12: return launch_xupd_t<F, 12, SPLIT>(sg, W, U, ar, s);
         case 20: return launch_xupd_t<F, 20, SPLIT>(sg, W, U, ar, s);
         case 16: return launch_xupd_t<F, 16, SPLIT>(sg, W, U, ar, s);

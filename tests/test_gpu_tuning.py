@@ -46,7 +46,7 @@ def variants(mcs, shp, dtype):
 # (z, y, x) shapes; lengths 128, 256 and 512 also run the fixed-plan kernels of their plans
 CASES = [((8, 512, 16), np.float32), ((8, 32, 1024), np.float32), ((32, 32, 32), np.float32),
          ((8, 16, 256), np.float64), ((48, 80), np.float32), ((64, 256), np.float64),
-         ((4, 256, 512), np.float32)]
+         ((4, 256, 512), np.float32), ((12, 96, 160), np.float32)]
 
 
 @pytest.mark.parametrize("shp,dtype", CASES)
