@@ -92,3 +92,17 @@ def test_unsupported_length_is_an_error(mcs):
     with pytest.raises(mcs.McsError) as e:
         mcs.FFTPlan((17, 34), np.float32, mcs.R2C)
     assert e.value.code == mcs.ERR_UNSUPPORTED
+
+
+@pytest.mark.parametrize("shp,dt,tol", [((24, 100, 256), np.float32, 1e-5), ((12, 70, 128), np.float64, 2e-14),
+                                        ((10, 130, 256), np.float32, 1e-5)])
+def test_long_stride_columns_in_row_interleaved_order(mcs, shp, dt, tol):
+    # z columns over rows of nx/2+1 values with a stride above 64 KB run in the row-interleaved
+    # XCD order, whose last ny % 8 rows of tiles take plain order (ny = 100, 70, 130 here; the
+    # 10-point z columns run on the generic engine)
+    x = np.random.default_rng(shp[1]).standard_normal(shp).astype(dt)
+    k = mcs.rfftn(x)
+    want = np.fft.rfftn(x.astype(np.float64))
+    assert np.abs(want - k).max() / np.sqrt(x.size) < tol
+    y = mcs.irfftn(k, shp)
+    assert np.abs(y.astype(np.float64) / x.size - x).max() < tol * 2
