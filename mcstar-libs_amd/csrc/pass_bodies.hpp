@@ -107,7 +107,7 @@ MCS_DEV int ksigned(int i, int n) { return i < n / 2 + 1 ? i : i - n; }
 //   U'(c, p) = E' + i O',  U'(m, -p) = conj(E') + i conj(O').
 // NOISE = false compiles the noise draw out (ar.noise must then be 0): the draw's call keeps
 // many registers live across the unrolled update (the 512-point FP32 x pass: 125 VGPRs against 191)
-template <typename F, bool NOISE = true>
+template <typename F, bool NOISE = true, bool INL = false>
 MCS_DEV void xupd_pair(C2<F> nA, C2<F> nBr, C2<F> pA, C2<F> pBr, int p, int iy, int iz, int my, int mz, C2<F> w,
                        const XArgs<F>& ar, uint32_t draw, C2<F>* uc, C2<F>* um) {
     const C2<F> nB = cconj(nBr), pB = cconj(pBr);
@@ -126,7 +126,8 @@ MCS_DEV void xupd_pair(C2<F> nA, C2<F> nBr, C2<F> pA, C2<F> pBr, int p, int iy, 
         // one Philox block gives both modes; a self-mirrored row takes mode p + L as the
         // conjugate of its partner L - p (a second block) and real modes at p = 0
         const long rc = (long)iz * ar.ny + iy;
-        const XiPair<F> x = xi_block<F>((uint64_t)rc * ar.L + p, ar.key0, ar.key1, draw);
+        const uint64_t ctr = (uint64_t)rc * ar.L + p;
+        const XiPair<F> x = INL ? xi_block_inl<F>(ctr, ar.key0, ar.key1, draw) : xi_block<F>(ctr, ar.key0, ar.key1, draw);
         xi1 = x.a;
         xi2 = x.b;
         if (iy == my && iz == mz) {
@@ -135,7 +136,8 @@ MCS_DEV void xupd_pair(C2<F> nA, C2<F> nBr, C2<F> pA, C2<F> pBr, int p, int iy, 
                 xi1 = mk<F>(s2 * x.a.x, F(0));
                 xi2 = mk<F>(s2 * x.b.x, F(0));
             } else {
-                xi2 = cconj(xi_block<F>((uint64_t)rc * ar.L + (ar.L - p), ar.key0, ar.key1, draw).a);
+                const uint64_t ctr2 = (uint64_t)rc * ar.L + (ar.L - p);
+                xi2 = cconj((INL ? xi_block_inl<F>(ctr2, ar.key0, ar.key1, draw) : xi_block<F>(ctr2, ar.key0, ar.key1, draw)).a);
             }
         }
     }
@@ -207,7 +209,7 @@ MCS_DEV void xupd_body(C2<F>* __restrict__ W, C2<F>* __restrict__ U, const int2*
         const int p = t + m * S;
         const int pp = p == 0 ? 0 : L - p;
         C2<F> uc, um;
-        xupd_pair<F, NOISE>(Nc[p], Nm[pp], a[m], b[m], p, iyc, izc, iym, izm, wph[p], ar, draw, &uc, &um);
+        xupd_pair<F, NOISE, (P <= 8)>(Nc[p], Nm[pp], a[m], b[m], p, iyc, izc, iym, izm, wph[p], ar, draw, &uc, &um);
         a[m] = uc;
         b[m] = um;
     }

@@ -103,12 +103,17 @@ template <typename F> MCS_DEV C2<F> box_muller(uint32_t x, uint32_t y, F* z0) {
 template <typename F> struct XiPair {
     C2<F> a, b;
 };
-template <typename F>
-__device__ __noinline__ XiPair<F> xi_block(uint64_t c, uint32_t k0, uint32_t k1, uint32_t draw) {
+template <typename F> MCS_DEV XiPair<F> xi_block_inl(uint64_t c, uint32_t k0, uint32_t k1, uint32_t draw) {
     const u32x4 o = philox4x32_10(u32x4{(uint32_t)c, (uint32_t)(c >> 32), draw, MCS_TAG_KROW}, k0, k1);
     F z;
     XiPair<F> r;
     r.a = box_muller<F>(o.x, o.y, &z);
     r.b = box_muller<F>(o.z, o.w, &z);
     return r;
+}
+// the same, out of line: inlined into the fully unrolled x pass of 12 or more points per
+// thread it outgrows the unroller's budget
+template <typename F>
+__device__ __noinline__ XiPair<F> xi_block(uint64_t c, uint32_t k0, uint32_t k1, uint32_t draw) {
+    return xi_block_inl<F>(c, k0, k1, draw);
 }
