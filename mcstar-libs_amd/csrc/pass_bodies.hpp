@@ -168,6 +168,9 @@ MCS_DEV void xupd_body(C2<F>* __restrict__ W, C2<F>* __restrict__ U, const int2*
                        int G, const XArgs<F>& ar, const FftPlanDev& pl, const C2<F>* __restrict__ tw,
                        const C2<F>* __restrict__ wph, uint32_t draw, C2<F>* lds, int blk, int tid) {
     const int L = FP::fixed ? FP::L : pl.L, S = L / P;
+    // the noise draw inline where the unrolled body stays in registers (with it, the 20-point
+    // FP32 and the 16- and 20-point FP64 bodies went to scratch); a call elsewhere
+    constexpr bool XI_INLINE = P <= 12 || (P == 16 && sizeof(F) == 4);
     const int t = tid % S, g = tid / S;
     const int q = blk * G + g;
     const bool valid = q < npairs && g < G;
@@ -209,7 +212,7 @@ MCS_DEV void xupd_body(C2<F>* __restrict__ W, C2<F>* __restrict__ U, const int2*
         const int p = t + m * S;
         const int pp = p == 0 ? 0 : L - p;
         C2<F> uc, um;
-        xupd_pair<F, NOISE, (P <= 8)>(Nc[p], Nm[pp], a[m], b[m], p, iyc, izc, iym, izm, wph[p], ar, draw, &uc, &um);
+        xupd_pair<F, NOISE, XI_INLINE>(Nc[p], Nm[pp], a[m], b[m], p, iyc, izc, iym, izm, wph[p], ar, draw, &uc, &um);
         a[m] = uc;
         b[m] = um;
     }

@@ -111,8 +111,8 @@ template <typename F> MCS_DEV XiPair<F> xi_block_inl(uint64_t c, uint32_t k0, ui
     r.b = box_muller<F>(o.z, o.w, &z);
     return r;
 }
-// the same, out of line: inlined into the fully unrolled x pass of 12 or more points per
-// thread it outgrows the unroller's budget
+// the same, out of line: inlined into the largest unrolled x passes it pushes their register
+// arrays to scratch
 template <typename F>
 __device__ __noinline__ XiPair<F> xi_block(uint64_t c, uint32_t k0, uint32_t k1, uint32_t draw) {
     return xi_block_inl<F>(c, k0, k1, draw);
