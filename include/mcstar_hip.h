@@ -67,11 +67,14 @@ int mcs_memset(void* dst, int value, size_t bytes);
  * input is left unmodified (the plan owns a scratch buffer).  Axis lengths
  * may have prime factors 2, 3, 5, 7, 11 and 13 (cuda/math.hpp:125-149 rounds
  * sizes to 2,3,5-smooth ones).  Lengths 2^a, 3*2^a and 5*2^a run in
- * registers up to 16384 points (f32) / 8192 (f64, powers of two); other
- * lengths need two copies of the line in 160 KB of LDS (at most 10239
- * points in f32, 5119 in f64).  Larger lengths return MCS_ERR_UNSUPPORTED.
+ * registers up to 8192, 12288 and 10240 points in f32 (8192, 6144 and 5120
+ * in f64); other lengths need two copies of the line in 160 KB of LDS (at
+ * most 10239 points in f32, 5119 in f64).  Larger lengths return
+ * MCS_ERR_UNSUPPORTED.
  * C2R of a non-Hermitian spectrum follows numpy.fft.irfftn: the
- * imaginary parts of the x = 0 and x = nx/2 entries are ignored. */
+ * imaginary parts of the x = 0 and x = nx/2 entries are ignored.
+ * in == out returns MCS_ERR_INVALID for R2C, and for C2R of a 1D transform
+ * (the reference's transforms are out of place: its real view is unpadded). */
 #define MCS_R2C 0
 #define MCS_C2R 1
 typedef struct mcs_fft_plan mcs_fft_plan;

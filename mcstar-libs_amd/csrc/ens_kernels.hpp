@@ -14,7 +14,7 @@ template <typename F> struct EnsRep {
 
 // ------------------------------------------------------------------ kernels
 template <typename F, int P, int SPLIT, bool WAVE, typename FP = RunPlan, bool NOISE = true>
-__global__ void __launch_bounds__(256) k_ens_xupd(const EnsRep<F>* __restrict__ reps, int nr, int units,
+__global__ void __launch_bounds__(MCS_XPASS_MAX_THREADS) k_ens_xupd(const EnsRep<F>* __restrict__ reps, int nr, int units,
                                                   const int2* __restrict__ pairs, int npairs, int G, FftPlanDev pl,
                                                   const C2<F>* __restrict__ tw, const C2<F>* __restrict__ wph,
                                                   const int64_t* __restrict__ d_step, int draw_off) {

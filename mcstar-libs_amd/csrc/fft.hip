@@ -388,6 +388,11 @@ int fft_exec_async(mcs_fft_plan* p, const void* in, void* out, hipStream_t s) {
 
 extern "C" int mcs_fft_exec(mcs_fft_plan* p, const void* in, void* out) {
     MCS_CHECK(p && in && out, MCS_ERR_INVALID, "null argument");
+    // the row pass of R2C reads real rows of nx values and writes complex rows of nx/2 + 1, so
+    // in place other workgroups would overwrite rows before they are read; C2R stages its column
+    // passes through the plan's scratch, except in 1D where the row pass alone runs
+    MCS_CHECK(in != out || (p->dir == MCS_C2R && (p->ext[1] > 1 || p->ext[2] > 1)), MCS_ERR_INVALID,
+              "in-place transform not supported (R2C, and C2R of one row, need separate input and output)");
     int st = fft_exec_async(p, in, out, mcs_stream());
     if (st) return st;
     MCS_HIP_TRY(hipStreamSynchronize(mcs_stream()));

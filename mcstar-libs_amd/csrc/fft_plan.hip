@@ -23,11 +23,43 @@ static bool generic_ok(int L, std::vector<int>* extra) {
     return m == 1;
 }
 
-// Register-engine plan: P points per thread, every radix divides P, S = L/P <= 1024.
-// Powers of two take in-register radix-16/32 passes so that 256, 512 and 1024 need a
-// single LDS exchange per transform; 3*2^a and 5*2^a use one radix-3/5 pass and radix-4 passes.
-// FP64 keeps P = 8 for powers of two (radix-8 passes): larger register blocks of doubles
-// drop the kernels to one wavefront per SIMD.
+// Register-engine plans: P points per thread, every radix divides P, and the S = L/P threads of
+// a line within the launch bound of the register kernels (mcs_reg_max_threads(P, prec): 256 when a
+// thread holds more than 64 bytes of data or 16+ floats, else 1024).
+
+// power-of-two L with P points per thread: radix-P passes after one remainder pass
+static bool reg_plan_pow2(int L, int P, int prec, std::vector<int>* rad) {
+    int a, b, c;
+    if (L < 4 || factor235(L, a, b, c) != 1 || b || c) return false;
+    if (P < 4 || (P & (P - 1)) || P > L || L / P > mcs_reg_max_threads(P, prec)) return false;
+    int lp = 0;
+    while ((1 << lp) < P) ++lp;
+    std::vector<int> r;
+    if (a % lp) r.push_back(1 << (a % lp));
+    for (int i = 0; i < a / lp; ++i) r.push_back(P);
+    if (rad) *rad = r;
+    return true;
+}
+
+// 3*2^a with 6 points per thread or 5*2^a with 10: the radix-3 or radix-5 pass, then radix-2
+// passes (twice the threads of the 12- and 20-point plans, for small grids)
+static bool reg_plan_small(int L, int P, int prec, std::vector<int>* rad) {
+    int a, b, c;
+    if (L < 6 || factor235(L, a, b, c) != 1 || a < 1 || a + 1 > MCS_MAX_PASSES) return false;
+    const bool three = P == 6 && b == 1 && c == 0, five = P == 10 && b == 0 && c == 1;
+    if (!(three || five) || L / P > mcs_reg_max_threads(P, prec)) return false;
+    std::vector<int> r{three ? 3 : 5};
+    for (int i = 0; i < a; ++i) r.push_back(2);
+    if (rad) *rad = r;
+    return true;
+}
+
+// The default plan.  Powers of two take in-register radix-16/32 butterflies so that 256, 512
+// and 1024 need a single LDS exchange per transform; 3*2^a and 5*2^a use one radix-3/5 pass and
+// radix-4 passes.  FP64 keeps P = 8 for powers of two (radix-8 passes): larger register blocks of
+// doubles drop the kernels to one wavefront per SIMD.  Where that plan's line would exceed the
+// launch bound, the next plan within it is taken (32 points for 8192 in FP32, 6 or 10 points for
+// the long 3*2^a and 5*2^a lines); none within it means no register plan.
 static int reg_plan(int L, int prec, std::vector<int>* rad, int max_p = 32) {
     int a, b, c;
     if (L < 4 || factor235(L, a, b, c) != 1) return 0;
@@ -62,39 +94,23 @@ static int reg_plan(int L, int prec, std::vector<int>* rad, int max_p = 32) {
         if (a % 2 == 1) r.push_back(2);
         for (int i = 0; i < a / 2; ++i) r.push_back(4);
     }
-    if (P == 0 || L / P > 1024) return 0;
+    if (P == 0) return 0;
+    if (L / P > mcs_reg_max_threads(P, prec)) {
+        P = 0;
+        for (int Q : {32, 16, 8}) {
+            if (Q > max_p || (prec == 64 && Q > 16)) continue;
+            if (reg_plan_pow2(L, Q, prec, &r)) { P = Q; break; }
+        }
+        if (!P)
+            for (int Q : {6, 10})
+                if (reg_plan_small(L, Q, prec, &r)) { P = Q; break; }
+        if (!P) return 0;
+    }
     if (rad) *rad = r;
     return P;
 }
 
 int line_plan_reg_P(int L, int prec) { return reg_plan(L, prec, nullptr); }
-
-// power-of-two L with P points per thread: radix-P passes after one remainder pass
-static bool reg_plan_pow2(int L, int P, std::vector<int>* rad) {
-    int a, b, c;
-    if (L < 4 || factor235(L, a, b, c) != 1 || b || c) return false;
-    if (P < 4 || (P & (P - 1)) || P > L || L / P > 1024) return false;
-    int lp = 0;
-    while ((1 << lp) < P) ++lp;
-    std::vector<int> r;
-    if (a % lp) r.push_back(1 << (a % lp));
-    for (int i = 0; i < a / lp; ++i) r.push_back(P);
-    if (rad) *rad = r;
-    return true;
-}
-
-// 3*2^a with 6 points per thread or 5*2^a with 10: the radix-3 or radix-5 pass, then radix-2
-// passes (twice the threads of the 12- and 20-point plans, for small grids)
-static bool reg_plan_small(int L, int P, std::vector<int>* rad) {
-    int a, b, c;
-    if (L < 6 || factor235(L, a, b, c) != 1 || a < 1 || a + 1 > MCS_MAX_PASSES) return false;
-    const bool three = P == 6 && b == 1 && c == 0, five = P == 10 && b == 0 && c == 1;
-    if (!(three || five) || L / P > 1024) return false;
-    std::vector<int> r{three ? 3 : 5};
-    for (int i = 0; i < a; ++i) r.push_back(2);
-    if (rad) *rad = r;
-    return true;
-}
 
 std::vector<int> line_plan_reg_options(int L, int prec) {
     std::vector<int> out;
@@ -104,30 +120,51 @@ std::vector<int> line_plan_reg_options(int L, int prec) {
     const size_t e = prec == 64 ? 16 : 8;
     for (int P : {4, 8, 16, 32}) {
         if (P == def || (prec == 64 && P > 16) || (size_t)L * e > MCS_LDS_MAX) continue;
-        if (reg_plan_pow2(L, P, nullptr)) out.push_back(P);
+        if (reg_plan_pow2(L, P, prec, nullptr)) out.push_back(P);
     }
     for (int P : {6, 10})
-        if (P != def && (size_t)L * e <= MCS_LDS_MAX && reg_plan_small(L, P, nullptr)) out.push_back(P);
+        if (P != def && (size_t)L * e <= MCS_LDS_MAX && reg_plan_small(L, P, prec, nullptr)) out.push_back(P);
     return out;
 }
 
-int col_lines_per_wg(const LinePlan& lp) {
-    if (lp.G > 0) return lp.G;
-    const int S = lp.L / lp.P;
-    const size_t esz = lp.prec == 64 ? 16 : 8;
+// exchange LDS of a tile: within 64 KB, or a single line (row pair) up to the workgroup maximum
+static bool tile_lds_ok(size_t bytes, int G) { return bytes <= 65536 || (G == 1 && bytes <= MCS_LDS_MAX); }
+
+bool col_tile_ok(int L, int P, int prec, int G) {
+    const size_t esz = prec == 64 ? 16 : 8;
+    return P > 0 && L % P == 0 && G >= 1 && (long)G * (L / P) <= mcs_reg_max_threads(P, prec) &&
+           tile_lds_ok((size_t)G * L * esz, G);
+}
+
+bool x_tile_ok(int L, int P, int prec, int G) {
+    const size_t esz = prec == 64 ? 16 : 8;
+    return P > 0 && L % P == 0 && G >= 1 && (long)G * (L / P) <= MCS_XPASS_MAX_THREADS &&
+           tile_lds_ok((size_t)G * 2 * L * esz, G);
+}
+
+int col_lines_per_wg_default(int L, int P, int prec) {
+    const int S = L / P;
+    const size_t esz = prec == 64 ? 16 : 8;
     int G = (int)(128 / esz);
-    if (G > mcs_reg_max_threads(lp.P, lp.prec) / S) G = mcs_reg_max_threads(lp.P, lp.prec) / S;
-    while (G > 1 && (size_t)G * lp.L * esz > 65536) --G;
+    if (G > mcs_reg_max_threads(P, prec) / S) G = mcs_reg_max_threads(P, prec) / S;
+    while (G > 1 && (size_t)G * L * esz > 65536) --G;
     return G < 1 ? 1 : G;
 }
 
-int xupd_pairs_per_wg(const LinePlan& lp) {
-    if (lp.G > 0) return lp.G;
-    const int S = lp.L / lp.P;
-    const size_t esz = lp.prec == 64 ? 16 : 8;
-    int G = S >= 256 ? 1 : 256 / S;
-    while (G > 1 && (size_t)G * 2 * lp.L * esz > 65536) --G;
+int xupd_pairs_per_wg_default(int L, int P, int prec) {
+    const int S = L / P;
+    const size_t esz = prec == 64 ? 16 : 8;
+    int G = S >= MCS_XPASS_MAX_THREADS ? 1 : MCS_XPASS_MAX_THREADS / S;
+    while (G > 1 && (size_t)G * 2 * L * esz > 65536) --G;
     return G;
+}
+
+int col_lines_per_wg(const LinePlan& lp) {
+    return lp.G > 0 ? lp.G : col_lines_per_wg_default(lp.L, lp.P, lp.prec);
+}
+
+int xupd_pairs_per_wg(const LinePlan& lp) {
+    return lp.G > 0 ? lp.G : xupd_pairs_per_wg_default(lp.L, lp.P, lp.prec);
 }
 
 static void gen_radices(int L, std::vector<int>& r) {
@@ -206,7 +243,7 @@ int line_plan_build_reg(LinePlan* lp, int L, int prec, int P) {
     if (P == reg_plan(L, prec, nullptr)) {
         reg_plan(L, prec, &rad);
     } else {
-        MCS_CHECK(reg_plan_pow2(L, P, &rad) || reg_plan_small(L, P, &rad), MCS_ERR_UNSUPPORTED,
+        MCS_CHECK(reg_plan_pow2(L, P, prec, &rad) || reg_plan_small(L, P, prec, &rad), MCS_ERR_UNSUPPORTED,
                   "no register plan with this many points per thread");
     }
     const size_t e = prec == 64 ? 16 : 8;

@@ -34,6 +34,16 @@ void line_plan_free(LinePlan* lp);
 // (lp.G when the tuner set it, else the default rule)
 int col_lines_per_wg(const LinePlan& lp);
 int xupd_pairs_per_wg(const LinePlan& lp);
+int col_lines_per_wg_default(int L, int P, int prec);
+int xupd_pairs_per_wg_default(int L, int P, int prec);
+// the launch bound of every x-pass kernel (single grid, fixed plan and work table)
+#define MCS_XPASS_MAX_THREADS 256
+// whether G lines (column pass) or G row pairs (x pass) of a register plan with P points per
+// thread fit the kernel's launch bound and 64 KB of exchange LDS (a single line or pair up to
+// MCS_LDS_MAX): the rule every tile obeys, the default ones, the tuner's candidates and explicit
+// plans alike
+bool col_tile_ok(int L, int P, int prec, int G);
+bool x_tile_ok(int L, int P, int prec, int G);
 // register engine eligibility for L (without building)
 int line_plan_reg_P(int L, int prec = 32);
 // generic engine LDS bytes for G lines

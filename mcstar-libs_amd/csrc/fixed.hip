@@ -120,7 +120,7 @@ __global__ void __launch_bounds__(MCS_REG_MAX_THREADS(P, F)) k_col_nl_fx(C2<F>* 
 }
 
 template <typename F, int P, int SPLIT, bool WAVE, typename FP, bool NOISE>
-__global__ void __launch_bounds__(256) k_xupd_fx(C2<F>* __restrict__ W, C2<F>* __restrict__ U,
+__global__ void __launch_bounds__(MCS_XPASS_MAX_THREADS) k_xupd_fx(C2<F>* __restrict__ W, C2<F>* __restrict__ U,
                                                  const int2* __restrict__ pairs, int npairs, int G, XArgs<F> ar,
                                                  const C2<F>* __restrict__ tw, const C2<F>* __restrict__ wph,
                                                  const int64_t* __restrict__ d_step) {

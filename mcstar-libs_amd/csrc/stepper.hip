@@ -45,7 +45,7 @@ __global__ void __launch_bounds__(MCS_REG_MAX_THREADS(P, F)) k_col_nl(C2<F>* W, 
 }
 
 template <typename F, int P, int SPLIT, bool WAVE, bool NOISE>
-__global__ void __launch_bounds__(256) k_xupd(C2<F>* __restrict__ W, C2<F>* __restrict__ U, const int2* __restrict__ pairs,
+__global__ void __launch_bounds__(MCS_XPASS_MAX_THREADS) k_xupd(C2<F>* __restrict__ W, C2<F>* __restrict__ U, const int2* __restrict__ pairs,
                                               int npairs, int G, XArgs<F> ar, FftPlanDev pl,
                                               const C2<F>* __restrict__ tw, const C2<F>* __restrict__ wph,
                                               const int64_t* __restrict__ d_step) {
@@ -392,6 +392,10 @@ static int stepper_create(mcs_stepper** out, const mcs_pfc_params* p, const int*
     MCS_CHECK(p->prec == 32 || p->prec == 64, MCS_ERR_INVALID, "precision must be 32 or 64");
     MCS_CHECK(p->split == MCS_SPLIT_NORTHSTAR || p->split == MCS_SPLIT_REFERENCE, MCS_ERR_INVALID, "bad split");
     MCS_CHECK(p->dt > 0 && p->h[0] > 0 && p->h[1] > 0 && p->h[2] > 0, MCS_ERR_INVALID, "dt and h must be positive");
+    if (plan) {
+        const int rc = step_plan_validate(p->ext, p->prec, plan);
+        if (rc) return rc;
+    }
     (void)mcs_stream();
     auto* st = new mcs_stepper();
     st->p = *p;

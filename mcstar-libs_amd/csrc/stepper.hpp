@@ -75,3 +75,6 @@ int step_geom_tune_runs(StepGeom* sg, int tag, int fixed, const PassRun& run_x, 
 void step_geom_plan_info(const StepGeom& sg, int info[6]);
 // installs an explicit plan in that layout after checking it against the candidate lists
 int step_geom_apply_plan(StepGeom* sg, const int plan[6]);
+// an explicit plan against the grid's axis lengths: each P one of line_plan_reg_options, each tile
+// within its kernel's launch bound and LDS (host only, before anything is allocated or launched)
+int step_plan_validate(const int ext[3], int prec, const int plan[6]);

@@ -66,25 +66,26 @@ template <typename Fn> double time_launch(hipStream_t s, Fn&& launch) {
 
 size_t esz(int prec) { return prec == 64 ? 16 : 8; }
 
-// tile shapes of a column pass: G lines of L points, G*S threads within the launch bound,
-// G*L elements of LDS within 64 KB
+// tile shapes of a column pass: the default tile first, then G lines of L points for G in
+// {4, 8, 16, 32}, every one within the launch bound and 64 KB of LDS (col_tile_ok)
 std::vector<int> col_tiles(int L, int P, int prec) {
-    const int S = L / P;
     std::vector<int> out;
+    const int def = col_lines_per_wg_default(L, P, prec);
+    if (col_tile_ok(L, P, prec, def)) out.push_back(def);
     for (int G : {4, 8, 16, 32})
-        if (G * S <= mcs_reg_max_threads(P, prec) && (size_t)G * L * esz(prec) <= 65536) out.push_back(G);
-    if (out.empty() && S <= mcs_reg_max_threads(P, prec) && (size_t)L * esz(prec) <= 65536) out.push_back(1);
+        if (G != def && col_tile_ok(L, P, prec, G)) out.push_back(G);
     return out;
 }
 
-// tile shapes of the x pass: G row pairs, 64..256 threads (the kernel's launch bound),
-// two rows of LDS per pair within 64 KB
+// tile shapes of the x pass: the default tile first, then G row pairs (powers of two) of 64..256
+// threads, every one within the launch bound and two rows of LDS per pair within 64 KB (x_tile_ok)
 std::vector<int> x_tiles(int L, int P, int prec) {
     const int S = L / P;
     std::vector<int> out;
-    for (int G = 1; G * S <= 256; G *= 2)
-        if ((G * S >= 64 || (G == 1 && S >= 64)) && (size_t)G * 2 * L * esz(prec) <= 65536) out.push_back(G);
-    if (out.empty() && S <= 256 && (size_t)2 * L * esz(prec) <= 65536) out.push_back(std::max(1, 64 / S));
+    const int def = xupd_pairs_per_wg_default(L, P, prec);
+    if (x_tile_ok(L, P, prec, def)) out.push_back(def);
+    for (int G = 1; G * S <= MCS_XPASS_MAX_THREADS; G *= 2)
+        if (G != def && (G * S >= 64 || (G == 1 && S >= 64)) && x_tile_ok(L, P, prec, G)) out.push_back(G);
     return out;
 }
 
@@ -214,19 +215,25 @@ int step_geom_tune(StepGeom* sg, const StepPhys& ph_in, void* W, void* U, hipStr
         [&](const LinePlan& lp) { return launch_col_nl(lp, W, gz, scale, ph.split, nullptr, s); }, s);
 }
 
-int step_geom_apply_plan(StepGeom* sg, const int plan[6]) {
-    const int prec = sg->prec;
-    const int n = sg->is3d ? 3 : 2;
-    const int L[3] = {sg->L, sg->ny, sg->nz};
-    for (int i = 0; i < n; ++i) {
+int step_plan_validate(const int ext[3], int prec, const int plan[6]) {
+    MCS_CHECK(plan, MCS_ERR_INVALID, "null plan");
+    const bool is3d = ext[2] > 1;
+    const int L[3] = {ext[0] / 2, ext[1], ext[2]};
+    for (int i = 0; i < (is3d ? 3 : 2); ++i) {
         const int P = plan[2 * i], G = plan[2 * i + 1];
         const std::vector<int> opts = line_plan_reg_options(L[i], prec);
         MCS_CHECK(std::find(opts.begin(), opts.end(), P) != opts.end(), MCS_ERR_UNSUPPORTED,
                   "points per thread not available for this axis length");
-        const std::vector<int> tiles = i == 0 ? x_tiles(L[i], P, prec) : col_tiles(L[i], P, prec);
-        MCS_CHECK(std::find(tiles.begin(), tiles.end(), G) != tiles.end(), MCS_ERR_UNSUPPORTED,
+        MCS_CHECK(i == 0 ? x_tile_ok(L[i], P, prec, G) : col_tile_ok(L[i], P, prec, G), MCS_ERR_UNSUPPORTED,
                   "tile shape outside the launch and LDS limits of this plan");
     }
+    return MCS_OK;
+}
+
+int step_geom_apply_plan(StepGeom* sg, const int plan[6]) {
+    const int ext[3] = {sg->nx, sg->ny, sg->nz};
+    int st = step_plan_validate(ext, sg->prec, plan);
+    if (st) return st;
     StepChoice c;
     c.x = PassChoice{plan[0], plan[1], 0};
     c.y = PassChoice{plan[2], plan[3], 0};

@@ -528,12 +528,20 @@ void orc_update_eom_hpfc(double* kpsid, const double* knld, const double* x1d, c
         }
 }
 
-/* eom.hpp:143-171: operator 1 - eps - P(k^2), P(x) = sum_j c_2j x^j with the reference's fitted
-   coefficients, denom = 1 - dt dop.x (1 - eps - P); knoise may be NULL */
+/* eom.hpp:143-149: the fitted coefficients c_0, c_2, ..., c_12 of P (data of that model) */
+static const double poly_ref[7] = {-57.94636917114257, 164.4074088310241, -181.1481995196213, 100.6942650417575,
+                                   -30.20985042718309, 4.678065276755401, -0.293423255328944};
+
+void orc_poly_coefficients(double out[7]) {
+    for (int j = 0; j < 7; j++) out[j] = poly_ref[j];
+}
+
+/* eom.hpp:151-171: operator 1 - eps - P(k^2), P(x) = sum_j c[j] x^j (coef[0..ncoef), NULL: the
+   reference's fitted coefficients), denom = 1 - dt dop.x (1 - eps - P); knoise may be NULL */
 void orc_update_eom_poly(double* kpsid, const double* knld, const double* knoised, const int n[3],
-                         const double lens[3], double namp, double dt, double eps) {
-    static const double c[7] = {-57.94636917114257, 164.4074088310241, -181.1481995196213, 100.6942650417575,
-                                -30.20985042718309, 4.678065276755401, -0.293423255328944};
+                         const double lens[3], double namp, double dt, double eps, const double* coef, int ncoef) {
+    const double* c = coef ? coef : poly_ref;
+    const int nc = coef ? ncoef : 7;
     const int nx = n[0], ny = n[1], nz = n[2], nxc = nx / 2 + 1;
     const double na = namp * sqrt((double)nx * ny * nz);
     cplx* kpsi = (cplx*)kpsid;
@@ -546,7 +554,7 @@ void orc_update_eom_poly(double* kpsid, const double* knld, const double* knoise
                 const double k0 = kval(i0, nx, lens[0]), k1 = kval(i1, ny, lens[1]), k2 = kval(i2, nz, lens[2]);
                 const double ksq = k0 * k0 + k1 * k1 + k2 * k2;
                 double p = 0, xp = 1;
-                for (int j = 0; j < 7; j++) { p += c[j] * xp; xp *= ksq; }
+                for (int j = 0; j < nc; j++) { p += c[j] * xp; xp *= ksq; }
                 const double denom = 1.0 + dt * ksq * (1.0 - eps - p);
                 cplx v = kpsi[idx] - dt * ksq * knl[idx];
                 if (knoise) v += dt * na * sqrt(ksq) * knoise[idx];

@@ -73,9 +73,12 @@ double orc_update_el_etd(const double* kop, double* kpsi, const double* knl, con
 /* 2D update with the three-array hpfc noise (eom.hpp:16-46) */
 void orc_update_eom_hpfc(double* kpsi, const double* knl, const double* x1, const double* x2, const double* x3,
                          const int n[3], const double lens[3], double namp, double dt, double eps);
-/* update with the polynomial linear operator (eom.hpp:143-171) */
+/* update with the polynomial linear operator (eom.hpp:143-171): denominator 1 + dt k^2 (1 - eps - P(k^2)),
+   P(q) = sum_j coef[j] q^j over ncoef coefficients; coef NULL takes the reference's fitted ones */
 void orc_update_eom_poly(double* kpsi, const double* knl, const double* knoise, const int n[3],
-                         const double lens[3], double namp, double dt, double eps);
+                         const double lens[3], double namp, double dt, double eps, const double* coef, int ncoef);
+/* the reference's fitted coefficients c_0, c_2, ..., c_12 (eom.hpp:143-149), P(q) = sum_j c[j] q^j */
+void orc_poly_coefficients(double out[7]);
 /* free energy F = dV * sum[ psi/2 * (-eps + (1-k^2)^2) psi + psi^4/4 ] (cuda/kernels/eom.hpp:111-141) */
 double orc_free_energy(const double* psi, const int n[3], const double h[3], double eps);
 

@@ -57,7 +57,9 @@ def lib():
         L.orc_update_el_etd.restype = ctypes.c_double
         L.orc_update_eom_hpfc.argtypes = [_d, _d, _d, _d, _d, _i, _d, ctypes.c_double, ctypes.c_double,
                                           ctypes.c_double]
-        L.orc_update_eom_poly.argtypes = [_d, _d, _d, _i, _d, ctypes.c_double, ctypes.c_double, ctypes.c_double]
+        L.orc_update_eom_poly.argtypes = [_d, _d, _d, _i, _d, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                          _d, ctypes.c_int]
+        L.orc_poly_coefficients.argtypes = [_d]
         L.orc_free_energy.argtypes = [_d, _i, _d, ctypes.c_double]
         L.orc_free_energy.restype = ctypes.c_double
         L.orc_findpeaks.argtypes = [_d, _i, ctypes.c_double, _i64, ctypes.c_int64]
@@ -218,12 +220,21 @@ def update_eom_hpfc(kpsi, knl, x1, x2, x3, shape_zyx, lens, namp, dt, eps):
     return kp
 
 
-def update_eom_poly(kpsi, knl, knoise, shape_zyx, lens, namp, dt, eps):
+def update_eom_poly(kpsi, knl, knoise, shape_zyx, lens, namp, dt, eps, coef=None):
+    """coef: P(q) = sum_j coef[j] q^j; None takes the reference's fitted coefficients"""
     kp = _c(kpsi).copy()
     kz = None if knoise is None else _c(knoise)
+    cf = None if coef is None else np.ascontiguousarray(coef, dtype=np.float64)
     lib().orc_update_eom_poly(_p(kp), _p(_c(knl)), None if kz is None else _p(kz), _p(_dims(shape_zyx), _i),
-                              _p(np.asarray(lens, dtype=np.float64)), namp, dt, eps)
+                              _p(np.asarray(lens, dtype=np.float64)), namp, dt, eps, None if cf is None else _p(cf),
+                              0 if cf is None else cf.size)
     return kp
+
+
+def poly_coefficients():
+    out = np.empty(7, dtype=np.float64)
+    lib().orc_poly_coefficients(_p(out))
+    return out
 
 
 def free_energy(psi, h, eps):

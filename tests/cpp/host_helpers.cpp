@@ -5,6 +5,20 @@
 #include <common/technicalities.hpp>
 
 #include <cstdio>
+#include <vector>
+
+// the consumer typedefs the kept potentials header expects (cufft_unittest/main.cu:6-16 style),
+// with a host-only array standing in for CPUArray (init_pot_line reads real_vext and ptr_real)
+typedef double Float;
+typedef double3 Float3;
+struct HostField {
+    int3 ext;
+    std::vector<Float> v;
+    int3 real_vext() const { return ext; }
+    Float* ptr_real() { return v.data(); }
+};
+typedef HostField CPUArray;
+#include <cuda/potentials.hpp>
 
 int main(int argc, char** argv) {
     const int ns[] = {1, 5, 7, 11, 13, 14, 26, 49, 97, 120, 625, 1000, 1023};
@@ -14,6 +28,14 @@ int main(int argc, char** argv) {
     printf("cond %.12f\n", crystal_condition(int3{1, 0, 0}, int3{0, 1, 1}));
     auto p = calc_closest_lattice_indices(int2{0, 0}, double2{1, 0}, double2{0.5, 0.8660254037844386}, double2{0.9, 0.1});
     printf("closest %d %d %.12f\n", p.first.x, p.first.y, p.second);
+    {
+        // 16 x 12 grid, h = 0.5, box (8, 6), power 2, sigma 2, amp 1.5
+        HostField f{int3{16, 12, 1}, std::vector<Float>(16 * 12)};
+        init_pot_line(f, Float3{8.0, 6.0, 1.0}, Float3{0.5, 0.5, 1.0}, 2, 2.0, 1.5);
+        printf("pot");
+        for (double v : f.v) printf(" %.17g", v);
+        printf("\n");
+    }
     if (argc > 1) {
         const int shape[3] = {2, 3, 4};
         double d[24];

@@ -87,17 +87,88 @@ def test_replicas_match_oracle(mcs, orc, dt, tol):
         assert np.linalg.norm(got - want) / np.linalg.norm(want) < tol
 
 
-def test_config_ensembles_build_and_step(mcs):
-    """The two ensemble workloads at full size: 64 replicas of {128..1024}^2 and 32 of {64..192}^3."""
-    for kind, n in (("2d", 10), ("3d", 4)):
-        reps = mcs.ensemble_config(kind)
-        e = mcs.Ensemble(reps, dtype=np.float32)
-        e.init()
-        e.step(n, mcs.ENS_GRAPH)
-        for r in (0, len(reps) - 1):
-            p = e.get_psi(r).astype(np.float64)
-            assert np.isfinite(p).all()
-            assert abs(p.mean() - reps[r]["psi0"]) < 2e-3
+def _size_groups(reps):
+    """one replica index per grid size"""
+    first = {}
+    for r, p in enumerate(reps):
+        first.setdefault(tuple(p["shape"]), r)
+    return sorted(first.values())
+
+
+def _graph_equals_loop(mcs, reps, nsteps):
+    a = mcs.Ensemble(reps, dtype=np.float32)
+    a.init()
+    a.step(nsteps, mcs.ENS_LOOP)
+    b = mcs.Ensemble(reps, dtype=np.float32)
+    b.init()
+    b.step(nsteps, mcs.ENS_GRAPH)
+    for r in _size_groups(reps):
+        assert np.array_equal(a.get_psi(r), b.get_psi(r)), (r, reps[r]["shape"])
+
+
+def _initial_means(mcs, reps):
+    """the exact mean of each replica's seeded FP32 initial field (the FP64 mass the ensemble carries)"""
+    return [float(mcs.init_field(p["shape"], p["psi0"], p["amp"], p["init_seed"], np.float32).astype(np.float64)
+                  .mean()) for p in reps]
+
+
+def _assert_mass(f, m0, r):
+    """the mass is carried in FP64; the exported FP32 field keeps the mean to the FP32 roundings of the
+    packed k = 0 value (which also holds the Nyquist mode) and of the export scale: two ulps of the
+    mean (3e-8 at |mean| = 0.25; a replica that loses mass is off by orders of magnitude more)"""
+    assert np.isfinite(f).all(), r
+    d = abs(f.mean() - m0)
+    assert d <= 2 * np.spacing(np.float32(abs(m0))), (r, d)
+
+
+def test_config_2d_ensemble_1000_steps(mcs, orc):
+    """64 replicas of {128,192,320,384,640,768,1024}^2, T = 5e-4, dt = 0.5, 1000 steps replayed from the
+    captured graph: every replica finite with its mean density carried exactly (FP64 k = 0; the rest is
+    the rounding of the exported FP32 field); graph and looping launches bit for bit over the first 24
+    steps for one replica of every size; the peak lists of two replicas equal the oracle's on their
+    exported fields (cuda/kernels/findpeaks_2d.hpp:7-27)"""
+    reps = mcs.ensemble_config("2d")
+    _graph_equals_loop(mcs, reps, 24)
+    e = mcs.Ensemble(reps, dtype=np.float32)
+    e.init()
+    m0 = _initial_means(mcs, reps)
+    e.step(1000, mcs.ENS_GRAPH)
+    for r in range(len(reps)):
+        _assert_mass(e.get_psi(r).astype(np.float64), m0[r], r)
+    for r in (_size_groups(reps)[0], _size_groups(reps)[-1]):
+        f = e.get_psi(r)
+        thr = float(f.mean())
+        got = mcs.findpeaks(f, thr)
+        assert got.size > 10 and np.array_equal(got, orc.findpeaks(f.astype(np.float64), thr))
+
+
+def test_config_3d_ensemble_2000_steps_with_in_graph_analysis(mcs, orc):
+    """32 replicas of {64,96,128,160,192}^3, T = 1e-3, dt = 0.25, 2000 steps through mcs_ensemble_run:
+    the last captured graph also holds every replica's peak finding and 12-NN q4/q6 64-bin
+    histograms (cuda/kernels/findpeaks.hpp:31-63 -> bond-order-analysis/bond-order.cpp:289-349).
+    Every replica's mean density is carried exactly, every histogram totals its peak count, two
+    replicas' peak counts and histograms equal the oracle's on their exported fields, and graph and
+    looping launches agree bit for bit over the first 24 steps for one replica of every size."""
+    reps = mcs.ensemble_config("3d")
+    _graph_equals_loop(mcs, reps, 24)
+    e = mcs.Ensemble(reps, dtype=np.float32)
+    e.init()
+    m0 = _initial_means(mcs, reps)
+    counts, h4, h6 = e.run(2000, -10.0, k=12, nbins=64)
+    assert (counts > 12).all()
+    assert np.array_equal(h4.sum(axis=1), counts) and np.array_equal(h6.sum(axis=1), counts)
+    for r in range(len(reps)):
+        _assert_mass(e.get_psi(r).astype(np.float64), m0[r], r)
+    groups = _size_groups(reps)
+    for r in (groups[0], groups[len(groups) // 2]):
+        f = e.get_psi(r).astype(np.float64)
+        n = reps[r]["shape"][0]
+        idx = orc.findpeaks(f, -10.0)
+        assert counts[r] == idx.size
+        ijk = np.column_stack([idx % n, (idx // n) % n, idx // (n * n)]).astype(np.int32)
+        _, out = orc.bond_order_knn(ijk, n, H, 12)
+        assert np.array_equal(h4[r], np.bincount(np.minimum((out[:, 0] * 64).astype(int), 63), minlength=64))
+        assert np.array_equal(h6[r], np.bincount(np.minimum((out[:, 1] * 64).astype(int), 63), minlength=64))
 
 
 def test_analysis_histograms_match_oracle(mcs, orc):

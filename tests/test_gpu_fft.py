@@ -106,3 +106,31 @@ def test_long_stride_columns_in_row_interleaved_order(mcs, shp, dt, tol):
     assert np.abs(want - k).max() / np.sqrt(x.size) < tol
     y = mcs.irfftn(k, shp)
     assert np.abs(y.astype(np.float64) / x.size - x).max() < tol * 2
+
+
+@pytest.mark.parametrize("shp,dt,tol", [((8192, 6), np.float32, 1e-5), ((6144, 4), np.float64, 2e-14),
+                                        ((10240, 4), np.float32, 1e-5), ((12288, 2), np.float32, 1e-5),
+                                        ((5120, 4), np.float64, 2e-14), ((8192, 4), np.float64, 2e-14)])
+def test_long_column_lines(mcs, shp, dt, tol):
+    """columns whose default plan would exceed the register kernels' launch bound take the next plan
+    within it (32 points for 8192 in FP32, 6 points for 6144 in FP64, 10 for 10240 in FP32)"""
+    x = np.random.default_rng(shp[0]).standard_normal(shp).astype(dt)
+    k = mcs.rfftn(x)
+    assert np.abs(np.fft.rfftn(x.astype(np.float64)) - k).max() / np.sqrt(x.size) < tol
+    y = mcs.irfftn(k, shp)
+    assert np.abs(x.astype(np.float64) - y.astype(np.float64) / x.size).max() < tol
+
+
+def test_in_place_r2c_is_rejected(mcs):
+    shp = (4, 6, 8)
+    x = np.random.default_rng(1).standard_normal(shp).astype(np.float32)
+    buf = mcs.to_device(x, mcs.buffer_bytes(shp, np.float32))
+    r2c = mcs.FFTPlan(shp, np.float32, mcs.R2C)
+    with pytest.raises(mcs.McsError):
+        r2c.execute(buf, buf)
+    # C2R of a 3D spectrum runs through the plan's scratch, so in place is fine there
+    ref = mcs.rfftn(x)
+    buf.upload(ref)
+    c2r = mcs.FFTPlan(shp, np.float32, mcs.C2R)
+    c2r.execute(buf, buf)
+    assert np.abs(buf.download(shp, np.float32) / x.size - x).max() < 1e-5

@@ -60,6 +60,21 @@ def test_resume_continues_bit_for_bit(tmp_path, prec):
     assert "step 16" in open(os.path.join(b, "checkpoint.txt")).read()
 
 
+@pytest.mark.parametrize("dims", ["2048x2048", "2560x768"])
+def test_resume_without_tuning_continues_bit_for_bit(tmp_path, dims):
+    """--no-tune runs the default tiles (2 lines per workgroup for the 2048-point FP32 y pass; 3 row
+    pairs for the 1280-point x pass and 10 lines for ny = 768), which the checkpoint's plan line
+    names; a resumed run must accept them and continue bit for bit"""
+    a, b = str(tmp_path / "a"), str(tmp_path / "b")
+    os.makedirs(a)
+    os.makedirs(b)
+    common = ["--no-tune", "--dims", dims, "--eps", 0.25, "--dt", 0.5, "--temp", 5e-4, "--noise-seed", 3]
+    sim("f32", *common, "--steps", 6, "--out", a)
+    sim("f32", *common, "--steps", 2, "--out", b)
+    sim("f32", "--no-tune", "--resume", b, "--steps", 4, "--out", b)
+    assert np.array_equal(np.load(os.path.join(a, "psi_00000006.npy")), np.load(os.path.join(b, "psi_00000006.npy")))
+
+
 def test_load_2d_and_bad_flags(mcs, orc, tmp_path):
     out = str(tmp_path)
     psi = orc.init_field((48, 64), -0.25, 0.05, 2)

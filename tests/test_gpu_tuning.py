@@ -91,7 +91,17 @@ def test_autotune_off_uses_default_rule_and_rejects_bad_plans(mcs):
     finally:
         mcs.set_autotune(True)
     with pytest.raises(mcs.McsError) as e:
-        mcs.Stepper(shp, [H] * 3, 0.3, 0.25, plan=[8, 8, 8, 3, 8, 16])  # tile 3 is not a candidate
+        mcs.Stepper(shp, [H] * 3, 0.3, 0.25, plan=[8, 8, 8, 512, 8, 16])  # 512 lines x 4 threads > 1024
     assert e.value.code == mcs.ERR_UNSUPPORTED
+    # any tile within the launch bound and LDS is accepted, also one the tuner would not try, and the
+    # tile shape does not change the arithmetic
+    psi = mcs.init_field(shp, -0.25, 0.05, 3, np.float32)
+    out = []
+    for plan in ([8, 8, 8, 3, 8, 16], [8, 8, 8, 8, 8, 8]):
+        st = mcs.Stepper(shp, [H] * 3, 0.3, 0.25, plan=plan)
+        st.set_psi(psi)
+        st.step(5)
+        out.append(st.get_psi())
+    assert np.array_equal(out[0], out[1])
     with pytest.raises(mcs.McsError):
         mcs.Stepper(shp, [H] * 3, 0.3, 0.25, plan=[12, 16, 8, 16, 8, 16])  # no 12-point plan for 16

@@ -35,7 +35,35 @@ def test_size_policy_and_layout(helper, tmp_path):
     assert txt[15].startswith("closest 1 0 ")
     a = np.load(npy)
     assert a.shape == (2, 3, 4) and a.dtype == np.float64 and a[0, 0, 0] == -3 and a[-1, -1, -1] == 8.5
-    assert txt[16] == "npy 24 2 3 4 -3 8.5" and txt[17] == "npy dtype mismatch rejected"
+    rest = [l for l in txt[16:] if not l.startswith('"init_pot_line"')]
+    assert rest[1] == "npy 24 2 3 4 -3 8.5" and rest[2] == "npy dtype mismatch rejected"
+
+
+def test_init_pot_line(helper):
+    """cuda/potentials.hpp:5-20 on a 16 x 12 grid (h = 0.5, box 8 x 6, power 2, sigma 2, amp 1.5):
+    x = i0 h - 4, y = i1 h - 3.  The square lattice of period sigma times a strip window in y:
+    equal to amp w(y) on lattice points (x, y multiples of sigma), zero on the cell centres (both
+    coordinates at half periods), (3/4)^power of that at a quarter period in x, periodic in x with
+    period sigma, even in y; w(y) = (1 + tanh(y + sigma/2)) (1 + tanh(sigma/2 - y)) / 4."""
+    out = subprocess.run([helper], check=True, capture_output=True, text=True).stdout.splitlines()
+    assert '"init_pot_line" potential initialized' in out
+    pot = np.array([float(v) for v in next(l for l in out if l.startswith("pot ")).split()[1:]]).reshape(12, 16)
+    amp, sigma = 1.5, 2.0
+    y = np.arange(12) * 0.5 - 3.0
+
+    def w(yy):
+        return (1 + np.tanh(yy + sigma / 2)) * (1 + np.tanh(sigma / 2 - yy)) / 4
+
+    for i1 in (2, 6, 10):          # y = -2, 0, 2
+        for i0 in (0, 4, 8, 12):   # x = -4, -2, 0, 2
+            assert abs(pot[i1, i0] - amp * w(y[i1])) < 1e-14
+    for i1 in (4, 8):              # y = -1, 1
+        for i0 in (2, 6, 10, 14):  # x = -3, -1, 1, 3
+            assert abs(pot[i1, i0]) < 1e-15
+    assert abs(pot[6, 9] - amp * w(0.0) * 0.75 ** 2) < 1e-14  # x = 0.5: cos = 0
+    assert np.abs(pot[:, :12] - pot[:, 4:]).max() < 1e-14
+    assert np.abs(pot[1:12] - pot[1:12][::-1]).max() < 1e-14
+    assert pot.min() >= 0 and abs(pot.max() - amp * w(0.0)) < 1e-14
 
 
 def test_driver_builds_against_kept_headers():

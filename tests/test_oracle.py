@@ -387,27 +387,72 @@ def test_hpfc_update_reduces_to_reference_split_and_filters(orc):
     got = orc.update_eom_hpfc(kp, nl, z, z, z, shp, lens, 0.05, 0.3, 0.2)
     want = orc.update(kp, nl, None, shp, lens, 0.3, 0.2, 0.0, orc.SPLIT_REFERENCE)
     assert np.abs(got - want).max() < 1e-13 * np.abs(want).max()
-    # the noise acts on |k| <= 1/4 only, with weights (c1 kx^2 + c2 ky^2, c2 kx^2 + c1 ky^2, 2 kx ky)
-    xs = [orc.kspace_noise(shp, 7 + j, 0) for j in range(3)]
-    gotn = orc.update_eom_hpfc(kp, nl, *xs, shp, lens, 0.05, 0.3, 0.2)
-    ky = 2 * np.pi * np.fft.fftfreq(48, d=0.7)[:, None]
-    kx = 2 * np.pi * np.fft.rfftfreq(64, d=0.7)[None, :]
+
+
+def test_hpfc_noise_is_isotropic_with_the_fixed_total_weight(orc):
+    """eom.hpp:16-46: the three noise arrays enter with weights w1, w2, w3 per mode; for
+    independent unit xi the per-mode variance w1^2 + w2^2 + w3^2 must not depend on the direction
+    of k.  For weights (c1 kx^2 + c2 ky^2, c2 kx^2 + c1 ky^2, 2 kx ky) with c1 c2 = kappa/2 that
+    holds exactly when (c2 - c1)^2 = 2, and then the variance is (kappa + 2)|k|^4, kappa = 4/3.
+    Each weight is read off by feeding a unit field into one array at a time; the filter keeps
+    |k| <= 1/4 only; the noise is linear in the three arrays."""
+    shp = (64, 64)
+    hh = 0.7
+    lens = [64 * hh, 64 * hh, 1.0]
+    dt, namp, eps = 0.3, 0.05, 0.2
+    rng = np.random.default_rng(2)
+    kp = np.fft.rfftn(rng.standard_normal(shp))
+    nl = np.fft.rfftn(0.2 * rng.standard_normal(shp))
+    z = np.zeros_like(kp)
+    one = np.ones_like(kp)
+    base = orc.update_eom_hpfc(kp, nl, z, z, z, shp, lens, namp, dt, eps)
+    ky = 2 * np.pi * np.fft.fftfreq(64, d=hh)[:, None]
+    kx = 2 * np.pi * np.fft.rfftfreq(64, d=hh)[None, :]
     k2 = kx ** 2 + ky ** 2
-    den = 1 + 0.3 * k2 * ((1 - k2) ** 2 + 1 - 0.2)
-    c2 = np.sqrt(1 + 2 / 3 + np.sqrt(1 + 4 / 3))
-    c1 = 4 / 3 / c2 / 2
-    noise = (c1 * kx ** 2 + c2 * ky ** 2) * xs[0] + (c2 * kx ** 2 + c1 * ky ** 2) * xs[1] + 2 * kx * ky * xs[2]
-    noise = np.where(np.sqrt(k2) <= 0.25, noise, 0)
-    assert np.abs((gotn - got) * den - 0.3 * 0.05 * np.sqrt(48 * 64) * noise).max() < 1e-12
-    assert np.any(np.sqrt(k2) > 0.25) and np.any((np.sqrt(k2) <= 0.25) & (k2 > 0))
+    den = 1 + dt * k2 * ((1 - k2) ** 2 + 1 - eps)
+    scale = dt * namp * np.sqrt(64 * 64)
+    w = []
+    for j in range(3):
+        xs = [z, z, z]
+        xs[j] = one
+        w.append(((orc.update_eom_hpfc(kp, nl, *xs, shp, lens, namp, dt, eps) - base) * den / scale).real)
+    inside = np.sqrt(k2) <= 0.25
+    assert np.any(inside & (k2 > 0)) and np.any(~inside)
+    var = w[0] ** 2 + w[1] ** 2 + w[2] ** 2
+    kappa = 4.0 / 3.0
+    assert np.abs(var[inside] - (kappa + 2) * k2[inside] ** 2).max() < 1e-12 * ((kappa + 2) * 0.25 ** 4)
+    assert np.all(var[~inside] == 0)
+    # the cross array carries 2 kx ky; the other two swap roles under kx <-> ky (a square box)
+    assert np.abs(w[2][inside] - (2 * kx * ky * np.ones_like(k2))[inside]).max() < 1e-14
+    for iy in range(1, 5):
+        for ix in range(1, 5):
+            if np.sqrt(kx[0, ix] ** 2 + ky[iy, 0] ** 2) <= 0.25:
+                assert abs(w[0][iy, ix] - w[1][ix, iy]) < 1e-14
+    # linear in the three arrays
+    xs = [orc.kspace_noise(shp, 7 + j, 0) for j in range(3)]
+    got = orc.update_eom_hpfc(kp, nl, *xs, shp, lens, namp, dt, eps)
+    assert np.abs((got - base) * den / scale - (w[0] * xs[0] + w[1] * xs[1] + w[2] * xs[2])).max() < 1e-12
 
 
 def test_polynomial_operator_update(orc):
+    """eom.hpp:151-171 evaluates 1 - eps - P(k^2) with a fitted degree-6 P.  With P(q) = -(1 - q)^2 =
+    -1 + 2q - q^2 the operator is the PFC one, 1 - eps + (1 - k^2)^2, so the update must equal the
+    reference-split semi-implicit update (eom.hpp:60-72) term for term, noise included; with the
+    fitted coefficients a single mode decays by 1/denom and k = 0 is unchanged."""
     shp, lens, kp, nl, k2 = _etd_case()
-    c = [-57.94636917114257, 164.4074088310241, -181.1481995196213, 100.6942650417575, -30.20985042718309,
-         4.678065276755401, -0.293423255328944]
-    got = orc.update_eom_poly(kp, np.zeros_like(nl), None, shp, lens, 0.0, 0.2, 0.3)
-    den = 1 + 0.2 * k2 * (1 - 0.3 - np.polynomial.polynomial.polyval(k2, c))
+    xi = orc.kspace_noise(shp, 3, 0)
+    n = int(np.prod(shp))
+    dt, eps, namp = 0.2, 0.3, 0.07
+    got = orc.update_eom_poly(kp, nl, xi, shp, lens, namp, dt, eps, coef=[-1.0, 2.0, -1.0])
+    want = orc.update(kp, nl, xi, shp, lens, dt, eps, dt * namp * np.sqrt(n), orc.SPLIT_REFERENCE)
+    assert np.abs(got - want).max() < 1e-13 * np.abs(want).max()
+    # the same with the trailing zero coefficients of a degree-6 polynomial
+    got6 = orc.update_eom_poly(kp, nl, xi, shp, lens, namp, dt, eps, coef=[-1.0, 2.0, -1.0, 0, 0, 0, 0])
+    assert np.abs(got6 - want).max() < 1e-13 * np.abs(want).max()
+    c = orc.poly_coefficients()
+    assert c.size == 7
+    got = orc.update_eom_poly(kp, np.zeros_like(nl), None, shp, lens, 0.0, dt, eps)
+    den = 1 + dt * k2 * (1 - eps - np.polynomial.polynomial.polyval(k2, c))
     assert np.abs(got - kp / den).max() < 1e-12 * np.abs(kp).max()
     assert got[0, 0, 0] == kp[0, 0, 0]
 
