@@ -123,6 +123,35 @@ extern "C" int mcs_memset(void* dst, int value, size_t bytes) {
     return MCS_OK;
 }
 
+static void* g_ws_ptr[MCS_WS_SLOTS] = {};
+static size_t g_ws_size[MCS_WS_SLOTS] = {};
+static std::mutex g_ws_mu;
+
+int mcs_workspace(int slot, size_t bytes, void** out) {
+    MCS_CHECK(slot >= 0 && slot < MCS_WS_SLOTS && out, MCS_ERR_INVALID, "bad workspace slot");
+    int st = ensure_init(-1);
+    if (st) return st;
+    std::lock_guard<std::mutex> lk(g_ws_mu);
+    if (g_ws_size[slot] < bytes) {
+        if (g_ws_ptr[slot]) {
+            MCS_HIP_TRY(hipStreamSynchronize(g_stream));  // kernels of an earlier call may still read it
+            (void)hipFree(g_ws_ptr[slot]);
+            g_ws_ptr[slot] = nullptr;
+            g_ws_size[slot] = 0;
+        }
+        // grow by at least a quarter so that slowly rising sizes do not reallocate every call
+        const size_t want = bytes + bytes / 4;
+        if (hipMalloc(&g_ws_ptr[slot], want) != hipSuccess) {
+            g_ws_ptr[slot] = nullptr;
+            mcs_set_last_error(__FILE__, __LINE__, "workspace allocation failed");
+            return MCS_ERR_NOMEM;
+        }
+        g_ws_size[slot] = want;
+    }
+    *out = g_ws_ptr[slot];
+    return MCS_OK;
+}
+
 static hipEvent_t g_t0 = nullptr, g_t1 = nullptr;
 
 extern "C" int mcs_timer_start(void) {

@@ -625,57 +625,78 @@ int knn_launch(const int64_t* d_idx, const int* d_ijk, const int64_t* d_count, i
     return MCS_OK;
 }
 
-// Sorted neighbour lists of the cutoff form through the cell list, when it applies: every
-// point in [0, box)^3, at least 3 cells of width >= rlim per axis, and no point with more
-// than CMAXN neighbours.  *listed tells whether the lists (*dnl, *dnc, allocated here) hold.
-static int cutoff_lists(const double* dx, const double* xyz, int P, double box, double rlim, hipStream_t s,
-                        bool* listed, int** dnl, int** dnc) {
-    *listed = false;
-    if (!(box > 0) || !(rlim > 0) || P < 256) return MCS_OK;
+// Carves 256-byte aligned arrays out of one workspace; with a null base it only adds up the bytes.
+struct Carve {
+    unsigned char* base = nullptr;
+    size_t off = 0;
+    template <typename T> T* take(size_t n) {
+        T* p = base ? reinterpret_cast<T*>(base + off) : nullptr;
+        off += (n * sizeof(T) + 255) & ~(size_t)255;
+        return p;
+    }
+};
+
+// The cell list of the cutoff form applies when every point lies in [0, box)^3 and at least 3
+// cells of width >= rlim fit per axis; returns the cells per axis, or 0.
+static int cutoff_grid(const double* xyz, int P, double box, double rlim) {
+    if (!(box > 0) || !(rlim > 0) || P < 256) return 0;
     for (long i = 0; i < 3L * P; ++i)
-        if (!(xyz[i] >= 0.0 && xyz[i] < box)) return MCS_OK;
+        if (!(xyz[i] >= 0.0 && xyz[i] < box)) return 0;
     const double gw = std::floor(box / (rlim * (1.0 + 1e-6)));
-    if (!(gw >= 3.0)) return MCS_OK;
-    const int g = (int)std::min(gw, 256.0);
+    if (!(gw >= 3.0)) return 0;
+    return (int)std::min(gw, 256.0);
+}
+
+struct CutoffBufs {
+    double *dx, *dq, *dout;
+    int *dnb, *dlast;
+    int *cnt, *start, *sorted, *ovf, *dnl, *dnc;  // cell list (g > 0)
+    KnnGrid* dkg;
+};
+
+static CutoffBufs cutoff_carve(Carve& c, int P, int g) {
+    CutoffBufs b{};
+    b.dx = c.take<double>(3L * P);
+    b.dq = c.take<double>((long)QS * P);
+    b.dout = c.take<double>(9L * P);
+    b.dnb = c.take<int>(P);
+    b.dlast = c.take<int>(P);
+    if (g > 0) {
+        const long ncell = (long)g * g * g;
+        b.cnt = c.take<int>(ncell + 1);
+        b.start = c.take<int>(ncell + 1);
+        b.sorted = c.take<int>(P);
+        b.ovf = c.take<int>(1);
+        b.dnl = c.take<int>((size_t)CMAXN * P);
+        b.dnc = c.take<int>(P);
+        b.dkg = c.take<KnnGrid>(1);
+    }
+    return b;
+}
+
+// Sorted neighbour lists of the cutoff form through the cell list on g^3 cells; *listed is false
+// when a point has more than CMAXN neighbours (the all-pairs scan runs instead).
+static int cutoff_lists(const CutoffBufs& b, int P, double box, double rlim, int g, hipStream_t s, bool* listed) {
+    *listed = false;
     const long ncell = (long)g * g * g;
-    int *cnt = nullptr, *start = nullptr, *sorted = nullptr, *ovf = nullptr;
-    int st = MCS_OK;
-    if (hipMalloc((void**)&cnt, (ncell + 1) * sizeof(int)) != hipSuccess ||
-        hipMalloc((void**)&start, (ncell + 1) * sizeof(int)) != hipSuccess ||
-        hipMalloc((void**)&sorted, (size_t)P * sizeof(int)) != hipSuccess ||
-        hipMalloc((void**)&ovf, sizeof(int)) != hipSuccess ||
-        hipMalloc((void**)dnl, (size_t)CMAXN * P * sizeof(int)) != hipSuccess ||
-        hipMalloc((void**)dnc, (size_t)P * sizeof(int)) != hipSuccess) {
-        mcs_set_last_error(__FILE__, __LINE__, "bond-order cell list allocation failed");
-        st = MCS_ERR_NOMEM;
-    }
+    KnnGrid kg{};
+    kg.ncell = (int)ncell;
+    const unsigned gp = (unsigned)std::min<long>(4096, (P + 255) / 256);
+    const unsigned gz = (unsigned)std::min<long>(1024, (ncell + 256) / 256);
+    MCS_HIP_TRY(hipMemcpyAsync(b.dkg, &kg, sizeof(KnnGrid), hipMemcpyHostToDevice, s));
+    MCS_HIP_TRY(hipMemsetAsync(b.ovf, 0, sizeof(int), s));
+    hipLaunchKernelGGL(k_knn_zero, dim3(gz), dim3(256), 0, s, b.dkg, b.cnt);
+    hipLaunchKernelGGL(k_cut_count, dim3(gp), dim3(256), 0, s, b.dx, P, box, g, b.cnt);
+    hipLaunchKernelGGL(k_knn_scan, dim3(1), dim3(1024), 0, s, b.dkg, b.cnt, b.start);
+    hipLaunchKernelGGL(k_cut_scatter, dim3(gp), dim3(256), 0, s, b.dx, P, box, g, b.start, b.cnt, b.sorted);
+    hipLaunchKernelGGL(k_cut_nbrs, dim3((unsigned)((P + 127) / 128)), dim3(128), 0, s, b.dx, P, box, rlim, g, b.start,
+                       b.sorted, b.dnl, b.dnc, b.ovf);
+    MCS_HIP_TRY(hipGetLastError());
     int overflow = 0;
-    if (!st) {
-        KnnGrid kg{};
-        kg.ncell = (int)ncell;
-        KnnGrid* dkg = nullptr;
-        const unsigned gp = (unsigned)std::min<long>(4096, (P + 255) / 256);
-        const unsigned gz = (unsigned)std::min<long>(1024, (ncell + 256) / 256);
-        if (hipMalloc((void**)&dkg, sizeof(KnnGrid)) != hipSuccess) st = MCS_ERR_NOMEM;
-        if (!st && hipMemcpyAsync(dkg, &kg, sizeof(KnnGrid), hipMemcpyHostToDevice, s) != hipSuccess) st = MCS_ERR_HIP;
-        if (!st && hipMemsetAsync(ovf, 0, sizeof(int), s) != hipSuccess) st = MCS_ERR_HIP;
-        if (!st) {
-            hipLaunchKernelGGL(k_knn_zero, dim3(gz), dim3(256), 0, s, dkg, cnt);
-            hipLaunchKernelGGL(k_cut_count, dim3(gp), dim3(256), 0, s, dx, P, box, g, cnt);
-            hipLaunchKernelGGL(k_knn_scan, dim3(1), dim3(1024), 0, s, dkg, cnt, start);
-            hipLaunchKernelGGL(k_cut_scatter, dim3(gp), dim3(256), 0, s, dx, P, box, g, start, cnt, sorted);
-            hipLaunchKernelGGL(k_cut_nbrs, dim3((unsigned)((P + 127) / 128)), dim3(128), 0, s, dx, P, box, rlim, g, start,
-                               sorted, *dnl, *dnc, ovf);
-            if (hipGetLastError() != hipSuccess) st = MCS_ERR_HIP;
-        }
-        if (!st && hipMemcpyAsync(&overflow, ovf, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess) st = MCS_ERR_HIP;
-        if (!st && hipStreamSynchronize(s) != hipSuccess) st = MCS_ERR_HIP;
-        if (dkg) (void)hipFree(dkg);
-        if (st == MCS_ERR_HIP) mcs_set_last_error(__FILE__, __LINE__, "HIP failure in the bond-order cell list");
-    }
-    (void)hipFree(cnt); (void)hipFree(start); (void)hipFree(sorted); (void)hipFree(ovf);
-    if (!st && !overflow) *listed = true;
-    return st;
+    MCS_HIP_TRY(hipMemcpyAsync(&overflow, b.ovf, sizeof(int), hipMemcpyDeviceToHost, s));
+    MCS_HIP_TRY(hipStreamSynchronize(s));
+    *listed = !overflow;
+    return MCS_OK;
 }
 
 extern "C" int mcs_bond_order_cutoff(const double* xyz, int npts, double box, double rlim, int* nb, double* out9) {
@@ -683,43 +704,39 @@ extern "C" int mcs_bond_order_cutoff(const double* xyz, int npts, double box, do
     if (npts == 0) return MCS_OK;
     MCS_CHECK(xyz && nb && out9, MCS_ERR_INVALID, "null argument");
     hipStream_t s = mcs_stream();
-    double *dx = nullptr, *dq = nullptr, *dout = nullptr;
-    int *dnb = nullptr, *dlast = nullptr, *dnl = nullptr, *dnc = nullptr;
-    int st = MCS_OK;
-    if (hipMalloc((void**)&dx, 3L * npts * sizeof(double)) != hipSuccess ||
-        hipMalloc((void**)&dq, (long)QS * npts * sizeof(double)) != hipSuccess ||
-        hipMalloc((void**)&dout, 9L * npts * sizeof(double)) != hipSuccess ||
-        hipMalloc((void**)&dnb, npts * sizeof(int)) != hipSuccess ||
-        hipMalloc((void**)&dlast, npts * sizeof(int)) != hipSuccess) {
-        mcs_set_last_error(__FILE__, __LINE__, "bond-order allocation failed");
-        st = MCS_ERR_NOMEM;
+    const int g = cutoff_grid(xyz, npts, box, rlim);
+    // one workspace, kept by the library between calls
+    Carve c;
+    cutoff_carve(c, npts, g);
+    void* ws = nullptr;
+    int st = mcs_workspace(MCS_WS_BOND, c.off, &ws);
+    if (st) return st;
+    c = Carve{reinterpret_cast<unsigned char*>(ws), 0};
+    const CutoffBufs b = cutoff_carve(c, npts, g);
+    const Wig4 W = wig4();
+    const unsigned gq = (unsigned)((npts + 127) / 128);
+    MCS_HIP_TRY(hipMemcpyAsync(b.dx, xyz, 3L * npts * sizeof(double), hipMemcpyHostToDevice, s));
+    bool listed = false;
+    if (g > 0) {
+        st = cutoff_lists(b, npts, box, rlim, g, s, &listed);
+        if (st) return st;
     }
-    if (!st) {
-        const Wig4 W = wig4();
-        const unsigned g = (unsigned)((npts + 127) / 128);
-        if (hipMemcpyAsync(dx, xyz, 3L * npts * sizeof(double), hipMemcpyHostToDevice, s) != hipSuccess) st = MCS_ERR_HIP;
-        bool listed = false;
-        if (!st) st = cutoff_lists(dx, xyz, npts, box, rlim, s, &listed, &dnl, &dnc);
-        if (!st && listed) {
-            hipLaunchKernelGGL(k_bo_qlm<true>, dim3(g), dim3(128), 0, s, dx, npts, box, rlim, dq, dnb, dlast, dnl, dnc);
-            hipLaunchKernelGGL(k_bo_stats<true>, dim3(g), dim3(128), 0, s, dx, npts, box, rlim, dq, dnb, dlast, W, dout,
-                               dnl, dnc);
-        } else if (!st) {
-            hipLaunchKernelGGL(k_bo_qlm<false>, dim3(g), dim3(128), 0, s, dx, npts, box, rlim, dq, dnb, dlast,
-                               (const int*)nullptr, (const int*)nullptr);
-            hipLaunchKernelGGL(k_bo_stats<false>, dim3(g), dim3(128), 0, s, dx, npts, box, rlim, dq, dnb, dlast, W, dout,
-                               (const int*)nullptr, (const int*)nullptr);
-        }
-        if (!st && hipGetLastError() != hipSuccess) st = MCS_ERR_HIP;
-        if (!st && hipMemcpyAsync(nb, dnb, npts * sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess) st = MCS_ERR_HIP;
-        if (!st && hipMemcpyAsync(out9, dout, 9L * npts * sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess)
-            st = MCS_ERR_HIP;
-        if (!st && hipStreamSynchronize(s) != hipSuccess) st = MCS_ERR_HIP;
-        if (st) mcs_set_last_error(__FILE__, __LINE__, "HIP failure in bond order");
+    if (listed) {
+        hipLaunchKernelGGL(k_bo_qlm<true>, dim3(gq), dim3(128), 0, s, b.dx, npts, box, rlim, b.dq, b.dnb, b.dlast, b.dnl,
+                           b.dnc);
+        hipLaunchKernelGGL(k_bo_stats<true>, dim3(gq), dim3(128), 0, s, b.dx, npts, box, rlim, b.dq, b.dnb, b.dlast, W,
+                           b.dout, b.dnl, b.dnc);
+    } else {
+        hipLaunchKernelGGL(k_bo_qlm<false>, dim3(gq), dim3(128), 0, s, b.dx, npts, box, rlim, b.dq, b.dnb, b.dlast,
+                           (const int*)nullptr, (const int*)nullptr);
+        hipLaunchKernelGGL(k_bo_stats<false>, dim3(gq), dim3(128), 0, s, b.dx, npts, box, rlim, b.dq, b.dnb, b.dlast, W,
+                           b.dout, (const int*)nullptr, (const int*)nullptr);
     }
-    (void)hipFree(dx); (void)hipFree(dq); (void)hipFree(dout); (void)hipFree(dnb); (void)hipFree(dlast);
-    (void)hipFree(dnl); (void)hipFree(dnc);
-    return st;
+    MCS_HIP_TRY(hipGetLastError());
+    MCS_HIP_TRY(hipMemcpyAsync(nb, b.dnb, npts * sizeof(int), hipMemcpyDeviceToHost, s));
+    MCS_HIP_TRY(hipMemcpyAsync(out9, b.dout, 9L * npts * sizeof(double), hipMemcpyDeviceToHost, s));
+    MCS_HIP_TRY(hipStreamSynchronize(s));
+    return MCS_OK;
 }
 
 extern "C" int mcs_bond_order_knn(const int* ijk, int npts, int n, double h, int k, int* nbr, double* out4,
@@ -731,39 +748,38 @@ extern "C" int mcs_bond_order_knn(const int* ijk, int npts, int n, double h, int
     if (npts == 0) return MCS_OK;
     MCS_CHECK(ijk && nbr && out4, MCS_ERR_INVALID, "null argument");
     hipStream_t s = mcs_stream();
-    int *dijk = nullptr, *dnbr = nullptr;
-    double* dout = nullptr;
-    unsigned long long* dh = nullptr;
-    void* ws = nullptr;
-    int st = MCS_OK;
     const int nb2 = hist_q4 ? 2 * nbins : 2;
-    if (hipMalloc((void**)&dijk, 3L * npts * sizeof(int)) != hipSuccess ||
-        hipMalloc((void**)&dnbr, (long)k * npts * sizeof(int)) != hipSuccess ||
-        hipMalloc((void**)&dout, 4L * npts * sizeof(double)) != hipSuccess ||
-        hipMalloc((void**)&dh, nb2 * sizeof(unsigned long long)) != hipSuccess ||
-        hipMalloc(&ws, knn_workspace_bytes(npts)) != hipSuccess) {
-        mcs_set_last_error(__FILE__, __LINE__, "bond-order allocation failed");
-        st = MCS_ERR_NOMEM;
-    }
+    // coordinates, outputs, histogram bins and the cell list in one workspace kept by the library
+    auto carve = [&](Carve& c, int** dijk, int** dnbr, double** dout, unsigned long long** dh, void** kws) {
+        *dijk = c.take<int>(3L * npts);
+        *dnbr = c.take<int>((long)k * npts);
+        *dout = c.take<double>(4L * npts);
+        *dh = c.take<unsigned long long>(nb2);
+        *kws = c.take<unsigned char>(knn_workspace_bytes(npts));
+    };
+    int *dijk, *dnbr;
+    double* dout;
+    unsigned long long* dh;
+    void* kws;
+    Carve c;
+    carve(c, &dijk, &dnbr, &dout, &dh, &kws);
+    void* ws = nullptr;
+    int st = mcs_workspace(MCS_WS_KNN, c.off, &ws);
+    if (st) return st;
+    c = Carve{reinterpret_cast<unsigned char*>(ws), 0};
+    carve(c, &dijk, &dnbr, &dout, &dh, &kws);
     std::vector<unsigned long long> hh(nb2);
-    if (!st) {
-        const int ext[3] = {n, n, n};
-        if (hipMemcpyAsync(dijk, ijk, 3L * npts * sizeof(int), hipMemcpyHostToDevice, s) != hipSuccess) st = MCS_ERR_HIP;
-        if (!st && hipMemsetAsync(dh, 0, nb2 * sizeof(unsigned long long), s) != hipSuccess) st = MCS_ERR_HIP;
-        if (!st) st = knn_launch(nullptr, dijk, nullptr, npts, npts, ext, h, k, ws, dnbr, dout,
-                                 hist_q4 ? dh : nullptr, hist_q4 ? dh + nbins : nullptr, nbins, s);
-        if (!st && hipMemcpyAsync(nbr, dnbr, (long)k * npts * sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess)
-            st = MCS_ERR_HIP;
-        if (!st && hipMemcpyAsync(out4, dout, 4L * npts * sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess)
-            st = MCS_ERR_HIP;
-        if (!st && hipMemcpyAsync(hh.data(), dh, nb2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s) != hipSuccess)
-            st = MCS_ERR_HIP;
-        if (!st && hipStreamSynchronize(s) != hipSuccess) st = MCS_ERR_HIP;
-        if (!st && hist_q4)
-            for (int b = 0; b < nbins; ++b) { hist_q4[b] = (int64_t)hh[b]; hist_q6[b] = (int64_t)hh[nbins + b]; }
-        if (st == MCS_ERR_HIP) mcs_set_last_error(__FILE__, __LINE__, "HIP failure in bond order");
-    }
-    if (st) (void)hipStreamSynchronize(s);
-    (void)hipFree(dijk); (void)hipFree(dnbr); (void)hipFree(dout); (void)hipFree(dh); (void)hipFree(ws);
-    return st;
+    const int ext[3] = {n, n, n};
+    MCS_HIP_TRY(hipMemcpyAsync(dijk, ijk, 3L * npts * sizeof(int), hipMemcpyHostToDevice, s));
+    MCS_HIP_TRY(hipMemsetAsync(dh, 0, nb2 * sizeof(unsigned long long), s));
+    st = knn_launch(nullptr, dijk, nullptr, npts, npts, ext, h, k, kws, dnbr, dout, hist_q4 ? dh : nullptr,
+                    hist_q4 ? dh + nbins : nullptr, nbins, s);
+    if (st) return st;
+    MCS_HIP_TRY(hipMemcpyAsync(nbr, dnbr, (long)k * npts * sizeof(int), hipMemcpyDeviceToHost, s));
+    MCS_HIP_TRY(hipMemcpyAsync(out4, dout, 4L * npts * sizeof(double), hipMemcpyDeviceToHost, s));
+    MCS_HIP_TRY(hipMemcpyAsync(hh.data(), dh, nb2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    MCS_HIP_TRY(hipStreamSynchronize(s));
+    if (hist_q4)
+        for (int b = 0; b < nbins; ++b) { hist_q4[b] = (int64_t)hh[b]; hist_q6[b] = (int64_t)hh[nbins + b]; }
+    return MCS_OK;
 }

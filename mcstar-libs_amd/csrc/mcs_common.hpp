@@ -67,4 +67,13 @@ template <int S, typename T> MCS_HD T cmuli(T a) {
 void mcs_set_last_error(const char* file, int line, const char* msg);
 hipStream_t mcs_stream();  // library stream (default priority)
 
+// Device workspaces the library keeps between calls instead of allocating per call: one
+// grow-only buffer per slot on the library's device.  A buffer is reallocated only when a call
+// needs more than it holds (after the stream has drained); its contents do not persist.  The
+// entry points that use a slot are synchronous on the library stream, so calls from one host
+// thread never overlap; calls from several host threads on the same slot must be serialised by
+// the caller, as for the library stream itself.
+enum { MCS_WS_PEAKS = 0, MCS_WS_KNN = 1, MCS_WS_BOND = 2, MCS_WS_SLOTS = 4 };
+int mcs_workspace(int slot, size_t bytes, void** out);
+
 static inline unsigned mcs_div_up(long a, long b) { return (unsigned)((a + b - 1) / b); }

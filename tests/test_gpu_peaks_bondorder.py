@@ -10,7 +10,8 @@ GOLD = os.path.join(os.path.dirname(__file__), "golden")
 
 
 @pytest.mark.parametrize("dt", [np.float32, np.float64])
-@pytest.mark.parametrize("shp", [(7, 9, 11), (64, 64, 64), (12, 10), (256, 256), (2, 2, 2), (1, 4, 4)])
+@pytest.mark.parametrize("shp", [(7, 9, 11), (64, 64, 64), (12, 10), (256, 256), (2, 2, 2), (1, 4, 4),
+                                 (33, 70, 130), (2, 3, 200), (40, 17, 65), (5, 1, 64), (3, 50, 1), (70, 33)])
 def test_peaks_match_oracle(mcs, orc, shp, dt):
     v = np.random.default_rng(sum(shp)).standard_normal(shp).astype(dt)
     for ths in (-10.0, 0.5):
@@ -33,6 +34,26 @@ def test_peaks_special_cases(mcs):
     with pytest.raises(mcs.McsError) as e:
         mcs.findpeaks(c, 0.5, cap=3)
     assert e.value.code == mcs.ERR_CAPACITY
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+def test_peaks_with_nan_neighbours(mcs, orc, dt):
+    # a NaN is never a peak and a cell next to one is not either (v > NaN is false), as in the
+    # reference's per-neighbour test
+    v = np.random.default_rng(5).standard_normal((20, 36, 70)).astype(dt)
+    v.reshape(-1)[np.random.default_rng(6).choice(v.size, 300, replace=False)] = np.nan
+    got = mcs.findpeaks(v, -10.0)
+    assert np.array_equal(got, orc.findpeaks(v.astype(np.float64), -10.0))
+
+
+def test_peaks_full_size_repeated_calls(mcs, orc):
+    # the workspace kept between calls: a large grid, then a smaller one, then the large one again
+    big = np.random.default_rng(1).standard_normal((96, 128, 256)).astype(np.float32)
+    small = np.random.default_rng(2).standard_normal((9, 40, 70)).astype(np.float32)
+    want_big = orc.findpeaks(big.astype(np.float64), 0.3)
+    assert np.array_equal(mcs.findpeaks(big, 0.3), want_big)
+    assert np.array_equal(mcs.findpeaks(small, 0.3), orc.findpeaks(small.astype(np.float64), 0.3))
+    assert np.array_equal(mcs.findpeaks(big, 0.3), want_big)
 
 
 def test_peaks_on_crystal_field(mcs, orc):
