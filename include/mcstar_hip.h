@@ -296,6 +296,18 @@ int mcs_ensemble_destroy(mcs_ensemble* e);
  * if count > cap). */
 int mcs_findpeaks(const void* real, const int ext[3], double ths, int64_t* idx_out, double* vals_out,
                   int64_t cap, int64_t* count, int prec);
+/* No counterpart in the reference: the allocation-free form of mcs_findpeaks, for
+ * callers that cannot allocate (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same peaks in the same order, written to
+ * DEVICE memory: idx_dev[cap] and vals_dev[cap] (vals_dev may be NULL); *count
+ * (host) gets the number found, MCS_ERR_CAPACITY when it exceeds cap.  The
+ * workspace (peak mask, counts, offsets) is a static 34 MiB device array that each
+ * GPU holds until the process ends; grids whose mask does not fit (more than
+ * about 2^28 cells when nx is a multiple of 64) return MCS_ERR_UNSUPPORTED before
+ * any launch.  Calls on one GPU share that array: calls that overlap on one GPU
+ * (several host threads) race and must be serialised by the caller. */
+int mcs_findpeaks_noalloc(const void* real, const int ext[3], double ths, int64_t* idx_dev, double* vals_dev,
+                          int64_t cap, int64_t* count, int prec);
 
 /* ---------------------------------------------------------------- bond order
  * bond_order_analysis<T> (bond-order-analysis/bond-order.cpp:289-349):
