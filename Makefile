@@ -29,8 +29,13 @@ $(PKG)/bin/mcs_sim_f64: $(PKG)/sim/mcs_sim.cpp $(LIB) $(wildcard include/*/*.hpp
 	$(HIPCC) -O2 -std=c++17 -DFLOAT=double -I include $< -o $@ -L$(PKG)/lib -lmcstar_hip -lz -Wl,-rpath,'$$ORIGIN/../lib'
 
 # C++ drop-in driver of the kept host API (tests/cpp), one build per precision
-DRV := tests/cpp/bin/kept_api_driver_f32 tests/cpp/bin/kept_api_driver_f64
+DRV := tests/cpp/bin/kept_api_driver_f32 tests/cpp/bin/kept_api_driver_f64 \
+       tests/cpp/bin/kept_api_exp_driver_f32 tests/cpp/bin/kept_api_exp_driver_f64
 driver: $(DRV)
+
+tests/cpp/bin/kept_api_exp_driver_%: tests/cpp/kept_api_exp_driver.cpp $(LIB) $(wildcard include/*/*.hpp include/*/*/*.hpp)
+	@mkdir -p $(dir $@)
+	$(HIPCC) -O2 -std=c++17 -DFLOAT=$(if $(filter f64,$*),double,float) -I include $< -o $@ -L$(PKG)/lib -lmcstar_hip -lz -Wl,-rpath,'$$ORIGIN/../../../$(PKG)/lib'
 
 tests/cpp/bin/kept_api_driver_f32: tests/cpp/kept_api_driver.cpp $(LIB) $(wildcard include/*/*.hpp include/*/*/*.hpp)
 	@mkdir -p $(dir $@)

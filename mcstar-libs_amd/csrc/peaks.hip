@@ -45,9 +45,10 @@ MCS_DEV int wrapi(int a, int n) {
     return a < 0 ? a + n : a;
 }
 
-// maxima that propagate NaN, so that a NaN neighbour fails the strict comparison as in the
-// reference's per-neighbour test (v > neighbour is false for a NaN neighbour)
-template <typename F> MCS_DEV F fmax2(F a, F b) { return (a > b || a != a) ? a : b; }
+// maxima that propagate NaN (IEEE 754-2019 maximum), so that a NaN neighbour fails the strict
+// comparison as in the reference's per-neighbour test (v > neighbour is false for a NaN
+// neighbour); in FP32 a pair of them is one v_maximum3_f32 on gfx950
+template <typename F> MCS_DEV F fmax2(F a, F b) { return __builtin_elementwise_maximum(a, b); }
 template <typename F> MCS_DEV F fmax3(F a, F b, F c) { return fmax2(fmax2(a, b), c); }
 
 // tile order: every XCD (workgroup b runs on XCD b % 8) takes a contiguous run of tiles, so the
@@ -80,11 +81,13 @@ __global__ void __launch_bounds__(PBS) k_peak_mark(const F* __restrict__ v, int 
         const int r = i / PPW, c = i % PPW;
         off[k] = (long)wrapi(y0 - 1 + r, ny) * nx + wrapi(x0 - 1 + c, nx);
     }
+    // the next plane in flight in registers while the current one is read from LDS (two planes in
+    // flight were slower: 293 against 245 us at 512^3)
     F pre[PLOADS];
-    auto load_plane = [&](int z) {
+    auto load_plane = [&](F(&dst)[PLOADS], int z) {
         const long pb = (long)wrapi(z, nz) * plane;
 #pragma unroll
-        for (int k = 0; k < PLOADS; ++k) pre[k] = slot[k] >= 0 ? v[pb + off[k]] : F(0);
+        for (int k = 0; k < PLOADS; ++k) dst[k] = slot[k] >= 0 ? v[pb + off[k]] : F(0);
     };
     auto store_plane = [&](int buf) {
         F* p = &patch[buf][0][0];
@@ -99,10 +102,10 @@ __global__ void __launch_bounds__(PBS) k_peak_mark(const F* __restrict__ v, int 
     const bool xok = x0 + lane < nx;
     // plane z0 - 1 (3D) first, then z0 .. z1 - 1, then z1 (3D): the first and last give P9 only
     const int zs = is3d ? z0 - 1 : z0, ze = is3d ? z1 : z1 - 1;
-    load_plane(zs);
+    load_plane(pre, zs);
     store_plane(0);
     __syncthreads();
-    if (zs + 1 <= ze) load_plane(zs + 1);
+    if (zs + 1 <= ze) load_plane(pre, zs + 1);
     for (int z = zs; z <= ze; ++z) {
         const int buf = (z - zs) & 1;
         F vn[4], m8n[4], p9n[4];
@@ -149,7 +152,7 @@ __global__ void __launch_bounds__(PBS) k_peak_mark(const F* __restrict__ v, int 
         if (z + 1 <= ze) {
             store_plane(buf ^ 1);  // the buffer of plane z - 1, which every thread has finished reading
             __syncthreads();
-            if (z + 2 <= ze) load_plane(z + 2);
+            if (z + 2 <= ze) load_plane(pre, z + 2);
         }
     }
 }

@@ -141,3 +141,103 @@ def test_relaxation_flow(orc, tmp_path, prec, dt, tol):
     got = np.load(fout)
     assert np.abs(got - p).max() < tol * np.abs(p).max() * 10
     assert maxes[-1] < maxes[0]
+
+
+def exp_drv(prec):
+    p = os.path.join(REPO, "tests", "cpp", "bin", f"kept_api_exp_driver_{prec}")
+    if not os.path.exists(p):
+        pytest.fail(f"{p} not built (make driver)")
+    return p
+
+
+def run_exp(prec, *args):
+    r = subprocess.run([exp_drv(prec)] + [str(a) for a in args], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    return r
+
+
+# the ETD tables take expf in both builds (eom-exp.hpp:7, el-exp.hpp:6), and the device's and the host's
+# expf may differ by an ulp of float, so the FP64 build agrees with the oracle to float rounding of the tables
+@pytest.mark.parametrize("prec,dt,tol", [("f32", np.float32, 2e-5), ("f64", np.float64, 1e-7)])
+@pytest.mark.parametrize("shp", [(12, 16, 20), (24, 32)])
+def test_etd_flow_through_kept_exp_headers(orc, tmp_path, prec, dt, tol, shp):
+    """eom-exp.hpp: init_eom_op once, then c2r -> psi3_split(norm) -> r2c -> update_eom (ETD) per step"""
+    psi = orc.init_field(shp, -0.25, 0.1, 4).astype(dt)
+    fin, fout = str(tmp_path / "in.npy"), str(tmp_path / "out.npy")
+    np.save(fin, psi)
+    ext = list(shp[::-1]) + [1] * (3 - len(shp))
+    steps, eps, dtt, h = 12, 0.3, 0.5, np.pi / 4
+    run_exp(prec, "etd", *ext, fin, fout, steps, eps, dtt)
+    lens = [e * h if e > 1 else 1.0 for e in ext]
+    kop = orc.etd_op(shp, lens, dtt, eps, 0)
+    N = psi.size
+    kp = orc.r2c(psi.astype(np.float64))
+    for _ in range(steps):
+        r = orc.c2r(kp, shp) / N
+        kp = orc.update_eom_etd(kop, kp, orc.r2c(r ** 3 - r), None, shp, lens, 0.0)
+    want = orc.c2r(kp, shp) / N
+    got = np.load(fout)
+    assert np.linalg.norm(got - want) / np.linalg.norm(want) < tol
+
+
+@pytest.mark.parametrize("prec,dt,tol", [("f32", np.float32, 1e-4), ("f64", np.float64, 1e-7)])
+def test_el_exp_flow_through_kept_exp_headers(orc, tmp_path, prec, dt, tol):
+    """el-exp.hpp: init_el_op, then the non-conserved ETD relaxation and its max |change| per step"""
+    shp = (20, 24, 16)
+    psi = orc.init_field(shp, -0.3, 0.2, 9).astype(dt)
+    fin, fout = str(tmp_path / "in.npy"), str(tmp_path / "out.npy")
+    np.save(fin, psi)
+    ext = list(shp[::-1])
+    steps, eps, dtt, h = 10, 0.3, 0.3, np.pi / 4
+    r = run_exp(prec, "elexp", *ext, fin, fout, steps, eps, dtt)
+    maxes = np.array([float(v) for v in r.stdout.split()])
+    lens = [e * h for e in ext]
+    kop = orc.etd_op(shp, lens, dtt, eps, 1)
+    N = psi.size
+    kp = orc.r2c(psi.astype(np.float64))
+    want_m = []
+    for _ in range(steps):
+        p = orc.c2r(kp, shp) / N
+        kp, m = orc.update_el_etd(kop, kp, orc.r2c(p ** 3 - p), shp)
+        want_m.append(m)
+    assert maxes.size == steps and np.abs(maxes - want_m).max() < tol * max(want_m) * 10
+    want = orc.c2r(kp, shp) / N
+    assert np.linalg.norm(np.load(fout) - want) / np.linalg.norm(want) < tol
+
+
+def test_findpeaks_2d_flow(orc, tmp_path):
+    """call_kernel_findpeaks_2d (findpeaks_2d.hpp:29-56): 8-neighbour maxima as "%f %f %f %f" lines"""
+    n1, n0 = 48, 64
+    y = np.arange(n1)[:, None] * 2 * np.pi / 8
+    x = np.arange(n0)[None, :] * 2 * np.pi / 8
+    f = np.cos(x) + np.cos(y) + 0.01 * np.random.default_rng(3).standard_normal((n1, n0))
+    fin, pk = str(tmp_path / "f.npy"), str(tmp_path / "peaks.txt")
+    np.save(fin, f)
+    h = 0.5
+    r = run_exp("f64", "peaks2d", n0, n1, fin, pk, h, 0.5)
+    idx = orc.findpeaks(f, 0.5)
+    assert idx.size > 0 and f"found {idx.size} maxima" in r.stderr
+    rows = np.loadtxt(pk, ndmin=2)
+    assert rows.shape == (idx.size, 4)
+    assert np.allclose(rows[:, 0], (idx % n0) * h, atol=1e-6) and np.allclose(rows[:, 1], (idx // n0) * h, atol=1e-6)
+    assert np.all(rows[:, 2] == 0) and np.allclose(rows[:, 3], f.reshape(-1)[idx], atol=1e-6)
+
+
+@pytest.mark.parametrize("prec,dt", [("f32", np.float32), ("f64", np.float64)])
+def test_potential_into_psi3_split(tmp_path, prec, dt):
+    """init_pot_line (potentials.hpp:5-20) uploaded and added by psi3_split (psi3_split.hpp:1-6):
+    a <- a^3 - a - mu + pot with mu = 0.1"""
+    n1, n0, h, power, sigma, amp = 24, 32, 0.5, 2, 2.0, 1.5
+    psi = np.random.default_rng(4).uniform(-0.5, 0.5, (n1, n0)).astype(dt)
+    fin, fout = str(tmp_path / "in.npy"), str(tmp_path / "out.npy")
+    np.save(fin, psi)
+    run_exp(prec, "pot", n0, n1, fin, fout, h, power, sigma, amp)
+    got = np.load(fout).astype(np.float64)
+    x = np.arange(n0)[None, :] * h - n0 * h / 2
+    yy = np.arange(n1)[:, None] * h - n1 * h / 2
+    f = 2 * np.pi / sigma
+    w = (1 + np.tanh(yy + sigma / 2)) * (1 + np.tanh(sigma / 2 - yy)) / 4
+    pot = amp * ((np.cos(f * x) + np.cos(f * yy) + 2) / 4) ** power * w
+    p = psi.astype(np.float64)
+    want = p ** 3 - p - 0.1 + pot.astype(dt).astype(np.float64)
+    assert np.abs(got - want).max() < (2e-6 if dt == np.float32 else 1e-14)

@@ -69,4 +69,5 @@ def test_init_pot_line(helper):
 def test_driver_builds_against_kept_headers():
     subprocess.run(["make", "-s", "driver"], cwd=REPO, check=True)
     for p in ("f32", "f64"):
-        assert os.path.exists(os.path.join(REPO, "tests", "cpp", "bin", f"kept_api_driver_{p}"))
+        for d in ("kept_api_driver", "kept_api_exp_driver"):
+            assert os.path.exists(os.path.join(REPO, "tests", "cpp", "bin", f"{d}_{p}"))
