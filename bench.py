@@ -192,9 +192,9 @@ def analysis_bench(mcs, st, field, h):
                                     "bond_order_analysis), cell list"}}
 
 
-def fft_bench(mcs, n, reps):
-    """boundary FFT (the FFT::CufftPlan replacement) R2C and C2R on an n^3 FP32 field"""
-    shp = (n, n, n)
+def fft_bench(mcs, n, reps, shp=None):
+    """boundary FFT (the FFT::CufftPlan replacement) R2C and C2R on an n^3 (or shp) FP32 field"""
+    shp = tuple(shp) if shp else (n, n, n)
     nb = mcs.buffer_bytes(shp, np.float32)
     x = mcs.DeviceBuffer(nb)
     mcs.check(mcs.lib().mcs_init_field(x.ptr, mcs._ext(shp), 0.0, 1.0, 3, 32), "mcs_init_field")
@@ -204,8 +204,9 @@ def fft_bench(mcs, n, reps):
     inv.execute(k, y)
     out = {}
     # algorithmic bytes: x pass reads/writes the real field and the half spectrum, y and z read/write the spectrum
-    half = (n // 2 + 1) * n * n * 8
-    byts = (n ** 3 * 4 + half) + 2 * 2 * half
+    ncell = int(np.prod(shp))
+    half = (shp[-1] // 2 + 1) * (ncell // shp[-1]) * 8
+    byts = (ncell * 4 + half) + 2 * (len(shp) - 1) * half
     for name, plan, a, b in (("r2c", fwd, x, k), ("c2r", inv, k, y)):
         mcs.sync()
         with mcs.Timer() as t:
@@ -259,7 +260,8 @@ def main():
         print(json.dumps({"ensemble_2d": ensemble_bench(mcs, "2d", 40), "ensemble_3d": ensemble_bench(mcs, "3d", 16)}))
         return
     if a.only_fft:
-        print(json.dumps({"boundary_fft_512": fft_bench(mcs, a.n, 5)}))
+        print(json.dumps({"boundary_fft_512": fft_bench(mcs, a.n, 5),
+                          "boundary_fft_1200sq": fft_bench(mcs, 1200, 20, shp=(1200, 1200))}))
         return
     n = a.n
     cfg = dict(eps=0.3, psi0=-0.25, amp=0.05, seed=11, dx=np.pi / 4, dt=0.5)
@@ -340,6 +342,9 @@ def main():
     log("[bench] boundary FFT")
     if not a.no_fft:
         out["boundary_fft"] = fft_bench(mcs, n, 5)
+        # a length of the reference's own FFT tests (cufft_unittest/my_unit_test.py:161-163) that the
+        # size policy (cuda/math.hpp:139-149) yields: 1200 = 3 5^2 2^4 (30-point column plan)
+        out["boundary_fft_1200sq"] = dict(fft_bench(mcs, 1200, 20, shp=(1200, 1200)), shape=[1200, 1200])
     if not a.no_ensemble:
         log("[bench] ensembles")
         out["ensemble_2d"] = ensemble_bench(mcs, "2d", 40)

@@ -179,6 +179,7 @@ static int ens_col_nl_t(mcs_ensemble* e, EnsGroup* g, const LinePlan& lp, const 
         case 20: { constexpr int PP = 20; return CALL; }                                    \
         case 16: { constexpr int PP = 16; return CALL; }                                    \
         case 32: { constexpr int PP = 32; return CALL; }                                    \
+        case 30: { constexpr int PP = 30; return CALL; }                                    \
         default: mcs_set_last_error(__FILE__, __LINE__, "no register plan"); return MCS_ERR_UNSUPPORTED; \
     }
 

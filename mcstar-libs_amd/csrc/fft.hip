@@ -223,6 +223,7 @@ static int launch_col_t(const LinePlan& lp, const void* src, void* dst, const Co
             case 20: return launch_col_reg_t<F, 20, DIR>(lp, src, dst, g, s);
             case 16: return launch_col_reg_t<F, 16, DIR>(lp, src, dst, g, s);
             case 32: return launch_col_reg_t<F, 32, DIR>(lp, src, dst, g, s);
+            case 30: return launch_col_reg_t<F, 30, DIR>(lp, src, dst, g, s);
             default: break;
         }
     }

@@ -41,22 +41,24 @@ static bool reg_plan_pow2(int L, int P, int prec, std::vector<int>* rad) {
     return true;
 }
 
-// 3*2^a with 6 points per thread or 5*2^a with 10: the radix-3 or radix-5 pass, then radix-2
+// 3^b*2^a with 6 points per thread or 5^c*2^a with 10: the radix-3 or radix-5 passes, then radix-2
 // passes (twice the threads of the 12- and 20-point plans, for small grids)
 static bool reg_plan_small(int L, int P, int prec, std::vector<int>* rad) {
     int a, b, c;
-    if (L < 6 || factor235(L, a, b, c) != 1 || a < 1 || a + 1 > MCS_MAX_PASSES) return false;
-    const bool three = P == 6 && b == 1 && c == 0, five = P == 10 && b == 0 && c == 1;
-    if (!(three || five) || L / P > mcs_reg_max_threads(P, prec)) return false;
-    std::vector<int> r{three ? 3 : 5};
+    if (L < 6 || factor235(L, a, b, c) != 1 || a < 1) return false;
+    const bool three = P == 6 && b >= 1 && c == 0, five = P == 10 && b == 0 && c >= 1;
+    if (!(three || five) || a + b + c > MCS_MAX_PASSES || L / P > mcs_reg_max_threads(P, prec)) return false;
+    std::vector<int> r;
+    for (int i = 0; i < (three ? b : c); ++i) r.push_back(three ? 3 : 5);
     for (int i = 0; i < a; ++i) r.push_back(2);
     if (rad) *rad = r;
     return true;
 }
 
 // The default plan.  Powers of two take in-register radix-16/32 butterflies so that 256, 512
-// and 1024 need a single LDS exchange per transform; 3*2^a and 5*2^a use one radix-3/5 pass and
-// radix-4 passes.  FP64 keeps P = 8 for powers of two (radix-8 passes): larger register blocks of
+// and 1024 need a single LDS exchange per transform; 3^b*2^a and 5^c*2^a use radix-3/5 passes and
+// radix-4 passes (12 and 20 points), and lengths with both 3 and 5 radix-3, -5 and -2 passes on
+// 30 points (the sizes cuda/math.hpp:139-149's roundup235 yields, as 1200 = 3 5^2 2^4).  FP64 keeps P = 8 for powers of two (radix-8 passes): larger register blocks of
 // doubles drop the kernels to one wavefront per SIMD.  Where that plan's line would exceed the
 // launch bound, the next plan within it is taken (32 points for 8192 in FP32, 6 or 10 points for
 // the long 3*2^a and 5*2^a lines); none within it means no register plan.
@@ -83,18 +85,23 @@ static int reg_plan(int L, int prec, std::vector<int>* rad, int max_p = 32) {
             if (a % 4 == 3) r.push_back(8);
             for (int i = 0; i < a / 4; ++i) r.push_back(16);
         }
-    } else if (b == 1 && c == 0 && a >= 2) {
+    } else if (b >= 1 && c == 0 && a >= 2) {  // 3^b 2^a: radix-3 passes, then radix-4 (and one radix-2)
         P = 12;
-        r.push_back(3);
+        for (int i = 0; i < b; ++i) r.push_back(3);
         if (a % 2 == 1) r.push_back(2);
         for (int i = 0; i < a / 2; ++i) r.push_back(4);
-    } else if (b == 0 && c == 1 && a >= 2) {
+    } else if (b == 0 && c >= 1 && a >= 2) {  // 5^c 2^a: radix-5 passes, then radix-4 (and one radix-2)
         P = 20;
-        r.push_back(5);
+        for (int i = 0; i < c; ++i) r.push_back(5);
         if (a % 2 == 1) r.push_back(2);
         for (int i = 0; i < a / 2; ++i) r.push_back(4);
+    } else if (b >= 1 && c >= 1 && a >= 1) {  // 3^b 5^c 2^a: 30 points, which radices 2, 3 and 5 divide
+        P = 30;
+        for (int i = 0; i < b; ++i) r.push_back(3);
+        for (int i = 0; i < c; ++i) r.push_back(5);
+        for (int i = 0; i < a; ++i) r.push_back(2);
     }
-    if (P == 0) return 0;
+    if (P == 0 || (int)r.size() > MCS_MAX_PASSES) return 0;
     if (L / P > mcs_reg_max_threads(P, prec)) {
         P = 0;
         for (int Q : {32, 16, 8}) {

@@ -107,6 +107,7 @@ static int launch_row_reg_d(const LinePlan& lp, const void* src, void* dst, long
         case 20: return launch_row_reg_t<F, 20, DIR>(lp, src, dst, nrows, s);
         case 16: return launch_row_reg_t<F, 16, DIR>(lp, src, dst, nrows, s);
         case 32: return launch_row_reg_t<F, 32, DIR>(lp, src, dst, nrows, s);
+        case 30: return launch_row_reg_t<F, 30, DIR>(lp, src, dst, nrows, s);
         default: mcs_set_last_error(__FILE__, __LINE__, "row plan not register-eligible"); return MCS_ERR_UNSUPPORTED;
     }
 }
@@ -146,6 +147,7 @@ static int launch_col_nl_d(const LinePlan& lp, void* W, const ColGeom& g, double
         case 20: return launch_col_nl_t<F, 20, SPLIT>(lp, W, g, scale, d_step, s);
         case 16: return launch_col_nl_t<F, 16, SPLIT>(lp, W, g, scale, d_step, s);
         case 32: return launch_col_nl_t<F, 32, SPLIT>(lp, W, g, scale, d_step, s);
+        case 30: return launch_col_nl_t<F, 30, SPLIT>(lp, W, g, scale, d_step, s);
         default: mcs_set_last_error(__FILE__, __LINE__, "column plan not register-eligible"); return MCS_ERR_UNSUPPORTED;
     }
 }
@@ -201,6 +203,7 @@ static int launch_xupd_d(const StepGeom& sg, void* W, void* U, const XArgs<F>& a
         case 20: return launch_xupd_t<F, 20, SPLIT>(sg, W, U, ar, s);
         case 16: return launch_xupd_t<F, 16, SPLIT>(sg, W, U, ar, s);
         case 32: return launch_xupd_t<F, 32, SPLIT>(sg, W, U, ar, s);
+        case 30: return launch_xupd_t<F, 30, SPLIT>(sg, W, U, ar, s);
         default: mcs_set_last_error(__FILE__, __LINE__, "x plan not register-eligible"); return MCS_ERR_UNSUPPORTED;
     }
 }

@@ -134,3 +134,14 @@ def test_in_place_r2c_is_rejected(mcs):
     c2r = mcs.FFTPlan(shp, np.float32, mcs.C2R)
     c2r.execute(buf, buf)
     assert np.abs(buf.download(shp, np.float32) / x.size - x).max() < 1e-5
+
+
+@pytest.mark.parametrize("shp", [(360, 36, 24), (90, 150, 72), (15, 18, 50), (1200, 30)])
+@pytest.mark.parametrize("dt,tol", [(np.float32, 1e-5), (np.float64, 2e-14)])
+def test_mixed_radix_columns(mcs, shp, dt, tol):
+    """columns of lengths 3^b 5^c 2^a (30-, 12- and 20-point register plans) against numpy"""
+    x = np.random.default_rng(sum(shp)).standard_normal(shp).astype(dt)
+    k = mcs.rfftn(x)
+    assert np.abs(np.fft.rfftn(x.astype(np.float64)) - k).max() / np.sqrt(x.size) < tol
+    y = mcs.irfftn(k, shp)
+    assert np.abs(x.astype(np.float64) - y.astype(np.float64) / x.size).max() < tol

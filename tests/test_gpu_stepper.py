@@ -103,3 +103,26 @@ def test_unsupported_sizes(mcs):
         with pytest.raises(mcs.McsError) as e:
             mcs.Stepper(shp, [H] * 3, 0.3, 0.5)
         assert e.value.code == mcs.ERR_UNSUPPORTED
+
+
+# the sizes cuda/math.hpp:139-149's roundup235 yields beyond 2^a, 3*2^a and 5*2^a, among them three of the
+# reference's FFT test sizes (cufft_unittest/my_unit_test.py:161-163): nx/2, ny, nz of the forms 9*2^a
+# (1152 -> 576 and 1152: 12-point plans), 25*2^a (1600 -> 800 and 1600: 20 points) and 3*5^2*2^a or
+# 3^2*5*2^a (1200 -> 600 and 1200, 360 -> 180 and 360: 30-point plans)
+@pytest.mark.parametrize("shp,steps", [((1152, 1152), 6), ((1200, 1200), 6), ((1600, 1600), 4), ((360, 360, 360), 3),
+                                       ((90, 150, 72), 8)])
+def test_mixed_radix_sizes_match_oracle(mcs, orc, shp, steps):
+    for L in set(shp) | {shp[-1] // 2}:
+        P = mcs.plan_options(L, np.float32)[0]
+        assert L % P == 0 and P in (6, 10, 12, 20, 30), (L, P)
+    got, want, psi, st = run_pair(mcs, orc, shp, 0.3, 0.5, -0.25, 0.05, 11, steps)
+    assert rel_l2(got, want) < 1e-12
+    g32, w32, _, st32 = run_pair(mcs, orc, shp, 0.3, 0.5, -0.25, 0.05, 11, steps, dtype=np.float32)
+    assert rel_l2(g32, w32) < 5e-6
+    assert abs(st32.mass() - psi.astype(np.float32).astype(np.float64).mean()) < 1e-15
+
+
+def test_mixed_radix_with_noise_fp32(mcs, orc):
+    got, want, psi, st = run_pair(mcs, orc, (60, 72, 120), 0.3, 0.25, -0.2, 0.1, 7, 10, temp=1e-3, noise_seed=3,
+                                  dtype=np.float32)
+    assert rel_l2(got, want) < 5e-5
