@@ -105,6 +105,10 @@ def ensemble_bench(mcs, kind, steps):
         out[name] = {"ms_per_step": t.ms / steps, "replica_steps_per_s": len(reps) * steps / (t.ms / 1e3),
                      "cell_updates_per_s": cells * steps / (t.ms / 1e3)}
     out["speedup_graph_over_loop"] = out["graph"]["replica_steps_per_s"] / out["loop"]["replica_steps_per_s"]
+    out["launches"] = e.launch_info()
+    # algorithmic HBM bytes per step: 40 B per cell in 3D (10 sweeps of 4 B), 24 B in 2D (6 sweeps)
+    bpc = 40 if kind == "3d" else 24
+    out["graph"]["algorithmic_GB_per_s"] = round(cells * bpc / (out["graph"]["ms_per_step"] / 1e3) / 1e9, 1)
     if kind == "3d":
         # mcs_ensemble_run: the graph-replayed steps with every replica's peaks (all local maxima) and
         # 12-NN q4/q6 64-bin histograms captured after the last step; wall clock including the downloads

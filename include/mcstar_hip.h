@@ -249,10 +249,10 @@ int mcs_timer_stop(double* ms);
 
 /* ---------------------------------------------------------------- ensembles
  * Independent replicas (own grid size, eps, dt, temperature, initial field and
- * noise seed) advanced together.  Replicas of equal size share plans and are
- * stepped by one launch per pass (persistent workgroups over a (replica, work
- * unit) table); all replicas take the same number of steps and share the
- * Philox draw number.  Same size rules as the stepper.  No counterpart in the
+ * noise seed) advanced together.  Replicas of equal size share plans; the size
+ * groups whose plans for a pass have the same points per thread are stepped by
+ * one launch (persistent workgroups over a (group, replica, work unit) table);
+ * all replicas take the same number of steps and share the Philox draw number.  Same size rules as the stepper.  No counterpart in the
  * reference (one replica per process there). */
 typedef struct {
     int ext[3];
@@ -264,9 +264,13 @@ typedef struct {
 } mcs_replica;
 typedef struct mcs_ensemble mcs_ensemble;
 #define MCS_ENS_LOOP 0    /* replica by replica, single-grid launches (the looping baseline) */
-#define MCS_ENS_BATCHED 1 /* one launch per pass and size group */
-#define MCS_ENS_GRAPH 2   /* replayed from a captured HIP graph of 8 steps in which every size group is an
-                             independent branch (the groups' kernels overlap); remainder steps run batched */
+#define MCS_ENS_BATCHED 1 /* eager: one launch per pass and plan class (the size groups whose plans for that
+                             pass fall in one register tier share one persistent work-table launch) or one
+                             launch per pass and size group, whichever ran faster when created */
+#define MCS_ENS_GRAPH 2   /* replayed from a captured HIP graph of 8 steps: either the plan-class launches
+                             (the classes of one pass on parallel branches) or one branch per size group,
+                             whichever replayed faster when the ensemble was created (both compute the same
+                             bits); remainder steps run batched */
 int mcs_ensemble_create(mcs_ensemble** e, const mcs_replica* reps, int nrep, int split, int prec);
 /* seeded initial fields for every replica */
 int mcs_ensemble_init(mcs_ensemble* e);
@@ -286,6 +290,17 @@ int mcs_ensemble_analyze(mcs_ensemble* e, double ths, int k, int nbins, int64_t*
 int mcs_ensemble_run(mcs_ensemble* e, int nsteps, double ths, int k, int nbins, int64_t* counts, int64_t* hist_q4,
                      int64_t* hist_q6);
 int mcs_ensemble_destroy(mcs_ensemble* e);
+/* Instrumentation (no reference counterpart): the number of size groups and of plan-class launches
+ * per step; whether batched steps launch the classes (1) or per size group (0), and whether the step
+ * graph replays the class launches (1) or one branch per size group (0); and the milliseconds per
+ * step each form took when the ensemble was created: {batched per group, batched classes, graph per
+ * group, graph classes} (-1 where not timed). */
+int mcs_ensemble_launch_info(mcs_ensemble* e, int* groups, int* classes_per_step, int* batched_classes,
+                             int* graph_classes, double ms_per_step[4]);
+/* Pins the launch forms instead of the timed choice (for pinned runs and tests): batched steps and
+ * the step graph through the plan classes (1) or per size group (0); the results are the same bits
+ * either way.  MCS_ERR_UNSUPPORTED when the ensemble has no classes. */
+int mcs_ensemble_set_launch_form(mcs_ensemble* e, int batched_classes, int graph_classes);
 
 /* ---------------------------------------------------------------- peaks
  * call_kernel_findpeaks / _2d (cuda/kernels/findpeaks.hpp:7-63,

@@ -4,6 +4,8 @@
 // (RunPlan) or a fixed plan.
 #pragma once
 
+#include <type_traits>
+
 #include "stepper.hpp"
 
 template <typename F> struct EnsRep {
@@ -65,3 +67,126 @@ __global__ void __launch_bounds__(MCS_REG_MAX_THREADS(P, F)) k_ens_col_nl(const 
     }
 }
 
+
+// ------------------------------------------------------------------ plan-class kernels
+// One launch covers every size group of one pass whose register plan falls in the same tier of
+// points per thread (below): the groups sit one after another in the launch's unit space
+// ([first, total) each), every group with its own runtime line plan, twiddles, geometry and
+// replicas, and G lines (row pairs) per workgroup chosen so that G * (L / P) is the launch's one
+// workgroup size (at most 256 threads).
+template <typename F> struct EnsClassGroup {
+    FftPlanDev pl;
+    const C2<F>* tw;
+    const EnsRep<F>* reps;
+    const int2* pairs;  // x pass
+    const C2<F>* wph;   // x pass
+    ColGeom g;          // column passes
+    F scale;            // nonlinear pass
+    int npairs;
+    int P;              // points per thread of this group's plan
+    int G;
+    int units;          // work units per replica
+    long first, total;  // this group's units in the launch
+};
+
+template <typename F> MCS_DEV int cls_p(const EnsClassGroup<F>& c) { return c.P; }
+
+// the group of unit `it`; a workgroup visits increasing units, so the search only moves forward
+template <typename F> MCS_DEV int cls_group(const EnsClassGroup<F>* cg, int gi, long it) {
+    while (it >= cg[gi].total) ++gi;
+    return gi;
+}
+
+// The points per thread of a class come in two register tiers, so that one launch can hold groups
+// of different plans without giving the small plans the register budget of the large ones: tier 0
+// takes P in {4, 6, 8, 10, 12}, tier 1 P in {16, 20, 30, 32}.  The body for the group's P is
+// selected at run time (every thread of a workgroup takes the same branch).
+inline int cls_tier(int P) { return P <= 12 ? 0 : 1; }
+template <int TIER, typename Fn> MCS_DEV void with_p(int P, Fn&& f) {
+    if constexpr (TIER == 0) {
+        switch (P) {
+            case 4: f(std::integral_constant<int, 4>{}); break;
+            case 6: f(std::integral_constant<int, 6>{}); break;
+            case 8: f(std::integral_constant<int, 8>{}); break;
+            case 10: f(std::integral_constant<int, 10>{}); break;
+            case 12: f(std::integral_constant<int, 12>{}); break;
+            default: break;
+        }
+    } else {
+        switch (P) {
+            case 16: f(std::integral_constant<int, 16>{}); break;
+            case 20: f(std::integral_constant<int, 20>{}); break;
+            case 30: f(std::integral_constant<int, 30>{}); break;
+            case 32: f(std::integral_constant<int, 32>{}); break;
+            default: break;
+        }
+    }
+}
+
+// the x pass's update does not depend on the split (the operator sits in the replica's arguments)
+template <typename F, int TIER, bool NOISE>
+__global__ void __launch_bounds__(MCS_XPASS_MAX_THREADS) k_cls_xupd(const EnsClassGroup<F>* __restrict__ cg, long total,
+                                                                    const int64_t* __restrict__ d_step, int draw_off) {
+    extern __shared__ unsigned char smem_raw[];
+    C2<F>* lds = reinterpret_cast<C2<F>*>(smem_raw);
+    const uint32_t draw = NOISE ? (uint32_t)(*d_step + draw_off) : 0u;
+    int gi = 0;
+    for (long it = blockIdx.x; it < total; it += gridDim.x) {
+        gi = cls_group(cg, gi, it);
+        const EnsClassGroup<F>& c = cg[gi];
+        const long rel = it - c.first;
+        const EnsRep<F>& R = c.reps[rel / c.units];
+        const int u = (int)(rel % c.units);
+        // a row pair whose S = L / P threads sit in one wavefront exchanges through LDS with
+        // wavefront fences instead of workgroup barriers, as in the per-group kernels
+        const bool wave = 64 % (c.pl.L / c.P) == 0;
+        with_p<TIER>(cls_p(c), [&](auto PC) {
+            if (wave)
+                xupd_body<F, decltype(PC)::value, MCS_SPLIT_NORTHSTAR, true, RunPlan, NOISE>(
+                    R.W, R.U, c.pairs, c.npairs, c.G, R.xa, c.pl, c.tw, c.wph, draw, lds, u, opaque_tid());
+            else
+                xupd_body<F, decltype(PC)::value, MCS_SPLIT_NORTHSTAR, false, RunPlan, NOISE>(
+                    R.W, R.U, c.pairs, c.npairs, c.G, R.xa, c.pl, c.tw, c.wph, draw, lds, u, opaque_tid());
+        });
+        __syncthreads();
+    }
+}
+
+template <typename F, int TIER, int DIR>
+__global__ void __launch_bounds__(256) k_cls_col(const EnsClassGroup<F>* __restrict__ cg, long total) {
+    extern __shared__ unsigned char smem_raw[];
+    C2<F>* lds = reinterpret_cast<C2<F>*>(smem_raw);
+    int gi = 0;
+    for (long it = blockIdx.x; it < total; it += gridDim.x) {
+        gi = cls_group(cg, gi, it);
+        const EnsClassGroup<F>& c = cg[gi];
+        const long rel = it - c.first;
+        C2<F>* W = c.reps[rel / c.units].W;
+        const int u = (int)(rel % c.units);
+        const int tilesx = (c.g.nxc + c.G - 1) / c.G;
+        with_p<TIER>(cls_p(c), [&](auto PC) {
+            col_body<F, decltype(PC)::value, DIR>(W, W, c.g, c.G, c.pl, c.tw, lds, u % tilesx, u / tilesx, opaque_tid());
+        });
+        __syncthreads();
+    }
+}
+
+template <typename F, int TIER, int SPLIT>
+__global__ void __launch_bounds__(256) k_cls_col_nl(const EnsClassGroup<F>* __restrict__ cg, long total) {
+    extern __shared__ unsigned char smem_raw[];
+    C2<F>* lds = reinterpret_cast<C2<F>*>(smem_raw);
+    int gi = 0;
+    for (long it = blockIdx.x; it < total; it += gridDim.x) {
+        gi = cls_group(cg, gi, it);
+        const EnsClassGroup<F>& c = cg[gi];
+        const long rel = it - c.first;
+        const int u = (int)(rel % c.units);
+        const int tilesx = (c.g.nxc + c.G - 1) / c.G;
+        C2<F>* W = c.reps[rel / c.units].W;
+        with_p<TIER>(cls_p(c), [&](auto PC) {
+            col_nl_body<F, decltype(PC)::value, SPLIT>(W, c.g, c.G, c.pl, c.tw, c.scale, lds, u % tilesx, u / tilesx,
+                                                       opaque_tid());
+        });
+        __syncthreads();
+    }
+}

@@ -38,6 +38,19 @@ struct EnsGroup {
     void* d_reps = nullptr;  // EnsRep<F>[members]
 };
 
+// A plan class: one launch of one pass slot (0 x; 1 y inverse, or the 2D nonlinear y pass; 2 z
+// nonlinear; 3 y forward) over the size groups whose plan for it falls in one register tier of
+// points per thread and whose tiles make T-thread workgroups.
+enum { CLS_X = 0, CLS_COL_INV = 1, CLS_COL_NL = 2, CLS_COL_FWD = 3 };
+struct EnsClass {
+    int slot = 0, kind = 0, tier = 0, T = 0;
+    size_t lds = 0;
+    long total = 0;
+    bool noise = false;
+    void* d_desc = nullptr;  // EnsClassGroup<F>[groups]
+    std::vector<int> groups;
+};
+
 struct mcs_ensemble {
     std::vector<mcs_replica> reps;
     int split = 0, prec = 32;
@@ -66,7 +79,15 @@ struct mcs_ensemble {
     std::vector<void*> a_knn;                // per group: bond-order cell-list workspace
     int64_t* a_counts = nullptr;             // [nrep]
     unsigned long long* a_hist = nullptr;    // [nrep][2][nbins]: q4 then q6
+    // plan classes: per pass slot, the launches that cover every size group sharing a plan
+    std::vector<EnsClass> cls;
+    bool graph_classes = false;              // the step graph replays class launches (else group branches)
+    bool batched_classes = false;            // eager batched steps launch the classes (else per group)
+    double strat_ms[4] = {-1, -1, -1, -1};   // per step: eager per group, eager classes, graph per group, graph classes
+    std::vector<hipStream_t> cside;          // fork streams of the class launches of one pass
+    std::vector<hipEvent_t> cev;             // [0] fork, [1 + i] join of class i
 };
+
 
 static size_t elem_size(int prec) { return prec == 64 ? 16 : 8; }
 
@@ -219,6 +240,166 @@ static int ens_step_group(mcs_ensemble* e, EnsGroup* g, int first, int nr, int d
     return st;
 }
 
+// ------------------------------------------------------------------ plan classes
+static void ens_classes_free(mcs_ensemble* e) {
+    for (EnsClass& c : e->cls)
+        if (c.d_desc) (void)hipFree(c.d_desc);
+    e->cls.clear();
+}
+
+// the classes of every pass slot: a group joins the class of its (slot, kind, register tier) and
+// workgroup size T = G * S, with G the most lines (row pairs) of S = L / P threads that fit 256
+// threads, the launch bound and the LDS rule (T = 256 whenever S divides 256).  A line of more
+// than 256 threads fits no class: then the ensemble keeps per-group launches.
+template <typename F> static int ens_build_classes_t(mcs_ensemble* e) {
+    ens_classes_free(e);
+    struct Item { int gi, slot, kind; const LinePlan* lp; ColGeom g; };
+    std::vector<Item> items;
+    for (int gi = 0; gi < (int)e->groups.size(); ++gi) {
+        const StepGeom& sg = e->groups[gi]->sg;
+        items.push_back({gi, 0, CLS_X, &sg.px, ColGeom{}});
+        if (sg.is3d) {
+            items.push_back({gi, 1, CLS_COL_INV, &sg.py, step_geom_y(sg)});
+            items.push_back({gi, 2, CLS_COL_NL, &sg.pz, step_geom_z(sg)});
+            items.push_back({gi, 3, CLS_COL_FWD, &sg.py, step_geom_y(sg)});
+        } else {
+            items.push_back({gi, 1, CLS_COL_NL, &sg.py, step_geom_y(sg)});
+        }
+    }
+    std::map<std::tuple<int, int, int, int>, std::vector<std::pair<int, int>>> by;  // key -> (item, G)
+    for (int i = 0; i < (int)items.size(); ++i) {
+        const Item& it = items[i];
+        const LinePlan& lp = *it.lp;
+        if (lp.engine != ENGINE_REG || lp.P <= 0) return MCS_OK;
+        const int S = lp.L / lp.P;
+        int G = 256 / S;
+        while (G >= 1 && !(it.kind == CLS_X ? x_tile_ok(lp.L, lp.P, e->prec, G) : col_tile_ok(lp.L, lp.P, e->prec, G)))
+            --G;
+        if (G < 1) return MCS_OK;
+        by[std::make_tuple(it.slot, it.kind, cls_tier(lp.P), G * S)].push_back({i, G});
+    }
+    for (auto& kv : by) {
+        EnsClass c;
+        c.slot = std::get<0>(kv.first);
+        c.kind = std::get<1>(kv.first);
+        c.tier = std::get<2>(kv.first);
+        c.T = std::get<3>(kv.first);
+        std::vector<EnsClassGroup<F>> h;
+        long first = 0;
+        for (const auto& ig : kv.second) {
+            const Item& it = items[ig.first];
+            const int G = ig.second;
+            EnsGroup* g = e->groups[it.gi];
+            const LinePlan& lp = *it.lp;
+            EnsClassGroup<F> d{};
+            d.pl = lp.dev;
+            d.tw = (const C2<F>*)lp.tw;
+            d.reps = (const EnsRep<F>*)g->d_reps;
+            d.pairs = g->sg.pairs;
+            d.wph = (const C2<F>*)g->sg.wph;
+            d.g = it.g;
+            d.scale = (F)(2.0 / ((double)g->sg.nx * g->sg.ny * g->sg.nz));
+            d.npairs = g->sg.npairs;
+            d.P = lp.P;
+            d.G = G;
+            d.units = it.kind == CLS_X ? (g->sg.npairs + G - 1) / G : ((it.g.nxc + G - 1) / G) * it.g.nouter;
+            d.first = first;
+            first += (long)d.units * (long)g->members.size();
+            d.total = first;
+            h.push_back(d);
+            const size_t lds = (size_t)G * (it.kind == CLS_X ? 2 : 1) * lp.L * sizeof(C2<F>);
+            c.lds = std::max(c.lds, lds);
+            c.groups.push_back(it.gi);
+            for (int r : g->members) c.noise = c.noise || (it.kind == CLS_X && e->reps[r].temp > 0);
+        }
+        c.total = first;
+        e->cls.push_back(c);
+        MCS_HIP_TRY(hipMalloc(&e->cls.back().d_desc, h.size() * sizeof(EnsClassGroup<F>)));
+        MCS_HIP_TRY(hipMemcpy(e->cls.back().d_desc, h.data(), h.size() * sizeof(EnsClassGroup<F>),
+                              hipMemcpyHostToDevice));
+    }
+    return MCS_OK;
+}
+
+static int cls_lds_attr(const void* fn, size_t bytes) {
+    if (bytes > 65536) MCS_HIP_TRY(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+    return MCS_OK;
+}
+
+template <typename F, int TIER>
+static int cls_launch_t(mcs_ensemble* e, const EnsClass& c, int draw_off, hipStream_t s) {
+    const EnsClassGroup<F>* d = (const EnsClassGroup<F>*)c.d_desc;
+    const unsigned grid = persistent_grid(e, c.T, c.lds, c.total);
+    const void* fn = nullptr;
+    const bool ref = e->split == MCS_SPLIT_REFERENCE;
+    long total = c.total;
+    if (c.kind == CLS_X) {
+        fn = c.noise ? (const void*)k_cls_xupd<F, TIER, true> : (const void*)k_cls_xupd<F, TIER, false>;
+        int st = cls_lds_attr(fn, c.lds);
+        if (st) return st;
+        const int64_t* ds = e->d_step;
+        void* args[] = {&d, &total, &ds, &draw_off};
+        MCS_HIP_TRY(hipLaunchKernel(fn, dim3(grid), dim3(c.T), args, c.lds, s));
+        return MCS_OK;
+    }
+    if (c.kind == CLS_COL_NL)
+        fn = ref ? (const void*)k_cls_col_nl<F, TIER, MCS_SPLIT_REFERENCE> : (const void*)k_cls_col_nl<F, TIER, MCS_SPLIT_NORTHSTAR>;
+    else
+        fn = c.kind == CLS_COL_INV ? (const void*)k_cls_col<F, TIER, +1> : (const void*)k_cls_col<F, TIER, -1>;
+    int st = cls_lds_attr(fn, c.lds);
+    if (st) return st;
+    void* args[] = {&d, &total};
+    MCS_HIP_TRY(hipLaunchKernel(fn, dim3(grid), dim3(c.T), args, c.lds, s));
+    return MCS_OK;
+}
+
+template <typename F> static int cls_launch(mcs_ensemble* e, const EnsClass& c, int draw_off, hipStream_t s) {
+    return c.tier == 0 ? cls_launch_t<F, 0>(e, c, draw_off, s) : cls_launch_t<F, 1>(e, c, draw_off, s);
+}
+
+static int ens_class_streams(mcs_ensemble* e, size_t n) {
+    while (e->cside.size() < n) {
+        hipStream_t b = nullptr;
+        MCS_HIP_TRY(hipStreamCreateWithFlags(&b, hipStreamNonBlocking));
+        e->cside.push_back(b);
+    }
+    while (e->cev.size() < n + 1) {
+        hipEvent_t ev = nullptr;
+        MCS_HIP_TRY(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        e->cev.push_back(ev);
+    }
+    return MCS_OK;
+}
+
+// one step as class launches, pass slot after pass slot; with fork, the classes of a slot run on
+// their own streams between a fork and a join on s
+template <typename F> static int ens_class_step(mcs_ensemble* e, int draw_off, hipStream_t s, bool fork) {
+    for (int slot = 0; slot < 4; ++slot) {
+        std::vector<const EnsClass*> cs;
+        for (const EnsClass& c : e->cls)
+            if (c.slot == slot) cs.push_back(&c);
+        if (cs.empty()) continue;
+        if (!fork || cs.size() == 1) {
+            for (const EnsClass* c : cs) {
+                int st = cls_launch<F>(e, *c, draw_off, s);
+                if (st) return st;
+            }
+            continue;
+        }
+        int st = ens_class_streams(e, cs.size());
+        if (st) return st;
+        MCS_HIP_TRY(hipEventRecord(e->cev[0], s));
+        for (size_t i = 0; i < cs.size(); ++i) {
+            MCS_HIP_TRY(hipStreamWaitEvent(e->cside[i], e->cev[0], 0));
+            st = cls_launch<F>(e, *cs[i], draw_off, e->cside[i]);
+            if (st) return st;
+            MCS_HIP_TRY(hipEventRecord(e->cev[1 + i], e->cside[i]));
+        }
+        for (size_t i = 0; i < cs.size(); ++i) MCS_HIP_TRY(hipStreamWaitEvent(s, e->cev[1 + i], 0));
+    }
+    return MCS_OK;
+}
+
 // plans and tile shapes of a size group, timed on the group's own work-table kernels over all
 // its members (the trial launches overwrite their states, which are loaded afterwards); cached
 // per shape, member count and noise
@@ -253,6 +434,8 @@ template <typename F> static int ens_steps_t(mcs_ensemble* e, int nsteps, int mo
         if (mode == MCS_ENS_LOOP) {
             for (int r = 0; r < (int)e->reps.size() && !st; ++r)
                 st = ens_step_group<F>(e, e->groups[e->group_of[r]], e->slot_of[r], 1, it, s);
+        } else if (e->batched_classes) {
+            st = ens_class_step<F>(e, it, s, false);
         } else {
             for (int gi = 0; gi < ng && !st; ++gi)
                 st = ens_step_group<F>(e, e->groups[gi], 0, (int)e->groups[gi]->members.size(), it, s);
@@ -293,6 +476,22 @@ static int ens_analysis_group(mcs_ensemble* e, int gi, hipStream_t b) {
 template <typename F> static int ens_branches_t(mcs_ensemble* e, int nsteps, hipStream_t s, bool analysis) {
     const int ng = (int)e->groups.size();
     int st = MCS_OK;
+    if (e->graph_classes) {
+        // class launches pass after pass, then (with analysis) every group's analysis on its branch
+        for (int it = 0; it < nsteps && !st; ++it) st = ens_class_step<F>(e, it, s, true);
+        if (!st && analysis) {
+            if (hipEventRecord(e->side_ev[0], s) != hipSuccess) return MCS_ERR_HIP;
+            for (int gi = 0; gi < ng && !st; ++gi) {
+                hipStream_t b = e->side[gi];
+                if (hipStreamWaitEvent(b, e->side_ev[0], 0) != hipSuccess) return MCS_ERR_HIP;
+                st = ens_analysis_group(e, gi, b);
+                if (!st && hipEventRecord(e->side_ev[1 + gi], b) != hipSuccess) st = MCS_ERR_HIP;
+                if (!st && hipStreamWaitEvent(s, e->side_ev[1 + gi], 0) != hipSuccess) st = MCS_ERR_HIP;
+            }
+        }
+        if (!st) st = ens_advance(e, nsteps, s);
+        return st;
+    }
     if (hipEventRecord(e->side_ev[0], s) != hipSuccess) return MCS_ERR_HIP;
     for (int gi = 0; gi < ng && !st; ++gi) {
         hipStream_t b = e->side[gi];
@@ -321,6 +520,10 @@ static int ens_capture(mcs_ensemble* e, hipStream_t s, bool analysis, hipGraph_t
         MCS_HIP_TRY(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
         e->side_ev.push_back(ev);
     }
+    size_t per_slot[4] = {0, 0, 0, 0};
+    for (const EnsClass& c : e->cls) ++per_slot[c.slot];
+    int cst = ens_class_streams(e, std::max(std::max(per_slot[0], per_slot[1]), std::max(per_slot[2], per_slot[3])));
+    if (cst) return cst;
     MCS_HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     int st = e->prec == 64 ? ens_branches_t<double>(e, GRAPH_STEPS, s, analysis)
                            : ens_branches_t<float>(e, GRAPH_STEPS, s, analysis);
@@ -337,6 +540,71 @@ static int ens_graph_build(mcs_ensemble* e, hipStream_t s) {
     if (e->gexec) return MCS_OK;
     int st = ens_capture(e, s, false, &e->graph, &e->gexec);
     if (!st) e->gsteps = GRAPH_STEPS;
+    return st;
+}
+
+// The step graph either replays every size group on its own branch or the plan-class launches
+// pass by pass, and eager batched steps launch either per group or per class; every form computes
+// the same bits (the same plans), so for each mode the faster is kept: all are timed on the new
+// ensemble's buffers (before any field is loaded; the draw counter is reset afterwards).
+static int ens_choose_graph(mcs_ensemble* e, hipStream_t s) {
+    hipEvent_t a = nullptr, b = nullptr;
+    MCS_HIP_TRY(hipEventCreate(&a));
+    MCS_HIP_TRY(hipEventCreate(&b));
+    double best = -1;
+    int st = MCS_OK;
+    for (int use = 0; use <= 1 && !st; ++use) {
+        if (use && e->cls.empty()) break;
+        e->graph_classes = use != 0;
+        hipGraph_t g = nullptr;
+        hipGraphExec_t x = nullptr;
+        st = ens_capture(e, s, false, &g, &x);
+        if (st) break;
+        double ms = -1;
+        float el = 0;
+        if (hipGraphLaunch(x, s) == hipSuccess && hipEventRecord(a, s) == hipSuccess) {
+            bool ok = true;
+            for (int i = 0; i < 3 && ok; ++i) ok = hipGraphLaunch(x, s) == hipSuccess;
+            if (ok && hipEventRecord(b, s) == hipSuccess && hipEventSynchronize(b) == hipSuccess &&
+                hipEventElapsedTime(&el, a, b) == hipSuccess)
+                ms = el;
+        }
+        if (ms > 0) e->strat_ms[2 + use] = ms / (3.0 * GRAPH_STEPS);
+        if (ms > 0 && (best < 0 || ms < best)) {
+            best = ms;
+            if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
+            if (e->graph) (void)hipGraphDestroy(e->graph);
+            e->gexec = x;
+            e->graph = g;
+            e->gsteps = GRAPH_STEPS;
+        } else {
+            (void)hipGraphExecDestroy(x);
+            (void)hipGraphDestroy(g);
+            if (use) e->graph_classes = false;
+        }
+    }
+    // eager batched steps: per-group launches against class launches, the same way
+    double best_eager = -1;
+    for (int use = 0; use <= 1 && !st; ++use) {
+        if (use && e->cls.empty()) break;
+        e->batched_classes = use != 0;
+        float el = 0;
+        if (ens_steps(e, 2, MCS_ENS_BATCHED, s) != MCS_OK || hipEventRecord(a, s) != hipSuccess ||
+            ens_steps(e, 6, MCS_ENS_BATCHED, s) != MCS_OK || hipEventRecord(b, s) != hipSuccess ||
+            hipEventSynchronize(b) != hipSuccess || hipEventElapsedTime(&el, a, b) != hipSuccess) {
+            (void)hipGetLastError();
+            if (use) e->batched_classes = false;
+            continue;
+        }
+        e->strat_ms[use] = el / 6.0;
+        if (best_eager < 0 || el < best_eager) best_eager = el;
+        else e->batched_classes = false;
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    if (!st && best < 0) { mcs_set_last_error(__FILE__, __LINE__, "no step graph ran"); st = MCS_ERR_HIP; }
+    if (!st) MCS_HIP_TRY(hipMemsetAsync(e->d_step, 0, sizeof(int64_t), s));
+    if (!st) MCS_HIP_TRY(hipStreamSynchronize(s));
     return st;
 }
 
@@ -398,6 +666,9 @@ extern "C" int mcs_ensemble_destroy(mcs_ensemble* e) {
     if (e->graph) (void)hipGraphDestroy(e->graph);
     for (hipStream_t b : e->side) (void)hipStreamDestroy(b);
     for (hipEvent_t ev : e->side_ev) (void)hipEventDestroy(ev);
+    ens_classes_free(e);
+    for (hipStream_t b : e->cside) (void)hipStreamDestroy(b);
+    for (hipEvent_t ev : e->cev) (void)hipEventDestroy(ev);
     for (EnsGroup* g : e->groups) {
         step_geom_free(&g->sg);
         if (g->d_reps) (void)hipFree(g->d_reps);
@@ -460,6 +731,8 @@ extern "C" int mcs_ensemble_create(mcs_ensemble** out, const mcs_replica* reps, 
     for (size_t gi = 0; gi < e->groups.size() && !rc; ++gi)
         rc = e->prec == 64 ? ens_tune_group<double>(e, e->groups[gi], mcs_stream())
                            : ens_tune_group<float>(e, e->groups[gi], mcs_stream());
+    if (!rc) rc = e->prec == 64 ? ens_build_classes_t<double>(e) : ens_build_classes_t<float>(e);
+    if (!rc) rc = ens_choose_graph(e, mcs_stream());
     if (rc) { mcs_ensemble_destroy(e); return rc; }
     *out = e;
     return MCS_OK;
@@ -526,6 +799,39 @@ extern "C" int mcs_ensemble_step_async(mcs_ensemble* e, int nsteps, int mode) {
     int rc = ens_steps(e, nsteps - done, mode == MCS_ENS_LOOP ? MCS_ENS_LOOP : MCS_ENS_BATCHED, s);
     if (rc) return rc;
     e->step += nsteps;
+    return MCS_OK;
+}
+
+extern "C" int mcs_ensemble_launch_info(mcs_ensemble* e, int* groups, int* classes_per_step, int* batched_classes,
+                                        int* graph_classes, double ms_per_step[4]) {
+    MCS_CHECK(e && groups && classes_per_step && batched_classes && graph_classes && ms_per_step, MCS_ERR_INVALID,
+              "null argument");
+    *groups = (int)e->groups.size();
+    *classes_per_step = (int)e->cls.size();
+    *batched_classes = e->batched_classes ? 1 : 0;
+    *graph_classes = e->graph_classes ? 1 : 0;
+    for (int i = 0; i < 4; ++i) ms_per_step[i] = e->strat_ms[i];
+    return MCS_OK;
+}
+
+extern "C" int mcs_ensemble_set_launch_form(mcs_ensemble* e, int batched_classes, int graph_classes) {
+    MCS_CHECK(e, MCS_ERR_INVALID, "null argument");
+    MCS_CHECK(!(batched_classes || graph_classes) || !e->cls.empty(), MCS_ERR_UNSUPPORTED,
+              "this ensemble has no plan classes (a line longer than 256 threads)");
+    MCS_HIP_TRY(hipStreamSynchronize(mcs_stream()));
+    e->batched_classes = batched_classes != 0;
+    if (e->graph_classes != (graph_classes != 0)) {
+        e->graph_classes = graph_classes != 0;
+        // the graphs are captured again on their next use
+        if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
+        if (e->graph) (void)hipGraphDestroy(e->graph);
+        e->gexec = nullptr;
+        e->graph = nullptr;
+        if (e->aexec) (void)hipGraphExecDestroy(e->aexec);
+        if (e->agraph) (void)hipGraphDestroy(e->agraph);
+        e->aexec = nullptr;
+        e->agraph = nullptr;
+    }
     return MCS_OK;
 }
 

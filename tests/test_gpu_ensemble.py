@@ -36,6 +36,27 @@ def test_modes_agree_bit_for_bit(mcs, make):
         assert np.array_equal(outs[0][r], outs[1][r]) and np.array_equal(outs[0][r], outs[2][r])
 
 
+@pytest.mark.parametrize("form", [(0, 0), (1, 1), (1, 0), (0, 1)])
+def test_launch_forms_agree_bit_for_bit(mcs, form):
+    # plan-class launches (size groups of one register tier in one persistent launch per pass) and
+    # per-group launches, eager and graph-replayed, against the looping launches; 2D and 3D replicas
+    # of power-of-two, 3*2^a and 5*2^a sizes, noisy and not
+    reps = small_3d() + small_2d()
+    a = mcs.Ensemble(reps, dtype=np.float32)
+    a.init()
+    a.step(19, mcs.ENS_LOOP)
+    b = mcs.Ensemble(reps, dtype=np.float32)
+    b.set_launch_form(*form)
+    b.init()
+    b.step(3, mcs.ENS_BATCHED)
+    b.step(16, mcs.ENS_GRAPH)
+    info = b.launch_info()
+    assert info["batched_uses_classes"] == bool(form[0]) and info["graph_replays_classes"] == bool(form[1])
+    assert info["class_launches_per_step"] < 2 * info["size_groups"] + 3 * 4
+    for r in range(len(reps)):
+        assert np.array_equal(a.get_psi(r), b.get_psi(r)), r
+
+
 def test_graph_replays_between_eager_steps(mcs):
     # the captured graph holds one branch per size group; replays interleaved with eager steps and host
     # synchronisations must continue the same noise sequence as the looping launches
