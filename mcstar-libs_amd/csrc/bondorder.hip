@@ -55,30 +55,41 @@ MCS_DEV void ylm_dev(int l, int m, double ct, double phi, double* re, double* im
     *re = r; *im = i;
 }
 
-// accumulate the 39 (l = 4, 6, 8) harmonics of one bond into q[78] (re, im interleaved)
-MCS_DEV void add_bond(double* q, double ct, double phi, bool pole_rule) {
-    const int ls[3] = {4, 6, 8}, off[3] = {0, 9, 22};
-    for (int t = 0; t < 3; ++t) {
-        const int l = ls[t];
-        for (int m = -l; m <= l; ++m) {
-            if (pole_rule && ((m < 0 && fabs(1.0 + ct) < 1e-3) || (m > 0 && fabs(1.0 - ct) < 1e-3))) continue;
-            double re, im;
-            ylm_dev(l, m, ct, phi, &re, &im);
-            q[2 * (off[t] + m + l)] += re;
-            q[2 * (off[t] + m + l) + 1] += im;
-        }
+// accumulate the 2l + 1 harmonics of degree L of one bond into q[2 OFF ..] (re, im interleaved).
+// Unrolled over m, so every index into the accumulator is a constant and it stays in registers
+// (with run-time indices the 78 accumulators of a thread went to scratch memory); the arithmetic
+// of each term is the same as with loops.
+template <int L, int OFF>
+MCS_DEV void add_l(double (&q)[78], double ct, double phi, bool pole_rule) {
+#pragma unroll
+    for (int m = -L; m <= L; ++m) {
+        if (pole_rule && ((m < 0 && fabs(1.0 + ct) < 1e-3) || (m > 0 && fabs(1.0 - ct) < 1e-3))) continue;
+        double re, im;
+        ylm_dev(L, m, ct, phi, &re, &im);
+        q[2 * (OFF + m + L)] += re;
+        q[2 * (OFF + m + L) + 1] += im;
     }
+}
+
+// accumulate the 39 (l = 4, 6, 8) harmonics of one bond into q[78] (re, im interleaved)
+MCS_DEV void add_bond(double (&q)[78], double ct, double phi, bool pole_rule) {
+    add_l<4, 0>(q, ct, phi, pole_rule);
+    add_l<6, 9>(q, ct, phi, pole_rule);
+    add_l<8, 22>(q, ct, phi, pole_rule);
 }
 
 MCS_DEV double ql_dev(const double* q, int l) {
     double s = 0;
+#pragma unroll
     for (int m = 0; m <= 2 * l; ++m) s += q[2 * m] * q[2 * m] + q[2 * m + 1] * q[2 * m + 1];
     return sqrt(4.0 * M_PI / (2.0 * l + 1.0) * s);
 }
 
 MCS_DEV double w4_dev(const double* q4, const Wig4& W) {
     double acc = 0;
+#pragma unroll
     for (int m1 = -4; m1 <= 4; ++m1)
+#pragma unroll
         for (int m2 = -4; m2 <= 4; ++m2) {
             const int m3 = -m1 - m2;
             if (m3 < -4 || m3 > 4) continue;
@@ -130,6 +141,7 @@ __global__ void __launch_bounds__(128) k_bo_qlm(const double* xyz, int P, double
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= P) return;
     double acc[78];
+#pragma unroll
     for (int t = 0; t < 78; ++t) acc[t] = 0;
     int cnt = 0, lj = -1;
     const int nn = nbr_count<LIST>(P, nc, i);
@@ -140,6 +152,7 @@ __global__ void __launch_bounds__(128) k_bo_qlm(const double* xyz, int P, double
         if (bond(xyz, i, j, box, rlim, &ct, &ph)) { add_bond(acc, ct, ph, true); ++cnt; lj = j; }
     }
     double* qi = q + (long)QS * i;
+#pragma unroll
     for (int t = 0; t < 78; ++t) qi[t] = cnt ? acc[t] / (double)cnt : 0.0;
     nb[i] = cnt;
     last[i] = lj;
@@ -498,6 +511,7 @@ __global__ void __launch_bounds__(128) k_knn_query(const KnnGrid* gp, const int4
         return;
     }
     double acc[78];
+#pragma unroll
     for (int e = 0; e < 78; ++e) acc[e] = 0;
     for (int e = 0; e < found; ++e) {
         const int4 q = pts[js[e][threadIdx.x]];
@@ -506,6 +520,7 @@ __global__ void __launch_bounds__(128) k_knn_query(const KnnGrid* gp, const int4
         const double r = sqrt(dx * dx + dy * dy + dz * dz);
         add_bond(acc, dz / r, atan2(dy, dx), false);
     }
+#pragma unroll
     for (int e = 0; e < 78; ++e) acc[e] /= (double)found;
     const double q4 = ql_dev(acc, 4), q6 = ql_dev(acc + 18, 6);
     if (FULL) {
