@@ -5,14 +5,13 @@ HIPCC ?= /opt/rocm/bin/hipcc
 ARCH ?= gfx950
 PKG := mcstar-libs_amd
 SRC := $(wildcard $(PKG)/csrc/*.hip)
-HDR := $(wildcard $(PKG)/csrc/*.hpp) include/mcstar_hip.h
 OBJDIR := $(PKG)/_build
 OBJ := $(patsubst $(PKG)/csrc/%.hip,$(OBJDIR)/%.o,$(SRC))
 LIB := $(PKG)/lib/libmcstar_hip.so
 # packed-fp32 instructions are disabled for all device code; multiply-adds are fused only where
 # the source writes fma (mfma), so every kernel built from the same expressions rounds alike
 HIPFLAGS := -O3 -std=c++17 -fPIC --offload-arch=$(ARCH) -Xclang -target-feature -Xclang -packed-fp32-ops -ffp-contract=off \
-            -Wall -Wno-unused-function -Wno-unused-result -I include
+            -Wall -Wno-unused-function -Wno-unused-result -I include -MMD -MP
 
 all: lib oracle driver sim
 
@@ -47,9 +46,12 @@ tests/cpp/bin/kept_api_driver_f64: tests/cpp/kept_api_driver.cpp $(LIB) $(wildca
 
 lib: $(LIB)
 
-$(OBJDIR)/%.o: $(PKG)/csrc/%.hip $(HDR)
+# each object depends on the headers it includes (dependency files written by the compiler)
+$(OBJDIR)/%.o: $(PKG)/csrc/%.hip
 	@mkdir -p $(OBJDIR)
 	$(HIPCC) $(HIPFLAGS) -c $< -o $@
+
+-include $(OBJ:.o=.d)
 
 $(LIB): $(OBJ) $(PKG)/csrc/exports.map
 	@mkdir -p $(dir $@)

@@ -36,14 +36,25 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def make_stepper(mcs, n, dtype, cfg, plan=None):
+def make_stepper(mcs, n, dtype, cfg, plan=None, form=None):
     shp = (n, n, n)
     buf = mcs.DeviceBuffer(n ** 3 * np.dtype(dtype).itemsize)
     prec = 64 if dtype == np.float64 else 32
     mcs.check(mcs.lib().mcs_init_field(buf.ptr, mcs._ext(shp), cfg["psi0"], cfg["amp"], cfg["seed"], prec))
-    st = mcs.Stepper(shp, [cfg["dx"]] * 3, cfg["eps"], cfg["dt"], temp=0.0, seed=0, dtype=dtype, plan=plan)
+    st = mcs.Stepper(shp, [cfg["dx"]] * 3, cfg["eps"], cfg["dt"], temp=0.0, seed=0, dtype=dtype, plan=plan,
+                     form=form)
     st.set_psi_device(buf)
     return st, buf
+
+
+def x_sweeps(mcs, st):
+    """sweeps of the state-sized arrays in the x pass: read W and the state, write the state (mixed form);
+    read and write both (spectral form)"""
+    return 3 if st.form() == mcs.FORM_MIXED else 4
+
+
+def form_name(mcs, st):
+    return "mixed" if st.form() == mcs.FORM_MIXED else "spectral"
 
 
 def timed_steps(mcs, st, k):
@@ -139,7 +150,8 @@ def noise_config_bench(mcs, steps):
         ms = timed_steps(mcs, st, steps) / steps
         passes = st.pass_times(10)
         sweep = (n ** 3 // 2) * 2 * np.dtype(dtype).itemsize
-        nbytes = {"x_update": 4 * sweep, "y_inverse": 2 * sweep, "z_nonlinear": 2 * sweep, "y_forward": 2 * sweep}
+        nbytes = {"x_update": x_sweeps(mcs, st) * sweep, "y_inverse": 2 * sweep, "z_nonlinear": 2 * sweep,
+                  "y_forward": 2 * sweep}
         out[name] = {"ms_per_step": ms, "cell_updates_per_s": n ** 3 / (ms / 1e3), "plans": st.plan_info(),
                      "pass_GB_per_s": {k: round(nbytes[k] / (v / 1e3) / 1e9, 1) for k, v in passes.items()},
                      "pass_ms": {k: round(v, 5) for k, v in passes.items()}}
@@ -237,6 +249,9 @@ def main():
     ap.add_argument("--only-fft", action="store_true", help="run only the boundary FFT lines (profiling)")
     ap.add_argument("--only-analysis", action="store_true",
                     help="run only the 500-step FP32 field and its peak / bond-order analysis (profiling)")
+    ap.add_argument("--no-spectral", action="store_true", help="skip the spectral-form comparison line")
+    ap.add_argument("--form", choices=["mixed", "spectral"], default=None,
+                    help="state form of the timed steppers (default: the library's default, mixed)")
     ap.add_argument("--no-fft", action="store_true",
                     help="skip the boundary FFT line (profiling the step alone: its column passes share kernels)")
     ap.add_argument("--plan", default=None,
@@ -253,6 +268,8 @@ def main():
         sys.exit(2)
 
     mcs = load_pkg()
+    if a.form is not None:
+        mcs.set_step_form(mcs.FORM_MIXED if a.form == "mixed" else mcs.FORM_SPECTRAL)
     if a.fixed is not None:
         mcs.set_fixed_kernels(a.fixed)
     if a.copy_calibration:
@@ -289,10 +306,12 @@ def main():
     log(f"[bench] fp32 {ms_step:.4f} ms/step, passes {passes}")
     steps32 = a.warmup + a.steps + 10
 
-    # algorithmic bytes per pass: every pass streams the packed state (N/2 complex = 4N bytes in FP32)
-    # once in and once out; x_update also reads and writes the state spectrum
+    # algorithmic bytes per pass: every pass streams an array of the state's size (N/2 complex = 4N bytes in
+    # FP32) once in and once out; x_update also reads the state (mixed form: 3 sweeps) or reads and writes it
+    # (spectral form: 4 sweeps)
     sweep = 4 * N
-    pass_bytes = {"x_update": 4 * sweep, "y_inverse": 2 * sweep, "z_nonlinear": 2 * sweep, "y_forward": 2 * sweep}
+    pass_bytes = {"x_update": x_sweeps(mcs, st) * sweep, "y_inverse": 2 * sweep, "z_nonlinear": 2 * sweep,
+                  "y_forward": 2 * sweep}
     dom = max(passes, key=passes.get)
     achieved = pass_bytes[dom] / (passes[dom] / 1e3) / 1e9
     traffic = measured_traffic(dom, n)
@@ -313,8 +332,20 @@ def main():
                                   f"dt=0.5, noise off", "n": n, "precision": "fp32", "parallelism": "1 GPU"},
            "steps_per_s": 1e3 / ms_step, "hbm_roofline_frac_step": round(step_frac, 4),
            "model_bytes_per_step": b_step, "pass_ms": {k: round(v, 5) for k, v in passes.items()},
-           "plans": st.plan_info(), "autotune": mcs.get_autotune(), "fixed_kernels": mcs.get_fixed_kernels(),
-           "roofline": roofline}
+           "plans": st.plan_info(), "form": form_name(mcs, st), "sweeps_per_step": x_sweeps(mcs, st) + 6,
+           "autotune": mcs.get_autotune(), "fixed_kernels": mcs.get_fixed_kernels(), "roofline": roofline}
+
+    if not a.no_spectral:
+        # the same step in the spectral form (the state kept as the packed spectrum: 10 sweeps)
+        log("[bench] fp32 spectral form")
+        sp, _ = make_stepper(mcs, n, np.float32, cfg, plan, form=mcs.FORM_SPECTRAL)
+        sp.step(a.warmup)
+        ksp = min(a.steps, 100)
+        msp = timed_steps(mcs, sp, ksp) / ksp
+        out["spectral_form"] = {"ms_per_step": msp, "cell_updates_per_s": N / (msp / 1e3),
+                                "pass_ms": {k: round(v, 5) for k, v in sp.pass_times(10).items()},
+                                "plans": sp.plan_info(), "sweeps_per_step": 10}
+        del sp
 
     if not a.no_fp64:
         log("[bench] fp64 form")

@@ -222,14 +222,48 @@ int mcs_get_autotune(int* on);
 #define MCS_FIXED_XPASS 2   /* x passes: the step's x pass with the update, the boundary FFT's row passes */
 int mcs_set_fixed_kernels(int mask);
 int mcs_get_fixed_kernels(int* mask);
+/* Lane shuffles in the fixed-plan x passes of the step (no reference counterpart; replaces the
+ * LDS exchanges inside cuFFT, cufft_wrapper.hpp:50-58).  Where a line's threads are lanes of
+ * one wavefront and its plan ends in two passes of radix P (points per thread), the exchange
+ * before the last pass runs as a register/lane transpose: DPP quad permutations and row
+ * rotations and gfx950's v_permlane16_swap / v_permlane32_swap, no LDS and no barrier.  The
+ * same values land in the same registers, so results are identical either way.  mode 0: never;
+ * 1 (default): where the autotuner timed that form faster; 2: wherever a plan has it. */
+int mcs_set_lane_shuffle(int mode);
+int mcs_get_lane_shuffle(int* mode);
 /* Exact checkpoint of the stepper (no reference counterpart; the simulators saved psi
- * with CPUArray::save, array_abstraction.hpp:155-180).  The state is the packed spectrum
- * U = FFT(u) of u(m,y,z) = psi(2m,y,z) + i psi(2m+1,y,z) over (nx/2) x ny x nz, complex,
- * unnormalised, nx*ny*nz reals of the stepper's precision, plus the FP64 mass (sum of
- * psi) and the step number (the noise draw).  spec may be host or device memory.
- * Restoring into a stepper with the same plans continues bit for bit. */
+ * with CPUArray::save, array_abstraction.hpp:155-180).  In the spectral form the state is
+ * the packed spectrum U = FFT(u) of u(m,y,z) = psi(2m,y,z) + i psi(2m+1,y,z) over
+ * (nx/2) x ny x nz, complex, unnormalised; in the mixed form it is S = IFFT_x(U) / (nx/2)
+ * (see MCS_FORM_MIXED below); nx*ny*nz reals of the stepper's precision either way, plus
+ * the FP64 mass (sum of psi) and the step number (the noise draw).  spec may be host or
+ * device memory.  Restoring into a stepper with the same plans and form continues bit for
+ * bit. */
 int mcs_stepper_get_state(mcs_stepper* st, void* spec, double* mass, int64_t* step);
 int mcs_stepper_set_state(mcs_stepper* st, const void* spec, double mass, int64_t step);
+/* The form of the stepper's state (no reference counterpart; the reference keeps the
+ * half spectrum between steps and runs c2r/r2c around the nonlinearity).
+ *   MCS_FORM_SPECTRAL: the state is the packed spectrum U above.  A 3D step moves 10 sweeps
+ *     of N/2 complex values (the x pass reads and writes U and the work array), a 2D step 6.
+ *   MCS_FORM_MIXED: the state is S = IFFT_x(U) / (nx/2), real along x and spectral along y
+ *     and z, same size.  The x pass reads the work array and S and writes S only: 9 sweeps
+ *     per 3D step, 5 per 2D step; the x pass does three pairs of line transforms instead of
+ *     two.  The same update on the same modes; the results differ from the spectral form's
+ *     by rounding only, but the state itself then passes through an x transform and back
+ *     every step, which in FP32 roughly doubles the error against an exact run (512^3,
+ *     500 steps: rel-L2 about 1.0e-4 against 4e-5), for about 5% less time per step.
+ * mcs_set_step_form sets the form of steppers created afterwards (MCS_FORM_SPECTRAL by
+ * default).
+ * mcs_stepper_set_form converts a stepper's state in place (one x transform of the state;
+ * the work array stays valid), so the next steps run in the new form; on a stepper without a
+ * field and with tuned plans it also tunes the x pass for the new form.  get_state/set_state
+ * carry the state of the stepper's current form. */
+#define MCS_FORM_SPECTRAL 0
+#define MCS_FORM_MIXED 1
+int mcs_set_step_form(int form);
+int mcs_get_step_form(int* form);
+int mcs_stepper_set_form(mcs_stepper* st, int form);
+int mcs_stepper_get_form(mcs_stepper* st, int* form);
 /* the stepper's plans: {P_x, row pairs per workgroup, P_y, lines per workgroup,
  * P_z, lines per workgroup} (P_z and its tile are 0 in 2D) */
 int mcs_stepper_plan_info(mcs_stepper* st, int info[6]);

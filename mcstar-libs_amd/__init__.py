@@ -22,6 +22,7 @@ OK, ERR_INVALID, ERR_HIP, ERR_UNSUPPORTED, ERR_NOMEM, ERR_CAPACITY = 0, 1, 2, 3,
 H2D, D2H, D2D, H2H = 1, 2, 3, 4
 R2C, C2R = 0, 1
 SPLIT_NORTHSTAR, SPLIT_REFERENCE = 0, 1
+FORM_SPECTRAL, FORM_MIXED = 0, 1
 
 _vp = ctypes.c_void_p
 _ip = ctypes.POINTER(ctypes.c_int)
@@ -104,6 +105,12 @@ _SIGS = {
     "mcs_stepper_destroy": ([_vp], ctypes.c_int),
     "mcs_stepper_pass_times": ([_vp, ctypes.c_int, _dp, ctypes.c_int, _ip], ctypes.c_int),
     "mcs_stepper_plan_info": ([_vp, _ip], ctypes.c_int),
+    "mcs_set_step_form": ([ctypes.c_int], ctypes.c_int),
+    "mcs_set_lane_shuffle": ([ctypes.c_int], ctypes.c_int),
+    "mcs_get_lane_shuffle": ([_ip], ctypes.c_int),
+    "mcs_get_step_form": ([_ip], ctypes.c_int),
+    "mcs_stepper_set_form": ([_vp, ctypes.c_int], ctypes.c_int),
+    "mcs_stepper_get_form": ([_vp, _ip], ctypes.c_int),
     "mcs_stepper_get_state": ([_vp, ctypes.c_void_p, _dp, _i64p], ctypes.c_int),
     "mcs_stepper_set_state": ([_vp, ctypes.c_void_p, ctypes.c_double, ctypes.c_int64], ctypes.c_int),
     "mcs_set_autotune": ([ctypes.c_int], ctypes.c_int),
@@ -584,8 +591,9 @@ class Stepper:
     """The fused PFC stepper (mcs_stepper_*)."""
 
     def __init__(self, shape_zyx, h, eps, dt, temp=0.0, seed=0, split=SPLIT_NORTHSTAR, dtype=np.float32,
-                 plan=None):
-        """plan: optional explicit [P_x, pairs/wg, P_y, lines/wg, P_z, lines/wg] instead of the tuned plans"""
+                 plan=None, form=None):
+        """plan: optional explicit [P_x, pairs/wg, P_y, lines/wg, P_z, lines/wg] instead of the tuned plans;
+        form: FORM_SPECTRAL or FORM_MIXED (default: the library's default form, mcs_set_step_form)"""
         self.shape = tuple(shape_zyx)
         self.dtype = np.dtype(dtype)
         p = PfcParams()
@@ -601,7 +609,18 @@ class Stepper:
             check(lib().mcs_stepper_create_planned(ctypes.byref(ptr), ctypes.addressof(p), pl),
                   "mcs_stepper_create_planned")
         self.ptr = ptr
+        if form is not None:
+            self.set_form(form)
         self.buf = DeviceBuffer(int(np.prod(shape_zyx)) * self.dtype.itemsize)
+
+    def set_form(self, form):
+        """switch the state's form (converted in place when a field is loaded)"""
+        check(lib().mcs_stepper_set_form(self.ptr, int(form)), "mcs_stepper_set_form")
+
+    def form(self):
+        v = ctypes.c_int()
+        check(lib().mcs_stepper_get_form(self.ptr, ctypes.byref(v)), "mcs_stepper_get_form")
+        return v.value
 
     def set_psi(self, psi):
         self.buf.upload(np.ascontiguousarray(psi, dtype=self.dtype))
@@ -640,7 +659,8 @@ class Stepper:
         return dict(zip(names, [ms[i] for i in range(npass.value)]))
 
     def get_state(self):
-        """(packed spectrum [nz][ny][nx/2] complex, FP64 mass, step): an exact checkpoint"""
+        """(state [nz][ny][nx/2] complex: the packed spectrum U in the spectral form, IFFT_x(U)/(nx/2) in the
+        mixed form; FP64 mass, step): an exact checkpoint for a stepper of the same plans and form"""
         ct = np.complex128 if self.dtype == np.float64 else np.complex64
         shp = self.shape[:-1] + (self.shape[-1] // 2,)
         spec = np.empty(shp, ct)
@@ -687,6 +707,29 @@ def tile_options(L, dtype, P, xpass):
     check(lib().mcs_tile_options(L, _prec(dtype), P, 1 if xpass else 0, buf, 16, ctypes.byref(n)),
           "mcs_tile_options")
     return [buf[i] for i in range(min(n.value, 16))]
+
+
+def set_step_form(form):
+    """form of steppers created afterwards: FORM_MIXED (the default; 9 sweeps per 3D step) or FORM_SPECTRAL"""
+    check(lib().mcs_set_step_form(int(form)), "mcs_set_step_form")
+
+
+def set_lane_shuffle(mode):
+    """lane shuffles in the mixed form's fixed-plan x pass: 0 never, 1 where the tuner timed them faster
+    (default), 2 wherever a plan has them (results are identical either way)"""
+    check(lib().mcs_set_lane_shuffle(int(mode)), "mcs_set_lane_shuffle")
+
+
+def get_lane_shuffle():
+    v = ctypes.c_int()
+    check(lib().mcs_get_lane_shuffle(ctypes.byref(v)), "mcs_get_lane_shuffle")
+    return v.value
+
+
+def get_step_form():
+    v = ctypes.c_int()
+    check(lib().mcs_get_step_form(ctypes.byref(v)), "mcs_get_step_form")
+    return v.value
 
 
 def get_autotune():

@@ -61,7 +61,7 @@ __global__ void __launch_bounds__(MCS_REG_MAX_THREADS(P, F)) k_ens_col_nl(const 
     for (long it = blockIdx.x; it < total; it += gridDim.x) {
         const int r = (int)(it / units);
         const long u = it % units;
-        col_nl_body<F, P, SPLIT, FP>(reps[r].W, g, G, pl, tw, scale, lds, (int)(u % tilesx), (int)(u / tilesx),
+        col_nl_body<F, P, SPLIT, FP>(reps[r].W, reps[r].W, g, G, pl, tw, scale, lds, (int)(u % tilesx), (int)(u / tilesx),
                                      opaque_tid());
         __syncthreads();
     }
@@ -184,7 +184,7 @@ __global__ void __launch_bounds__(256) k_cls_col_nl(const EnsClassGroup<F>* __re
         const int tilesx = (c.g.nxc + c.G - 1) / c.G;
         C2<F>* W = c.reps[rel / c.units].W;
         with_p<TIER>(cls_p(c), [&](auto PC) {
-            col_nl_body<F, decltype(PC)::value, SPLIT>(W, c.g, c.G, c.pl, c.tw, c.scale, lds, u % tilesx, u / tilesx,
+            col_nl_body<F, decltype(PC)::value, SPLIT>(W, W, c.g, c.G, c.pl, c.tw, c.scale, lds, u % tilesx, u / tilesx,
                                                        opaque_tid());
         });
         __syncthreads();

@@ -282,6 +282,76 @@ MCS_DEV void reg_fft(C2<F> (&a)[P], const FftPlanDev& pl, const C2<F>* __restric
     }
 }
 
+// ------------------------------------------------------------ lane exchanges
+// A swap stage of a register/lane transpose within a wavefront: the lanes whose bit D is clear
+// take the lo value of their partner lane l ^ D into hi; the lanes whose bit D is set take the
+// partner's hi into lo.  D = 1, 2: DPP quad permutations; 4, 8: DPP rotations within a row of 16
+// lanes, each written only to the lanes (banks of 4) that need it; 16, 32: gfx950's
+// v_permlane16_swap / v_permlane32_swap, which exchange exactly those halves between two
+// registers.  No LDS, no barrier.
+template <int D> MCS_DEV void lane_swap32(unsigned& lo, unsigned& hi) {
+    static_assert(D == 1 || D == 2 || D == 4 || D == 8 || D == 16 || D == 32, "lane distance");
+    if constexpr (D == 1 || D == 2) {
+        constexpr int qp = D == 1 ? 0xB1 : 0x4E;  // quad_perm [1,0,3,2] and [2,3,0,1]
+        const unsigned plo = (unsigned)__builtin_amdgcn_mov_dpp((int)lo, qp, 0xF, 0xF, false);
+        const unsigned phi = (unsigned)__builtin_amdgcn_mov_dpp((int)hi, qp, 0xF, 0xF, false);
+        const bool up = (__lane_id() & D) != 0;
+        lo = up ? phi : lo;
+        hi = up ? hi : plo;
+    } else if constexpr (D == 4 || D == 8) {
+        // row_ror:n gives lane l the value of lane (l - n) mod 16 within its row
+        constexpr int from_up = D == 4 ? 0x12C : 0x128;  // from lane l + D (ror 12, ror 8)
+        constexpr int from_dn = D == 4 ? 0x124 : 0x128;  // from lane l - D (ror 4, ror 8)
+        constexpr int clear = D == 4 ? 0x5 : 0x3;        // banks whose lanes have bit D clear
+        const unsigned nhi = (unsigned)__builtin_amdgcn_update_dpp((int)hi, (int)lo, from_up, 0xF, clear, false);
+        const unsigned nlo = (unsigned)__builtin_amdgcn_update_dpp((int)lo, (int)hi, from_dn, 0xF, 0xF ^ clear, false);
+        lo = nlo;
+        hi = nhi;
+    } else if constexpr (D == 16) {
+        const auto r = __builtin_amdgcn_permlane16_swap(lo, hi, false, false);
+        lo = r[0];
+        hi = r[1];
+    } else {
+        const auto r = __builtin_amdgcn_permlane32_swap(lo, hi, false, false);
+        lo = r[0];
+        hi = r[1];
+    }
+}
+
+template <int D> MCS_DEV void lane_swap(float& lo, float& hi) {
+    unsigned a = __float_as_uint(lo), b = __float_as_uint(hi);
+    lane_swap32<D>(a, b);
+    lo = __uint_as_float(a);
+    hi = __uint_as_float(b);
+}
+
+template <int D> MCS_DEV void lane_swap(double& lo, double& hi) {
+    const unsigned long long a = (unsigned long long)__double_as_longlong(lo);
+    const unsigned long long b = (unsigned long long)__double_as_longlong(hi);
+    unsigned al = (unsigned)a, ah = (unsigned)(a >> 32), bl = (unsigned)b, bh = (unsigned)(b >> 32);
+    lane_swap32<D>(al, bl);
+    lane_swap32<D>(ah, bh);
+    lo = __longlong_as_double((long long)(((unsigned long long)ah << 32) | al));
+    hi = __longlong_as_double((long long)(((unsigned long long)bh << 32) | bl));
+}
+
+// The exchange before a last pass of radix P that follows a pass of radix P, done in registers.
+// There thread t = a NS + k (a in [0, P), k in [0, NS), NS = L / P^2) holds output q of its
+// butterfly in v[q], and thread q NS + k needs it in slot a: a P x P transpose between the lane
+// bits of a and the register index, in log2(P) swap stages (stage i pairs lane bit NS 2^i with
+// register bit i).  Needs the line's S = L/P threads to be lanes of one wavefront in order.
+template <typename F, int P, int NS, int I = 1> MCS_DEV void lane_transpose(C2<F> (&v)[P]) {
+    if constexpr (I < P) {
+#pragma unroll
+        for (int j = 0; j < P; ++j)
+            if ((j & I) == 0) {
+                lane_swap<NS * I>(v[j].x, v[j + I].x);
+                lane_swap<NS * I>(v[j].y, v[j + I].y);
+            }
+        lane_transpose<F, P, NS, 2 * I>(v);
+    }
+}
+
 // ------------------------------------------------------------ fixed plans
 // The register engine with the whole plan known at compile time: length L and
 // radices Rs... in pass order (sub-lengths and twiddle-table offsets follow as
@@ -297,12 +367,37 @@ template <int L_, int... Rs> struct FixPlan {
     static constexpr int L = L_;
 };
 
-template <typename F, int P, int DIR, bool WAVE, int L, int Ns, int TwOff, int R, int... Rest>
+template <int R, int... Rest> constexpr int first_radix() { return R; }
+
+// whether the exchange before the last pass of plan (L, Rs...) can run as lane_transpose: the
+// last two radices equal P and a line's S = L/P threads are lanes of one wavefront
+template <int P, int L, int... Rs> constexpr bool fix_shuffle_ok() {
+    constexpr int n = sizeof...(Rs);
+    constexpr int r[] = {Rs...};
+    return n >= 2 && r[n - 1] == P && r[n - 2] == P && 64 % (L / P) == 0 && L % (P * P) == 0;
+}
+
+// SHUF: the exchange before the last pass through lane_transpose where fix_shuffle_ok holds
+// (the same values land in the same registers, so results are the same bits)
+template <typename F, int P, int DIR, bool WAVE, bool SHUF, int L, int Ns, int TwOff, int R, int... Rest>
 MCS_DEV void fix_pass(C2<F> (&a)[P], const C2<F>* __restrict__ tw, int t, C2<F>* lds, int lbase, int lps) {
     constexpr int S = L / P;
     constexpr int B = P / R;
     constexpr bool last = sizeof...(Rest) == 0;
     static_assert(P % R == 0, "every radix of a register plan divides P");
+    if constexpr (!last) {
+        constexpr bool shuffle = SHUF && WAVE && sizeof...(Rest) == 1 && R == P && first_radix<Rest...>() == P &&
+                                 64 % S == 0;
+        if constexpr (shuffle) {
+            const int k = Ns > 1 ? t % Ns : 0;
+            if constexpr (Ns > 1) apply_tw<F, R, DIR>(a, tw + TwOff, k);
+            dft<F, R, DIR>(a);
+            lane_transpose<F, P, Ns>(a);
+            fix_pass<F, P, DIR, WAVE, SHUF, L, Ns * R, TwOff + (Ns > 1 ? Ns * (R - 1) : 0), Rest...>(a, tw, t, lds,
+                                                                                               lbase, lps);
+            return;
+        }
+    }
 #pragma unroll
     for (int b = 0; b < B; ++b) {
         C2<F> v[R];
@@ -326,14 +421,22 @@ MCS_DEV void fix_pass(C2<F> (&a)[P], const C2<F>* __restrict__ tw, int t, C2<F>*
 #pragma unroll
         for (int m = 0; m < P; ++m) a[m] = lds[lbase + (t + m * S) * lps];
         lds_sync<WAVE>();
-        fix_pass<F, P, DIR, WAVE, L, Ns * R, TwOff + (Ns > 1 ? Ns * (R - 1) : 0), Rest...>(a, tw, t, lds, lbase, lps);
+        fix_pass<F, P, DIR, WAVE, SHUF, L, Ns * R, TwOff + (Ns > 1 ? Ns * (R - 1) : 0), Rest...>(a, tw, t, lds,
+                                                                                           lbase, lps);
     }
 }
 
 template <typename F, int P, int DIR, bool WAVE, int L, int... Rs>
 MCS_DEV void fix_fft(C2<F> (&a)[P], const C2<F>* __restrict__ tw, int t, C2<F>* lds, int lbase, int lps,
                      FixPlan<L, Rs...>) {
-    fix_pass<F, P, DIR, WAVE, L, 1, 0, Rs...>(a, tw, t, lds, lbase, lps);
+    fix_pass<F, P, DIR, WAVE, false, L, 1, 0, Rs...>(a, tw, t, lds, lbase, lps);
+}
+
+// the same with the last exchange through lane shuffles where the plan allows it
+template <typename F, int P, int DIR, bool WAVE, int L, int... Rs>
+MCS_DEV void fix_fft_shuf(C2<F> (&a)[P], const C2<F>* __restrict__ tw, int t, C2<F>* lds, int lbase, int lps,
+                          FixPlan<L, Rs...>) {
+    fix_pass<F, P, DIR, WAVE, true, L, 1, 0, Rs...>(a, tw, t, lds, lbase, lps);
 }
 
 // entries of a fixed plan's twiddle table (Ns (R - 1) per pass with Ns > 1)
@@ -343,11 +446,13 @@ template <int Ns, int R, int... Rest> constexpr int fix_tw_count_from() {
 }
 template <int L, int... Rs> constexpr int fix_tw_count(FixPlan<L, Rs...>) { return fix_tw_count_from<1, Rs...>(); }
 
-// the line transform of a pass body: runtime plan pl, or the fixed plan FP (pl unused)
-template <typename F, int P, int DIR, bool WAVE, typename FP>
+// the line transform of a pass body: runtime plan pl, or the fixed plan FP (pl unused); SHUF:
+// a fixed plan's last exchange through lane shuffles where the plan allows it
+template <typename F, int P, int DIR, bool WAVE, typename FP, bool SHUF = false>
 MCS_DEV void line_fft(C2<F> (&a)[P], const FftPlanDev& pl, const C2<F>* __restrict__ tw, int t, C2<F>* lds,
                       int lbase, int lps) {
-    if constexpr (FP::fixed) fix_fft<F, P, DIR, WAVE>(a, tw, t, lds, lbase, lps, FP{});
+    if constexpr (FP::fixed && SHUF) fix_fft_shuf<F, P, DIR, WAVE>(a, tw, t, lds, lbase, lps, FP{});
+    else if constexpr (FP::fixed) fix_fft<F, P, DIR, WAVE>(a, tw, t, lds, lbase, lps, FP{});
     else reg_fft<F, P, DIR, WAVE>(a, pl, tw, t, lds, lbase, lps);
 }
 

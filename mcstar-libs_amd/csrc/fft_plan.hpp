@@ -15,6 +15,7 @@ struct LinePlan {
     int P = 0;             // register engine: points per thread
     int G = 0;             // lines (or row pairs) per workgroup chosen by the tuner; 0 = default rule
     bool nofx = false;     // the tuner found the runtime-plan kernel faster than the fixed-plan one
+    bool shuf = false;     // the tuner found the fixed-plan kernel's lane-shuffle form faster
     FftPlanDev dev{};      // passed by value to kernels
     void* tw = nullptr;    // device twiddle table (C2<F>)
     size_t twcount = 0;

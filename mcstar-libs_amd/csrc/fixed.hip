@@ -32,6 +32,26 @@ extern "C" int mcs_set_fixed_kernels(int mask) {
     return MCS_OK;
 }
 
+int fixed_kernel_mask() { return g_fixed.load(); }
+
+// lane shuffles in the fixed-plan x passes: 0 never, 1 where the tuner timed them faster
+// (default), 2 wherever a plan has them
+static std::atomic<int> g_shuffle{1};
+
+extern "C" int mcs_set_lane_shuffle(int mode) {
+    MCS_CHECK(mode >= 0 && mode <= 2, MCS_ERR_INVALID, "lane shuffle mode is 0, 1 or 2");
+    g_shuffle.store(mode);
+    return MCS_OK;
+}
+
+extern "C" int mcs_get_lane_shuffle(int* mode) {
+    MCS_CHECK(mode, MCS_ERR_INVALID, "null argument");
+    *mode = g_shuffle.load();
+    return MCS_OK;
+}
+
+int lane_shuffle_mode() { return g_shuffle.load(); }
+
 extern "C" int mcs_get_fixed_kernels(int* mask) {
     MCS_CHECK(mask, MCS_ERR_INVALID, "null argument");
     *mask = g_fixed.load();
@@ -85,10 +105,11 @@ __global__ void __launch_bounds__(MCS_REG_MAX_THREADS(P, F)) k_col_fx(const C2<F
     fx_col_store<F, P, FP>(a, db, mst, off, valid);
 }
 
-// the z pass of the step: inverse transform, N = psi^3 (or psi^3 - psi), forward transform
+// the z pass of the step (the y pass in 2D): inverse transform, N = psi^3 (or psi^3 - psi),
+// forward transform, src -> W (src may be W)
 template <typename F, int P, int SPLIT, typename FP>
-__global__ void __launch_bounds__(MCS_REG_MAX_THREADS(P, F)) k_col_nl_fx(C2<F>* W, ColGeom g, int G,
-                                                                        const C2<F>* __restrict__ tw, F scale,
+__global__ void __launch_bounds__(MCS_REG_MAX_THREADS(P, F)) k_col_nl_fx(const C2<F>* src, C2<F>* W, ColGeom g,
+                                                                        int G, const C2<F>* __restrict__ tw, F scale,
                                                                         int64_t* d_step) {
     extern __shared__ unsigned char smem_raw[];
     C2<F>* lds = reinterpret_cast<C2<F>*>(smem_raw);
@@ -97,6 +118,7 @@ __global__ void __launch_bounds__(MCS_REG_MAX_THREADS(P, F)) k_col_nl_fx(C2<F>* 
     const int tile = blockIdx.x % tilesx, outer = blockIdx.x / tilesx;
     const int c = threadIdx.x % G, t = threadIdx.x / G;
     const bool valid = tile * G + c < g.nxc;
+    const char* sb = reinterpret_cast<const char*>(src + (long)outer * g.os + (long)tile * G);
     char* wb = reinterpret_cast<char*>(W + (long)outer * g.os + (long)tile * G);
     const long mst = (long)S * g.ls * (long)sizeof(C2<F>);
     const unsigned off = (unsigned)((c + (long)t * g.ls) * (long)sizeof(C2<F>));
@@ -106,7 +128,7 @@ __global__ void __launch_bounds__(MCS_REG_MAX_THREADS(P, F)) k_col_nl_fx(C2<F>* 
     for (int i = threadIdx.x; i < NTW; i += blockDim.x) twl[i] = tw[i];
     __syncthreads();
     C2<F> a[P];
-    fx_col_load<F, P, FP>(a, wb, mst, off, valid);
+    fx_col_load<F, P, FP>(a, sb, mst, off, valid);
     fix_fft<F, P, +1, false>(a, twl, t, lds, c, G, FP{});
 #pragma unroll
     for (int m = 0; m < P; ++m) {
@@ -119,7 +141,7 @@ __global__ void __launch_bounds__(MCS_REG_MAX_THREADS(P, F)) k_col_nl_fx(C2<F>* 
     if (d_step && blockIdx.x == 0 && threadIdx.x == 0) *d_step += 1;
 }
 
-template <typename F, int P, int SPLIT, bool WAVE, typename FP, bool NOISE>
+template <typename F, int P, int SPLIT, bool WAVE, typename FP, bool NOISE, bool SHUF>
 __global__ void __launch_bounds__(MCS_XPASS_MAX_THREADS) k_xupd_fx(C2<F>* __restrict__ W, C2<F>* __restrict__ U,
                                                  const int2* __restrict__ pairs, int npairs, int G, XArgs<F> ar,
                                                  const C2<F>* __restrict__ tw, const C2<F>* __restrict__ wph,
@@ -129,7 +151,7 @@ __global__ void __launch_bounds__(MCS_XPASS_MAX_THREADS) k_xupd_fx(C2<F>* __rest
     const FftPlanDev unused{};
     // (twiddles staged in LDS, also with a persistent grid that copies them once per
     // workgroup, did not make this pass faster: it keeps the global tables)
-    xupd_body<F, P, SPLIT, WAVE, FP, NOISE>(W, U, pairs, npairs, G, ar, unused, tw, wph, draw,
+    xupd_body<F, P, SPLIT, WAVE, FP, NOISE, SHUF>(W, U, pairs, npairs, G, ar, unused, tw, wph, draw,
                                             reinterpret_cast<C2<F>*>(smem_raw), blockIdx.x, threadIdx.x);
 }
 
@@ -147,10 +169,9 @@ struct ColEntry {
 struct XEntry {
     int prec, P, L, np;
     int rad[4];
-    const void* ns;   // psi^3 split, with the noise draw
-    const void* ref;  // psi^3 - psi split, with the noise draw
-    const void* ns_quiet;  // the same without it (noise off)
-    const void* ref_quiet;
+    // [shuffle][split][noise]: split 0 psi^3, 1 psi^3 - psi; noise 0 without the draw (noise off);
+    // the lane-shuffle forms are null where the plan has none
+    const void* fn[2][2][2];
 };
 
 template <typename F, int P, int L, int... Rs> static ColEntry col_entry() {
@@ -172,6 +193,7 @@ template <typename F, int P, int L, int... Rs> static ColEntry col_entry() {
 template <typename F, int P, int L, int... Rs> static XEntry x_entry() {
     using FP = FixPlan<L, Rs...>;
     constexpr bool wave = 64 % (L / P) == 0;
+    constexpr bool shuf = wave && fix_shuffle_ok<P, L, Rs...>();
     XEntry e{};
     e.prec = sizeof(F) == 8 ? 64 : 32;
     e.P = P;
@@ -179,10 +201,16 @@ template <typename F, int P, int L, int... Rs> static XEntry x_entry() {
     const int r[] = {Rs...};
     e.np = (int)sizeof...(Rs);
     for (int i = 0; i < e.np; ++i) e.rad[i] = r[i];
-    e.ns = (const void*)k_xupd_fx<F, P, MCS_SPLIT_NORTHSTAR, wave, FP, true>;
-    e.ref = (const void*)k_xupd_fx<F, P, MCS_SPLIT_REFERENCE, wave, FP, true>;
-    e.ns_quiet = (const void*)k_xupd_fx<F, P, MCS_SPLIT_NORTHSTAR, wave, FP, false>;
-    e.ref_quiet = (const void*)k_xupd_fx<F, P, MCS_SPLIT_REFERENCE, wave, FP, false>;
+    e.fn[0][0][1] = (const void*)k_xupd_fx<F, P, MCS_SPLIT_NORTHSTAR, wave, FP, true, false>;
+    e.fn[0][1][1] = (const void*)k_xupd_fx<F, P, MCS_SPLIT_REFERENCE, wave, FP, true, false>;
+    e.fn[0][0][0] = (const void*)k_xupd_fx<F, P, MCS_SPLIT_NORTHSTAR, wave, FP, false, false>;
+    e.fn[0][1][0] = (const void*)k_xupd_fx<F, P, MCS_SPLIT_REFERENCE, wave, FP, false, false>;
+    if constexpr (shuf) {
+        e.fn[1][0][1] = (const void*)k_xupd_fx<F, P, MCS_SPLIT_NORTHSTAR, wave, FP, true, true>;
+        e.fn[1][1][1] = (const void*)k_xupd_fx<F, P, MCS_SPLIT_REFERENCE, wave, FP, true, true>;
+        e.fn[1][0][0] = (const void*)k_xupd_fx<F, P, MCS_SPLIT_NORTHSTAR, wave, FP, false, true>;
+        e.fn[1][1][0] = (const void*)k_xupd_fx<F, P, MCS_SPLIT_REFERENCE, wave, FP, false, true>;
+    }
     return e;
 }
 
@@ -264,9 +292,9 @@ int fixed_col(const LinePlan& lp, const void* src, void* dst, const ColGeom& g, 
     return MCS_OK;
 }
 
-int fixed_step_col(const LinePlan& lp, void* W, const ColGeom& g, int kind, double scale, int split,
+int fixed_step_col(const LinePlan& lp, const void* src, void* W, const ColGeom& g, int kind, double scale, int split,
                    int64_t* d_step, hipStream_t s, bool* done) {
-    if (kind != 0) return fixed_col(lp, W, W, g, kind, s, done);
+    if (kind != 0) return fixed_col(lp, src, W, g, kind, s, done);
     *done = false;
     const ColEntry* e = col_match(lp, g);
     if (!e) return MCS_OK;
@@ -281,11 +309,11 @@ int fixed_step_col(const LinePlan& lp, void* W, const ColGeom& g, int kind, doub
     void* tw = lp.tw;
     if (lp.prec == 64) {
         double sc = scale;
-        void* args[] = {&W, (void*)&g, (void*)&G, &tw, &sc, &d_step};
+        void* args[] = {(void*)&src, &W, (void*)&g, (void*)&G, &tw, &sc, &d_step};
         MCS_HIP_TRY(hipLaunchKernel(fn, dim3((unsigned)tiles), dim3(G * S), args, lds, s));
     } else {
         float sc = (float)scale;
-        void* args[] = {&W, (void*)&g, (void*)&G, &tw, &sc, &d_step};
+        void* args[] = {(void*)&src, &W, (void*)&g, (void*)&G, &tw, &sc, &d_step};
         MCS_HIP_TRY(hipLaunchKernel(fn, dim3((unsigned)tiles), dim3(G * S), args, lds, s));
     }
     *done = true;
@@ -299,8 +327,9 @@ static int fixed_xupd_t(const XEntry& e, const StepGeom& sg, const StepPhys& ph,
     const int S = lp.L / lp.P;
     const int G = xupd_pairs_per_wg(lp);
     const size_t lds = (size_t)G * 2 * lp.L * sizeof(C2<F>);
-    const void* fn = ar.noise ? (ph.split == MCS_SPLIT_REFERENCE ? e.ref : e.ns)
-                              : (ph.split == MCS_SPLIT_REFERENCE ? e.ref_quiet : e.ns_quiet);
+    const int mode = g_shuffle.load();
+    const bool sh = e.fn[1][0][0] && (mode == 2 || (mode == 1 && lp.shuf));
+    const void* fn = e.fn[sh ? 1 : 0][ph.split == MCS_SPLIT_REFERENCE ? 1 : 0][ar.noise ? 1 : 0];
     int st = set_lds_fx(fn, lds);
     if (st) return st;
     const int2* pairs = sg.pairs;
@@ -439,6 +468,15 @@ int fixed_row(int mode, const LinePlan& lp, const void* wph, const void* src, vo
     MCS_HIP_TRY(hipLaunchKernel(fn, dim3((unsigned)((nrows + G - 1) / G)), dim3(G * (lp.L / lp.P)), args, lds, s));
     *done = true;
     return MCS_OK;
+}
+
+bool fixed_shuffle_available(const LinePlan& lp) {
+    if (!(g_fixed.load() & MCS_FIXED_XPASS) || g_shuffle.load() == 0) return false;
+    int n = 0;
+    const XEntry* t = x_table(&n);
+    for (int i = 0; i < n; ++i)
+        if (plan_matches(t[i], lp)) return t[i].fn[1][0][0] != nullptr;
+    return false;
 }
 
 bool fixed_available(const LinePlan& lp, bool xpass) {

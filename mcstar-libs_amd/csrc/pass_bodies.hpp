@@ -40,12 +40,12 @@ MCS_DEV void col_body(const C2<F>* src, C2<F>* dst, const ColGeom& g, int G, con
     }
 }
 
-// column pass with the nonlinearity between the inverse and forward transforms
-// (FP: RunPlan, the line plan pl, or a FixPlan):
+// column pass with the nonlinearity between the inverse and forward transforms, src -> W
+// (src may be W; FP: RunPlan, the line plan pl, or a FixPlan):
 // u = scale * value holds psi at (2m, 2m+1); N = psi^3 (or psi^3 - psi)
 template <typename F, int P, int SPLIT, typename FP = RunPlan>
-MCS_DEV void col_nl_body(C2<F>* W, const ColGeom& g, int G, const FftPlanDev& pl, const C2<F>* __restrict__ tw,
-                         F scale, C2<F>* lds, int tile, int outer, int tid) {
+MCS_DEV void col_nl_body(const C2<F>* src, C2<F>* W, const ColGeom& g, int G, const FftPlanDev& pl,
+                         const C2<F>* __restrict__ tw, F scale, C2<F>* lds, int tile, int outer, int tid) {
     const int S = (FP::fixed ? FP::L : pl.L) / P;
     const int c = tid % G, t = tid / G;
     const int x = tile * G + c;
@@ -54,7 +54,7 @@ MCS_DEV void col_nl_body(C2<F>* W, const ColGeom& g, int G, const FftPlanDev& pl
     const long base = (long)outer * g.os + x;
     C2<F> a[P];
 #pragma unroll
-    for (int m = 0; m < P; ++m) a[m] = valid ? W[base + (long)(t + m * S) * g.ls] : mk<F>(0, 0);
+    for (int m = 0; m < P; ++m) a[m] = valid ? src[base + (long)(t + m * S) * g.ls] : mk<F>(0, 0);
     line_fft<F, P, +1, false, FP>(a, pl, tw, tt, lds, c, G);
 #pragma unroll
     for (int m = 0; m < P; ++m) {
@@ -162,8 +162,10 @@ MCS_DEV void xupd_pair(C2<F> nA, C2<F> nBr, C2<F> pA, C2<F> pBr, int p, int iy, 
 // Each thread updates the mode pairs {(c, p), (m, -p)} of its own positions p once; the
 // mirror row's results move to their owners through LDS.  A self-mirrored row (c == m)
 // is processed as row c alone.
-// FP: RunPlan (the line plan pl) or a FixPlan of length L = nx/2; NOISE as for xupd_pair.
-template <typename F, int P, int SPLIT, bool WAVE = false, typename FP = RunPlan, bool NOISE = true>
+// FP: RunPlan (the line plan pl) or a FixPlan of length L = nx/2; NOISE as for xupd_pair;
+// SHUF: a fixed plan's last exchange of every line transform through lane shuffles (fix_fft_shuf).
+template <typename F, int P, int SPLIT, bool WAVE = false, typename FP = RunPlan, bool NOISE = true,
+          bool SHUF = false>
 MCS_DEV void xupd_body(C2<F>* __restrict__ W, C2<F>* __restrict__ U, const int2* __restrict__ pairs, int npairs,
                        int G, const XArgs<F>& ar, const FftPlanDev& pl, const C2<F>* __restrict__ tw,
                        const C2<F>* __restrict__ wph, uint32_t draw, C2<F>* lds, int blk, int tid) {
@@ -189,8 +191,8 @@ MCS_DEV void xupd_body(C2<F>* __restrict__ W, C2<F>* __restrict__ U, const int2*
         a[m] = valid ? W[rc * L + p] : mk<F>(0, 0);
         b[m] = valid ? W[rm * L + p] : mk<F>(0, 0);
     }
-    line_fft<F, P, -1, WAVE, FP>(a, pl, tw, t, lds, base, 1);
-    line_fft<F, P, -1, WAVE, FP>(b, pl, tw, t, lds, base + L, 1);
+    line_fft<F, P, -1, WAVE, FP, SHUF>(a, pl, tw, t, lds, base, 1);
+    line_fft<F, P, -1, WAVE, FP, SHUF>(b, pl, tw, t, lds, base + L, 1);
 #pragma unroll
     for (int m = 0; m < P; ++m) {
         const int p = t + m * S;
@@ -233,8 +235,8 @@ MCS_DEV void xupd_body(C2<F>* __restrict__ W, C2<F>* __restrict__ U, const int2*
 #pragma unroll
     for (int m = 0; m < P; ++m) b[m] = self ? a[m] : Nm[t + m * S];
     lds_sync<WAVE>();
-    line_fft<F, P, +1, WAVE, FP>(a, pl, tw, t, lds, base, 1);
-    line_fft<F, P, +1, WAVE, FP>(b, pl, tw, t, lds, base + L, 1);
+    line_fft<F, P, +1, WAVE, FP, SHUF>(a, pl, tw, t, lds, base, 1);
+    line_fft<F, P, +1, WAVE, FP, SHUF>(b, pl, tw, t, lds, base + L, 1);
     if (valid) {
 #pragma unroll
         for (int m = 0; m < P; ++m) {

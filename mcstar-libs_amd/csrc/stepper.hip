@@ -22,6 +22,7 @@
 #include "pass_bodies.hpp"
 #include "stepper.hpp"
 
+#include <atomic>
 #include <cmath>
 #include <vector>
 
@@ -35,11 +36,11 @@ __global__ void __launch_bounds__(MCS_REG_MAX_THREADS(P, F)) k_row_reg(const C2<
 }
 
 template <typename F, int P, int SPLIT>
-__global__ void __launch_bounds__(MCS_REG_MAX_THREADS(P, F)) k_col_nl(C2<F>* W, ColGeom g, int G, FftPlanDev pl,
-                                                 const C2<F>* __restrict__ tw, F scale, int64_t* d_step) {
+__global__ void __launch_bounds__(MCS_REG_MAX_THREADS(P, F)) k_col_nl(const C2<F>* src, C2<F>* W, ColGeom g, int G,
+                                                 FftPlanDev pl, const C2<F>* __restrict__ tw, F scale, int64_t* d_step) {
     extern __shared__ unsigned char smem_raw[];
     const int tilesx = (g.nxc + G - 1) / G;
-    col_nl_body<F, P, SPLIT>(W, g, G, pl, tw, scale, reinterpret_cast<C2<F>*>(smem_raw), blockIdx.x % tilesx,
+    col_nl_body<F, P, SPLIT>(src, W, g, G, pl, tw, scale, reinterpret_cast<C2<F>*>(smem_raw), blockIdx.x % tilesx,
                              blockIdx.x / tilesx, threadIdx.x);
     if (d_step && blockIdx.x == 0 && threadIdx.x == 0) *d_step += 1;
 }
@@ -119,8 +120,8 @@ int launch_row_packed(const LinePlan& lp, const void* src, void* dst, long nrows
 }
 
 template <typename F, int P, int SPLIT>
-static int launch_col_nl_t(const LinePlan& lp, void* W, const ColGeom& g, double scale, int64_t* d_step,
-                           hipStream_t s) {
+static int launch_col_nl_t(const LinePlan& lp, const void* src, void* W, const ColGeom& g, double scale,
+                           int64_t* d_step, hipStream_t s) {
     const int S = lp.L / P;
     const size_t esz = sizeof(C2<F>);
     const int G = col_lines_per_wg(lp);
@@ -129,25 +130,25 @@ static int launch_col_nl_t(const LinePlan& lp, void* W, const ColGeom& g, double
     int st = set_lds_attr((const void*)fn, lds);
     if (st) return st;
     const long tiles = (long)((g.nxc + G - 1) / G) * g.nouter;
-    hipLaunchKernelGGL(fn, dim3((unsigned)tiles), dim3(G * S), lds, s, (C2<F>*)W, g, G, lp.dev, (const C2<F>*)lp.tw,
-                       (F)scale, d_step);
+    hipLaunchKernelGGL(fn, dim3((unsigned)tiles), dim3(G * S), lds, s, (const C2<F>*)src, (C2<F>*)W, g, G, lp.dev,
+                       (const C2<F>*)lp.tw, (F)scale, d_step);
     MCS_HIP_TRY(hipGetLastError());
     return MCS_OK;
 }
 
 template <typename F, int SPLIT>
-static int launch_col_nl_d(const LinePlan& lp, void* W, const ColGeom& g, double scale, int64_t* d_step,
-                           hipStream_t s) {
+static int launch_col_nl_d(const LinePlan& lp, const void* src, void* W, const ColGeom& g, double scale,
+                           int64_t* d_step, hipStream_t s) {
     switch (lp.P) {
-        case 4: return launch_col_nl_t<F, 4, SPLIT>(lp, W, g, scale, d_step, s);
-        case 8: return launch_col_nl_t<F, 8, SPLIT>(lp, W, g, scale, d_step, s);
-        case 6: return launch_col_nl_t<F, 6, SPLIT>(lp, W, g, scale, d_step, s);
-        case 10: return launch_col_nl_t<F, 10, SPLIT>(lp, W, g, scale, d_step, s);
-        case 12: return launch_col_nl_t<F, 12, SPLIT>(lp, W, g, scale, d_step, s);
-        case 20: return launch_col_nl_t<F, 20, SPLIT>(lp, W, g, scale, d_step, s);
-        case 16: return launch_col_nl_t<F, 16, SPLIT>(lp, W, g, scale, d_step, s);
-        case 32: return launch_col_nl_t<F, 32, SPLIT>(lp, W, g, scale, d_step, s);
-        case 30: return launch_col_nl_t<F, 30, SPLIT>(lp, W, g, scale, d_step, s);
+        case 4: return launch_col_nl_t<F, 4, SPLIT>(lp, src, W, g, scale, d_step, s);
+        case 8: return launch_col_nl_t<F, 8, SPLIT>(lp, src, W, g, scale, d_step, s);
+        case 6: return launch_col_nl_t<F, 6, SPLIT>(lp, src, W, g, scale, d_step, s);
+        case 10: return launch_col_nl_t<F, 10, SPLIT>(lp, src, W, g, scale, d_step, s);
+        case 12: return launch_col_nl_t<F, 12, SPLIT>(lp, src, W, g, scale, d_step, s);
+        case 20: return launch_col_nl_t<F, 20, SPLIT>(lp, src, W, g, scale, d_step, s);
+        case 16: return launch_col_nl_t<F, 16, SPLIT>(lp, src, W, g, scale, d_step, s);
+        case 32: return launch_col_nl_t<F, 32, SPLIT>(lp, src, W, g, scale, d_step, s);
+        case 30: return launch_col_nl_t<F, 30, SPLIT>(lp, src, W, g, scale, d_step, s);
         default: mcs_set_last_error(__FILE__, __LINE__, "column plan not register-eligible"); return MCS_ERR_UNSUPPORTED;
     }
 }
@@ -155,23 +156,29 @@ static int launch_col_nl_d(const LinePlan& lp, void* W, const ColGeom& g, double
 // a column pass of the stepper on W in place: kind -1 forward, +1 inverse, 0 nonlinear
 int launch_step_col(const LinePlan& lp, void* W, const ColGeom& g, int kind, double scale, int split,
                     int64_t* d_step, hipStream_t s) {
-    if (kind == 0) return launch_col_nl(lp, W, g, scale, split, d_step, s);
-    bool done = false;
-    const int st = fixed_step_col(lp, W, g, kind, scale, split, d_step, s, &done);
-    if (st || done) return st;
-    return launch_col(lp, W, W, g, kind, s);
+    return launch_step_col_to(lp, W, W, g, kind, scale, split, d_step, s);
 }
 
-int launch_col_nl(const LinePlan& lp, void* W, const ColGeom& g, double scale, int split, int64_t* d_step,
-                  hipStream_t s) {
+int launch_step_col_to(const LinePlan& lp, const void* src, void* W, const ColGeom& g, int kind, double scale,
+                       int split, int64_t* d_step, hipStream_t s) {
+    if (kind == 0) return launch_col_nl(lp, src, W, g, scale, split, d_step, s);
     bool done = false;
-    const int st = fixed_step_col(lp, W, g, 0, scale, split, d_step, s, &done);
+    const int st = fixed_step_col(lp, src, W, g, kind, scale, split, d_step, s, &done);
+    if (st || done) return st;
+    return launch_col(lp, src, W, g, kind, s);
+}
+
+int launch_col_nl(const LinePlan& lp, const void* src, void* W, const ColGeom& g, double scale, int split,
+                  int64_t* d_step, hipStream_t s) {
+    bool done = false;
+    const int st = fixed_step_col(lp, src, W, g, 0, scale, split, d_step, s, &done);
     if (st || done) return st;
     if (lp.prec == 64)
-        return split == MCS_SPLIT_REFERENCE ? launch_col_nl_d<double, MCS_SPLIT_REFERENCE>(lp, W, g, scale, d_step, s)
-                                            : launch_col_nl_d<double, MCS_SPLIT_NORTHSTAR>(lp, W, g, scale, d_step, s);
-    return split == MCS_SPLIT_REFERENCE ? launch_col_nl_d<float, MCS_SPLIT_REFERENCE>(lp, W, g, scale, d_step, s)
-                                        : launch_col_nl_d<float, MCS_SPLIT_NORTHSTAR>(lp, W, g, scale, d_step, s);
+        return split == MCS_SPLIT_REFERENCE
+                   ? launch_col_nl_d<double, MCS_SPLIT_REFERENCE>(lp, src, W, g, scale, d_step, s)
+                   : launch_col_nl_d<double, MCS_SPLIT_NORTHSTAR>(lp, src, W, g, scale, d_step, s);
+    return split == MCS_SPLIT_REFERENCE ? launch_col_nl_d<float, MCS_SPLIT_REFERENCE>(lp, src, W, g, scale, d_step, s)
+                                        : launch_col_nl_d<float, MCS_SPLIT_NORTHSTAR>(lp, src, W, g, scale, d_step, s);
 }
 
 template <typename F, int P, int SPLIT>
@@ -272,18 +279,25 @@ ColGeom step_geom_z(const StepGeom& sg) { return ColGeom{sg.L, (long)sg.L * sg.n
 static ColGeom geom_y(const StepGeom& sg) { return step_geom_y(sg); }
 static ColGeom geom_z(const StepGeom& sg) { return step_geom_z(sg); }
 
-// W holds IFFT_x(U) on entry; leaves FFT_yz(N(psi)) in W
-int step_nonlinear_half(const StepGeom& sg, const StepPhys& ph, void* W, int64_t* d_step, hipStream_t s) {
-    const double scale = 2.0 / ((double)sg.nx * sg.ny * sg.nz);
+// src holds x-real rows (IFFT_x(U) in W in the spectral form, the state S in the mixed form);
+// leaves FFT_yz(N(psi)) in W
+int step_nonlinear_half(const StepGeom& sg, const StepPhys& ph, const void* src, void* W, int64_t* d_step,
+                        hipStream_t s) {
+    const double scale = step_nl_scale(sg);
     int st;
     if (sg.is3d) {
-        st = launch_step_col(sg.py, W, geom_y(sg), +1, scale, ph.split, nullptr, s);
+        st = launch_step_col_to(sg.py, src, W, geom_y(sg), +1, scale, ph.split, nullptr, s);
         if (!st) st = launch_step_col(sg.pz, W, geom_z(sg), 0, scale, ph.split, d_step, s);
         if (!st) st = launch_step_col(sg.py, W, geom_y(sg), -1, scale, ph.split, nullptr, s);
     } else {
-        st = launch_step_col(sg.py, W, geom_y(sg), 0, scale, ph.split, d_step, s);
+        st = launch_step_col_to(sg.py, src, W, geom_y(sg), 0, scale, ph.split, d_step, s);
     }
     return st;
+}
+
+// the rows the first column pass of a step reads: W (spectral form) or the state (mixed form)
+static const void* step_rows(const StepGeom& sg, const void* W, const void* U) {
+    return sg.form == MCS_FORM_MIXED ? U : W;
 }
 
 // the same passes as step_run with an event after each; returns per-pass milliseconds summed over nsteps
@@ -294,21 +308,22 @@ int step_run_timed(const StepGeom& sg, const StepPhys& ph, void* W, void* U, int
     hipEvent_t ev[5];
     for (int i = 0; i <= np; ++i) MCS_HIP_TRY(hipEventCreate(&ev[i]));
     for (int i = 0; i < np; ++i) acc[i] = 0;
-    const double scale = 2.0 / ((double)sg.nx * sg.ny * sg.nz);
+    const double scale = step_nl_scale(sg);
+    const void* rows = step_rows(sg, W, U);
     int st = MCS_OK;
     for (int it = 0; it < nsteps && !st; ++it) {
         (void)hipEventRecord(ev[0], s);
-        st = launch_xupd(sg, ph, W, U, s);
+        st = launch_xpass(sg, ph, W, U, s);
         (void)hipEventRecord(ev[1], s);
         if (sg.is3d) {
-            if (!st) st = launch_step_col(sg.py, W, geom_y(sg), +1, scale, ph.split, nullptr, s);
+            if (!st) st = launch_step_col_to(sg.py, rows, W, geom_y(sg), +1, scale, ph.split, nullptr, s);
             (void)hipEventRecord(ev[2], s);
             if (!st) st = launch_step_col(sg.pz, W, geom_z(sg), 0, scale, ph.split, sg.d_step, s);
             (void)hipEventRecord(ev[3], s);
             if (!st) st = launch_step_col(sg.py, W, geom_y(sg), -1, scale, ph.split, nullptr, s);
             (void)hipEventRecord(ev[4], s);
         } else {
-            if (!st) st = launch_step_col(sg.py, W, geom_y(sg), 0, scale, ph.split, sg.d_step, s);
+            if (!st) st = launch_step_col_to(sg.py, rows, W, geom_y(sg), 0, scale, ph.split, sg.d_step, s);
             (void)hipEventRecord(ev[2], s);
         }
         if (!st && hipEventSynchronize(ev[np]) != hipSuccess) st = MCS_ERR_HIP;
@@ -323,22 +338,33 @@ int step_run_timed(const StepGeom& sg, const StepPhys& ph, void* W, void* U, int
 }
 
 int step_run(const StepGeom& sg, const StepPhys& ph, void* W, void* U, int nsteps, hipStream_t s) {
+    const void* rows = step_rows(sg, W, U);
     for (int i = 0; i < nsteps; ++i) {
-        int st = launch_xupd(sg, ph, W, U, s);
-        if (!st) st = step_nonlinear_half(sg, ph, W, sg.d_step, s);
+        int st = launch_xpass(sg, ph, W, U, s);
+        if (!st) st = step_nonlinear_half(sg, ph, rows, W, sg.d_step, s);
         if (st) return st;
     }
     return MCS_OK;
 }
 
-// psi (real, reference layout, N values) -> U; W primed with the nonlinear term; returns sum(psi) in *mass
+// psi (real, reference layout, N values) -> the state U; W primed with the nonlinear term;
+// returns sum(psi) in *mass.  Spectral form: U = FFT_yz(FFT_x(u)); mixed form: U = FFT_yz(u)
+// (u is psi read as N/2 complex values)
 int step_import(const StepGeom& sg, const StepPhys& ph, const void* psi, void* W, void* U, double* mass,
                 hipStream_t s) {
     const long rows = (long)sg.ny * sg.nz;
     const long n = (long)sg.nx * sg.ny * sg.nz;
-    int st = launch_row_packed(sg.px, psi, U, rows, -1, s);
-    if (!st && sg.is3d) st = launch_col(sg.pz, U, U, geom_z(sg), -1, s);
-    if (!st) st = launch_col(sg.py, U, U, geom_y(sg), -1, s);
+    int st = MCS_OK;
+    const void* src = psi;
+    if (sg.form != MCS_FORM_MIXED) {
+        st = launch_row_packed(sg.px, psi, U, rows, -1, s);
+        src = U;
+    }
+    if (!st && sg.is3d) {
+        st = launch_col(sg.pz, src, U, geom_z(sg), -1, s);
+        src = U;
+    }
+    if (!st) st = launch_col(sg.py, src, U, geom_y(sg), -1, s);
     if (st) return st;
     double* part = nullptr;
     MCS_HIP_TRY(hipMalloc((void**)&part, 1024 * sizeof(double)));
@@ -354,21 +380,32 @@ int step_import(const StepGeom& sg, const StepPhys& ph, const void* psi, void* W
     double m = 0;
     for (double v : hp) m += v;
     *mass = m;
-    st = launch_row_packed(sg.px, U, W, rows, +1, s);
-    if (!st) st = step_nonlinear_half(sg, ph, W, nullptr, s);
+    return step_prime(sg, ph, W, U, s);
+}
+
+// W as the last pass of a step leaves it (FFT_yz(N(psi)) in x-real layout), from the state U
+int step_prime(const StepGeom& sg, const StepPhys& ph, void* W, const void* U, hipStream_t s) {
+    if (sg.form == MCS_FORM_MIXED) return step_nonlinear_half(sg, ph, U, W, nullptr, s);
+    int st = launch_row_packed(sg.px, U, W, (long)sg.ny * sg.nz, +1, s);
+    if (!st) st = step_nonlinear_half(sg, ph, W, W, nullptr, s);
     return st;
 }
 
-// U -> psi (real, reference layout); tmp must hold N/2 complex values
+// the state U -> psi (real, reference layout); tmp must hold N/2 complex values
 int step_export(const StepGeom& sg, const void* U, void* psi, void* tmp, hipStream_t s) {
     const long rows = (long)sg.ny * sg.nz;
     const long n = (long)sg.nx * sg.ny * sg.nz;
-    int st = launch_row_packed(sg.px, U, tmp, rows, +1, s);
-    if (!st) st = launch_col(sg.py, tmp, tmp, geom_y(sg), +1, s);
+    int st = MCS_OK;
+    const void* src = U;
+    if (sg.form != MCS_FORM_MIXED) {
+        st = launch_row_packed(sg.px, U, tmp, rows, +1, s);
+        src = tmp;
+    }
+    if (!st) st = launch_col(sg.py, src, tmp, geom_y(sg), +1, s);
     if (!st && sg.is3d) st = launch_col(sg.pz, tmp, tmp, geom_z(sg), +1, s);
     if (st) return st;
-    // the packed inverse over N/2 points returns (N/2) u
-    const double sc = 2.0 / (double)n;
+    // the packed inverse over N/2 points returns (N/2) u; over y and z of the mixed state, ny nz u
+    const double sc = step_nl_scale(sg);
     if (sg.prec == 64)
         hipLaunchKernelGGL(k_scale_copy<double>, dim3(2048), dim3(256), 0, s, (const double*)tmp, (double*)psi, n, sc);
     else
@@ -378,16 +415,56 @@ int step_export(const StepGeom& sg, const void* U, void* psi, void* tmp, hipStre
     return MCS_OK;
 }
 
+// converts the state U in place to the other form: spectral -> mixed IFFT_x and 1/L,
+// mixed -> spectral FFT_x (W, the nonlinear term, is the same in both forms)
+int step_convert_form(StepGeom* sg, void* U, int form, hipStream_t s) {
+    if (form == sg->form) return MCS_OK;
+    const long rows = (long)sg->ny * sg->nz;
+    const long n = (long)sg->nx * sg->ny * sg->nz;
+    int st;
+    if (form == MCS_FORM_MIXED) {
+        st = launch_row_packed(sg->px, U, U, rows, +1, s);
+        if (st) return st;
+        const double sc = 1.0 / sg->L;
+        if (sg->prec == 64)
+            hipLaunchKernelGGL(k_scale_copy<double>, dim3(2048), dim3(256), 0, s, (const double*)U, (double*)U, n, sc);
+        else
+            hipLaunchKernelGGL(k_scale_copy<float>, dim3(2048), dim3(256), 0, s, (const float*)U, (float*)U, n,
+                               (float)sc);
+        MCS_HIP_TRY(hipGetLastError());
+    } else {
+        st = launch_row_packed(sg->px, U, U, rows, -1, s);
+        if (st) return st;
+    }
+    sg->form = form;
+    return MCS_OK;
+}
+
 // ------------------------------------------------------------------ C ABI
 struct mcs_stepper {
     mcs_pfc_params p;
     StepGeom sg;
     StepPhys ph;
-    void* U = nullptr;
+    void* U = nullptr;  // the state (U in the spectral form, S in the mixed form)
     void* W = nullptr;
     bool loaded = false;
+    bool planned = false;  // explicit plans (not tuned)
     int64_t step = 0;
 };
+
+static std::atomic<int> g_form{MCS_FORM_SPECTRAL};
+
+extern "C" int mcs_set_step_form(int form) {
+    MCS_CHECK(form == MCS_FORM_SPECTRAL || form == MCS_FORM_MIXED, MCS_ERR_INVALID, "unknown step form");
+    g_form.store(form);
+    return MCS_OK;
+}
+
+extern "C" int mcs_get_step_form(int* form) {
+    MCS_CHECK(form, MCS_ERR_INVALID, "null argument");
+    *form = g_form.load();
+    return MCS_OK;
+}
 
 static int stepper_create(mcs_stepper** out, const mcs_pfc_params* p, const int* plan) {
     MCS_CHECK(out && p, MCS_ERR_INVALID, "null argument");
@@ -412,6 +489,8 @@ static int stepper_create(mcs_stepper** out, const mcs_pfc_params* p, const int*
     st->ph.h[0] = p->h[0]; st->ph.h[1] = p->h[1]; st->ph.h[2] = p->h[2];
     st->ph.eps = p->eps; st->ph.dt = p->dt; st->ph.temp = p->temp; st->ph.seed = p->seed; st->ph.split = p->split;
     st->ph.mass = 0;
+    st->sg.form = g_form.load();
+    st->planned = plan != nullptr;
     rc = plan ? step_geom_apply_plan(&st->sg, plan) : step_geom_tune(&st->sg, st->ph, st->W, st->U, mcs_stream());
     if (rc) { mcs_stepper_destroy(st); return rc; }
     *out = st;
@@ -445,8 +524,7 @@ extern "C" int mcs_stepper_set_state(mcs_stepper* st, const void* spec, double m
     st->step = step;
     MCS_HIP_TRY(hipMemcpyAsync(st->sg.d_step, &st->step, sizeof(int64_t), hipMemcpyHostToDevice, s));
     // W as the last pass of a step leaves it: FFT_yz(N(psi)) in x-real layout
-    int rc = launch_row_packed(st->sg.px, st->U, st->W, (long)st->sg.ny * st->sg.nz, +1, s);
-    if (!rc) rc = step_nonlinear_half(st->sg, st->ph, st->W, nullptr, s);
+    int rc = step_prime(st->sg, st->ph, st->W, st->U, s);
     if (rc) return rc;
     MCS_HIP_TRY(hipStreamSynchronize(s));
     st->loaded = true;
@@ -456,6 +534,32 @@ extern "C" int mcs_stepper_set_state(mcs_stepper* st, const void* spec, double m
 extern "C" int mcs_stepper_plan_info(mcs_stepper* st, int info[6]) {
     MCS_CHECK(st && info, MCS_ERR_INVALID, "null argument");
     step_geom_plan_info(st->sg, info);
+    return MCS_OK;
+}
+
+extern "C" int mcs_stepper_set_form(mcs_stepper* st, int form) {
+    MCS_CHECK(st, MCS_ERR_INVALID, "null argument");
+    MCS_CHECK(form == MCS_FORM_SPECTRAL || form == MCS_FORM_MIXED, MCS_ERR_INVALID, "unknown step form");
+    if (form == st->sg.form) return MCS_OK;
+    hipStream_t s = mcs_stream();
+    if (st->loaded) {
+        int rc = step_convert_form(&st->sg, st->U, form, s);
+        if (rc) return rc;
+        MCS_HIP_TRY(hipStreamSynchronize(s));
+        return MCS_OK;
+    }
+    st->sg.form = form;
+    // no field yet: the buffers are free, so tuned plans are tuned again for this form's x pass
+    if (!st->planned) {
+        int rc = step_geom_tune(&st->sg, st->ph, st->W, st->U, s);
+        if (rc) return rc;
+    }
+    return MCS_OK;
+}
+
+extern "C" int mcs_stepper_get_form(mcs_stepper* st, int* form) {
+    MCS_CHECK(st && form, MCS_ERR_INVALID, "null argument");
+    *form = st->sg.form;
     return MCS_OK;
 }
 

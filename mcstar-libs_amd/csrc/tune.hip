@@ -27,6 +27,7 @@ struct PassChoice {
     int P = 0, G = 0;
     double ms = 0;
     bool nofx = false;  // runtime-plan kernel (where a fixed-plan one exists and was slower)
+    bool shuf = false;  // the fixed-plan kernel's lane-shuffle form (where it has one and was faster)
 };
 struct StepChoice {
     PassChoice x, y, z;
@@ -102,21 +103,28 @@ int rebuild(LinePlan* lp, int L, int prec, int P) {
 // best (P, G) of one pass; `run` launches the pass with the candidate plan; with_fixed: also time
 // the fixed-plan kernel of each candidate that has one
 template <typename Run>
-int tune_pass(int L, int prec, int maxP, bool xpass, bool with_fixed, hipStream_t s, Run&& run, PassChoice* best) {
+int tune_pass(int L, int prec, int maxP, bool xpass, bool with_fixed, int xform, hipStream_t s, Run&& run,
+              PassChoice* best) {
     best->ms = -1;
     const std::vector<int> opts = line_plan_reg_options(L, prec);
     for (int P : opts) {
         if (P > maxP && P != opts[0]) continue;  // the default plan is always a candidate
         LinePlan cand;
         if (line_plan_build_reg(&cand, L, prec, P) != MCS_OK) continue;
-        const bool fx = with_fixed && fixed_available(cand, xpass);
+        const bool fx = with_fixed && (xpass && xform == MCS_FORM_MIXED ? fixed_mixed_available(cand)
+                                                                         : fixed_available(cand, xpass));
+        const bool sh = fx && xpass && (xform == MCS_FORM_MIXED ? fixed_mixed_shuffle_available(cand)
+                                                                 : fixed_shuffle_available(cand));
         for (int G : xpass ? x_tiles(L, P, prec) : col_tiles(L, P, prec)) {
             cand.G = G;
-            // the fixed-plan kernel (when one exists) and the runtime-plan kernel of this plan
-            for (int nofx = fx ? 0 : 1; nofx <= 1; ++nofx) {
-                cand.nofx = nofx != 0;
+            // the fixed-plan kernel (when one exists; with and without lane shuffles where it has
+            // them) and the runtime-plan kernel of this plan
+            for (int v = fx ? 0 : 2; v <= 2; ++v) {
+                if (v == 1 && !sh) continue;
+                cand.nofx = v == 2;
+                cand.shuf = v == 1;
                 const double ms = time_launch(s, [&]() { return run(cand); });
-                if (ms > 0 && (best->ms < 0 || ms < best->ms)) *best = PassChoice{P, G, ms, cand.nofx};
+                if (ms > 0 && (best->ms < 0 || ms < best->ms)) *best = PassChoice{P, G, ms, cand.nofx, cand.shuf};
             }
         }
         line_plan_free(&cand);
@@ -135,6 +143,7 @@ int apply(StepGeom* sg, const StepChoice& c) {
     sg->py.G = c.y.G;
     sg->px.nofx = c.x.nofx;
     sg->py.nofx = c.y.nofx;
+    sg->px.shuf = c.x.shuf;
     if (sg->is3d) {
         sg->pz.G = c.z.G;
         sg->pz.nofx = c.z.nofx;
@@ -158,7 +167,7 @@ extern "C" int mcs_get_autotune(int* on) {
 }
 
 int step_geom_tune_runs(StepGeom* sg, int tag, int fixed, const PassRun& run_x, const PassRun& run_y,
-                        const PassRun& run_z, hipStream_t s) {
+                        const PassRun& run_z, hipStream_t s, int xform) {
     const auto key = std::make_tuple(sg->nx, sg->ny, sg->nz, sg->prec, tag);
     {
         std::lock_guard<std::mutex> l(g_mu);
@@ -178,9 +187,9 @@ int step_geom_tune_runs(StepGeom* sg, int tag, int fixed, const PassRun& run_x, 
     (void)hipGetLastError();
     StepChoice c;
     const bool fx = (fixed & MCS_FIXED_XPASS) != 0, fc = (fixed & MCS_FIXED_COLUMNS) != 0;
-    int st = tune_pass(sg->L, prec, maxX, true, fx, s, run_x, &c.x);
-    if (!st) st = tune_pass(sg->ny, prec, maxC, false, fc, s, run_y, &c.y);
-    if (!st && sg->is3d) st = tune_pass(sg->nz, prec, maxC, false, fc, s, run_z, &c.z);
+    int st = tune_pass(sg->L, prec, maxX, true, fx, xform, s, run_x, &c.x);
+    if (!st) st = tune_pass(sg->ny, prec, maxC, false, fc, xform, s, run_y, &c.y);
+    if (!st && sg->is3d) st = tune_pass(sg->nz, prec, maxC, false, fc, xform, s, run_z, &c.z);
     if (st) return st;
     MCS_HIP_TRY(hipStreamSynchronize(s));
     {
@@ -195,24 +204,27 @@ int step_geom_tune_runs(StepGeom* sg, int tag, int fixed, const PassRun& run_x, 
 int step_geom_tune(StepGeom* sg, const StepPhys& ph_in, void* W, void* U, hipStream_t s) {
     StepPhys ph = ph_in;
     ph.mass = 0;
-    const double scale = 2.0 / ((double)sg->nx * sg->ny * sg->nz);
+    const double scale = step_nl_scale(*sg);
     const ColGeom gy = step_geom_y(*sg), gz = step_geom_z(*sg);
     const bool is3d = sg->is3d;
-    // noisy and noise-free steps run different x-pass kernels (the draw costs registers), so
-    // each tunes on its own
-    const int noisy = ph.temp > 0 ? 1 : 0;
+    const int form = sg->form;
+    // noisy and noise-free steps run different x-pass kernels (the draw costs registers), and
+    // the two forms different x passes, so each tunes on its own; the 2D nonlinear pass of the
+    // mixed form reads the state and writes W
+    const int tag = (ph.temp > 0 ? 1 : 0) + 2 * form;
+    const void* rows = form == MCS_FORM_MIXED ? U : W;
     return step_geom_tune_runs(
-        sg, noisy, MCS_FIXED_COLUMNS | MCS_FIXED_XPASS,
+        sg, tag, MCS_FIXED_COLUMNS | MCS_FIXED_XPASS,
         [&](const LinePlan& lp) {
             StepGeom g = *sg;
             g.px = lp;
-            return launch_xupd(g, ph, W, U, s);
+            return launch_xpass(g, ph, W, U, s);
         },
         [&](const LinePlan& lp) {
             return is3d ? launch_step_col(lp, W, gy, -1, scale, ph.split, nullptr, s)
-                        : launch_col_nl(lp, W, gy, scale, ph.split, nullptr, s);
+                        : launch_col_nl(lp, rows, W, gy, scale, ph.split, nullptr, s);
         },
-        [&](const LinePlan& lp) { return launch_col_nl(lp, W, gz, scale, ph.split, nullptr, s); }, s);
+        [&](const LinePlan& lp) { return launch_col_nl(lp, W, W, gz, scale, ph.split, nullptr, s); }, s, form);
 }
 
 int step_plan_validate(const int ext[3], int prec, const int plan[6]) {
@@ -261,7 +273,7 @@ extern "C" int mcs_tile_options(int L, int prec, int P, int xpass, int* G, int c
 
 void step_geom_plan_info(const StepGeom& sg, int info[6]) {
     info[0] = sg.px.P;
-    info[1] = xupd_pairs_per_wg(sg.px);
+    info[1] = sg.form == MCS_FORM_MIXED ? xupd_mixed_pairs_per_wg(sg.px) : xupd_pairs_per_wg(sg.px);
     info[2] = sg.py.P;
     info[3] = col_lines_per_wg(sg.py);
     info[4] = sg.is3d ? sg.pz.P : 0;

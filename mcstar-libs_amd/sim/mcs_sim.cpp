@@ -120,8 +120,9 @@ Opts parse(int argc, char** argv) {
 
 // ------------------------------------------------------------------ checkpoint
 // DIR/checkpoint.txt holds the parameters, the step, the FP64 mass (printed
-// exactly) and the plans; DIR/state.npy the packed spectrum (complex, numpy
-// shape (nz, ny, nx/2)).
+// exactly), the plans and the state's form; DIR/state.npy the stepper's state
+// (complex, numpy shape (nz, ny, nx/2): the packed spectrum in the spectral
+// form, its inverse x transform over nx/2 in the mixed form).
 typedef std::complex<Float> Cplx;
 
 void save_checkpoint(const Opts& o, mcs_stepper* st) {
@@ -130,23 +131,25 @@ void save_checkpoint(const Opts& o, mcs_stepper* st) {
     double mass = 0;
     int64_t step = 0;
     CUERR(mcs_stepper_get_state(st, spec.data(), &mass, &step));
-    int plan[6];
+    int plan[6], form = 0;
     CUERR(mcs_stepper_plan_info(st, plan));
+    CUERR(mcs_stepper_get_form(st, &form));
     const int shp[3] = {d.z, d.y, d.x / 2};
     aoba::SaveArrayAsNumpy(o.out + "/state.npy", 3, shp, spec.data());
     FILE* f = std::fopen((o.out + "/checkpoint.txt").c_str(), "w");
     if (!f) { std::cerr << "cannot write " << o.out << "/checkpoint.txt" << std::endl; std::exit(1); }
     std::fprintf(f, "dims %dx%dx%d\nprecision %d\nsplit %d\nh %.17g\neps %.17g\ndt %.17g\ntemp %.17g\n", d.x, d.y,
                  d.z, (int)(8 * sizeof(Float)), o.split, o.h, o.eps, o.dt, o.temp);
-    std::fprintf(f, "noise_seed %llu\nstep %lld\nmass %.17g\nplan %d,%d,%d,%d,%d,%d\n",
+    std::fprintf(f, "noise_seed %llu\nstep %lld\nmass %.17g\nplan %d,%d,%d,%d,%d,%d\nform %d\n",
                  (unsigned long long)o.noise_seed, (long long)step, mass, plan[0], plan[1], plan[2], plan[3],
-                 plan[4], plan[5]);
+                 plan[4], plan[5], form);
     std::fclose(f);
 }
 
 struct Resume {
     int64_t step = 0;
     double mass = 0;
+    int form = MCS_FORM_SPECTRAL;  // checkpoints without a form line hold the packed spectrum
     std::vector<Cplx> spec;
 };
 
@@ -170,6 +173,7 @@ Resume read_checkpoint(Opts& o) {
     Resume r;
     r.step = std::strtoll(kv["step"].c_str(), nullptr, 10);
     r.mass = std::strtod(kv["mass"].c_str(), nullptr);
+    if (kv.count("form")) r.form = std::atoi(kv["form"].c_str());
     std::vector<int> shape;
     aoba::LoadArrayFromNumpy(o.resume + "/state.npy", shape, r.spec);
     if (shape.size() != 3 || shape[0] != o.dims.z || shape[1] != o.dims.y || shape[2] != o.dims.x / 2)
@@ -253,6 +257,7 @@ int main(int argc, char** argv) {
     std::unique_ptr<Lowpass> lp(o.lowpass > 0 ? new Lowpass(o) : nullptr);
     int64_t step0 = 0;
     if (resuming) {
+        CUERR(mcs_stepper_set_form(st, res.form));
         CUERR(mcs_stepper_set_state(st, res.spec.data(), res.mass, res.step));
         step0 = res.step;
     } else {

@@ -156,18 +156,20 @@ def test_precision_study_64cubed_500_steps(mcs, orc):
     assert h4.sum() == ijk.shape[0] and h6.sum() == ijk.shape[0]
 
 
-@pytest.mark.parametrize("dtype", [np.float32, np.float64])
-def test_precision_study_512cubed_500_steps_against_oracle_fixture(mcs, dtype):
+@pytest.mark.parametrize("dtype,form", [(np.float32, 0), (np.float64, 0), (np.float32, 1)])
+def test_precision_study_512cubed_500_steps_against_oracle_fixture(mcs, dtype, form):
     """the headline precision gate: FP32 (FFT and pointwise in FP32, twiddles from FP64, k = 0 in
     FP64) within rel-L2 1e-4 and max-abs 5e-3 of the FP64 oracle after 500 steps at 512^3; the
-    all-FP64 GPU form within 1e-10 (cuda/kernels/eom.hpp:60-72 is the update both run)"""
+    all-FP64 GPU form within 1e-10 (cuda/kernels/eom.hpp:60-72 is the update both run).  form 0 is
+    the stepper's default (spectral) state; the mixed form (form 1: the state passes through an x
+    transform and back every step) is held to its own stated bound, rel-L2 2e-4"""
     g = np.load(os.path.join(GOLD, "c3_oracle_512.npz"))
     n, eps, dt, psi0, amp, seed, steps, _ = g["params"]
     n, seed, steps = int(n), int(seed), int(steps)
     stride, b0 = int(g["stride"]), int(g["block_origin"])
     bw = g["block"].shape[0]
     psi = mcs.init_field((n, n, n), psi0, amp, seed, dtype=np.float64)
-    st = mcs.Stepper((n, n, n), [H] * 3, eps, dt, dtype=dtype)
+    st = mcs.Stepper((n, n, n), [H] * 3, eps, dt, dtype=dtype, form=form)
     st.set_psi(psi)
     del psi
     st.step(steps)
@@ -181,13 +183,14 @@ def test_precision_study_512cubed_500_steps_against_oracle_fixture(mcs, dtype):
     sumsq = float(np.sum(got * got))
     r_q = abs(sumsq - float(g["sumsq"])) / float(g["sumsq"])
     mean = float(got.mean())
-    print(f"512^3 {np.dtype(dtype).name} {steps} steps vs FP64 oracle: sampled rel-L2 {r_s:.3e} max-abs {m_s:.3e}, "
+    print(f"512^3 {np.dtype(dtype).name} form {form} {steps} steps vs FP64 oracle: sampled rel-L2 {r_s:.3e} max-abs {m_s:.3e}, "
           f"block rel-L2 {r_b:.3e} max-abs {m_b:.3e}, sum psi^2 {r_q:.2e} apart, max {got.max():.6f} "
           f"({float(g['vmax']):.6f}), min {got.min():.6f} ({float(g['vmin']):.6f})", file=sys.__stderr__, flush=True)
     if dtype == np.float32:
         # the gate is on the whole field (estimated from the 64^3 sample); the block is a local check
         # of one 32^3 region, where an interface moving through it makes the local error larger
-        assert r_s <= 1e-4 and m_s <= 5e-3 and r_b <= 5e-4 and m_b <= 5e-3
+        gate = 1e-4 if form == 0 else 2e-4
+        assert r_s <= gate and m_s <= 5e-3 and r_b <= 5e-4 and m_b <= 5e-3
         assert r_q < 1e-5 and abs(got.max() - float(g["vmax"])) < 5e-3 and abs(got.min() - float(g["vmin"])) < 5e-3
         # the mean is carried exactly; what remains is the rounding of the exported FP32 field
         assert abs(mean - float(g["total"]) / n ** 3) < 2e-8

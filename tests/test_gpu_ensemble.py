@@ -77,7 +77,8 @@ def test_graph_replays_between_eager_steps(mcs):
 def test_replicas_match_single_stepper(mcs):
     # the same plans (the default rule: steppers and ensembles tune separately, since the
     # ensemble's work-table kernels run the runtime plans) give the same fields bit for bit,
-    # with the stepper's fixed-plan kernels where its shapes have them
+    # with the stepper's fixed-plan kernels where its shapes have them; replicas keep the
+    # spectral form's state
     reps = small_2d()
     mcs.set_autotune(False)
     try:
@@ -86,7 +87,7 @@ def test_replicas_match_single_stepper(mcs):
         e.step(12, mcs.ENS_GRAPH)
         for r, p in enumerate(reps):
             st = mcs.Stepper(p["shape"], [H] * 3, p["eps"], p["dt"], temp=p["temp"], seed=p["noise_seed"],
-                             dtype=np.float32)
+                             dtype=np.float32, form=mcs.FORM_SPECTRAL)
             st.set_psi(mcs.init_field(p["shape"], p["psi0"], p["amp"], p["init_seed"], np.float32))
             st.step(12)
             assert np.array_equal(st.get_psi(), e.get_psi(r))

@@ -12,10 +12,14 @@ def rel_l2(a, b):
     return np.linalg.norm(a - b) / np.linalg.norm(b)
 
 
-def run_pair(mcs, orc, shp, eps, dt, psi0, amp, seed, steps, split=0, temp=0.0, noise_seed=0, dtype=np.float64):
+FORMS = [0, 1]  # FORM_SPECTRAL, FORM_MIXED
+
+
+def run_pair(mcs, orc, shp, eps, dt, psi0, amp, seed, steps, split=0, temp=0.0, noise_seed=0, dtype=np.float64,
+             form=None):
     psi = orc.init_field(shp, psi0, amp, seed)
     h3 = [H, H, H]
-    st = mcs.Stepper(shp, h3, eps, dt, temp=temp, seed=noise_seed, split=split, dtype=dtype)
+    st = mcs.Stepper(shp, h3, eps, dt, temp=temp, seed=noise_seed, split=split, dtype=dtype, form=form)
     st.set_psi(psi)
     st.step(steps)
     got = st.get_psi()
@@ -23,34 +27,91 @@ def run_pair(mcs, orc, shp, eps, dt, psi0, amp, seed, steps, split=0, temp=0.0, 
     return got, want, psi, st
 
 
+@pytest.mark.parametrize("form", FORMS)
 @pytest.mark.parametrize("split", [0, 1])
 @pytest.mark.parametrize("shp", [(32, 32, 32), (24, 20, 48), (64, 64), (48, 80)])
-def test_fp64_matches_oracle(mcs, orc, shp, split):
-    got, want, psi, st = run_pair(mcs, orc, shp, 0.3, 0.5, -0.25, 0.05, 11, 20, split=split)
+def test_fp64_matches_oracle(mcs, orc, shp, split, form):
+    got, want, psi, st = run_pair(mcs, orc, shp, 0.3, 0.5, -0.25, 0.05, 11, 20, split=split, form=form)
+    assert st.form() == form
     assert rel_l2(got, want) < 1e-12
     assert abs(got.mean() - psi.mean()) < 1e-14
 
 
+@pytest.mark.parametrize("form", FORMS)
 @pytest.mark.parametrize("shp", [(32, 32, 32), (64, 96)])
-def test_fp64_with_noise_matches_oracle(mcs, orc, shp):
-    got, want, psi, st = run_pair(mcs, orc, shp, 0.3, 0.25, -0.2, 0.1, 7, 15, temp=1e-3, noise_seed=7)
+def test_fp64_with_noise_matches_oracle(mcs, orc, shp, form):
+    got, want, psi, st = run_pair(mcs, orc, shp, 0.3, 0.25, -0.2, 0.1, 7, 15, temp=1e-3, noise_seed=7, form=form)
     assert rel_l2(got, want) < 1e-11
     assert abs(got.mean() - psi.mean()) < 1e-14
 
 
+@pytest.mark.parametrize("form", FORMS)
 @pytest.mark.parametrize("shp", [(32, 32, 32), (128, 128)])
-def test_fp32_matches_oracle(mcs, orc, shp):
-    got, want, psi, st = run_pair(mcs, orc, shp, 0.3, 0.5, -0.25, 0.05, 11, 50, dtype=np.float32)
+def test_fp32_matches_oracle(mcs, orc, shp, form):
+    got, want, psi, st = run_pair(mcs, orc, shp, 0.3, 0.5, -0.25, 0.05, 11, 50, dtype=np.float32, form=form)
     assert rel_l2(got, want) < 1e-5
     assert np.abs(got - want).max() < 1e-5
     assert abs(st.mass() - psi.astype(np.float32).astype(np.float64).mean()) < 1e-15
     assert abs(got.astype(np.float64).mean() - st.mass()) < 2e-8
 
 
-def test_fp32_with_noise_matches_oracle(mcs, orc):
+@pytest.mark.parametrize("form", FORMS)
+def test_fp32_with_noise_matches_oracle(mcs, orc, form):
     got, want, psi, st = run_pair(mcs, orc, (32, 32, 32), 0.3, 0.25, -0.2, 0.1, 7, 20, temp=1e-3, noise_seed=7,
-                                  dtype=np.float32)
+                                  dtype=np.float32, form=form)
     assert rel_l2(got, want) < 5e-5
+
+
+def test_default_form_is_spectral(mcs):
+    assert mcs.get_step_form() == mcs.FORM_SPECTRAL
+    st = mcs.Stepper((32, 32, 32), [H] * 3, 0.3, 0.5)
+    assert st.form() == mcs.FORM_SPECTRAL
+    mcs.set_step_form(mcs.FORM_MIXED)
+    try:
+        assert mcs.Stepper((32, 32, 32), [H] * 3, 0.3, 0.5).form() == mcs.FORM_MIXED
+        with pytest.raises(mcs.McsError):
+            mcs.set_step_form(2)
+    finally:
+        mcs.set_step_form(mcs.FORM_SPECTRAL)
+    with pytest.raises(mcs.McsError):
+        st.set_form(2)
+
+
+@pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-13), (np.float32, 1e-5)])
+@pytest.mark.parametrize("shp", [(32, 32, 32), (64, 96), (24, 40, 60)])
+def test_forms_agree_convert_and_restore(mcs, orc, shp, dtype, tol):
+    # the two forms run the same update on the same modes: they differ by rounding only; a
+    # stepper converted mid-run (both ways) continues the same run; the mixed state is the
+    # inverse x transform of the spectral state over nx/2 (numpy's normalised ifft), and a
+    # restored mixed state continues bit for bit
+    psi = orc.init_field(shp, -0.2, 0.1, 7)
+    kw = dict(temp=1e-3, seed=7, dtype=dtype)
+    a = mcs.Stepper(shp, [H] * 3, 0.3, 0.25, form=mcs.FORM_SPECTRAL, **kw)
+    b = mcs.Stepper(shp, [H] * 3, 0.3, 0.25, form=mcs.FORM_MIXED, **kw)
+    c = mcs.Stepper(shp, [H] * 3, 0.3, 0.25, form=mcs.FORM_SPECTRAL, **kw)
+    for st in (a, b, c):
+        st.set_psi(psi)
+    a.step(12)
+    b.step(12)
+    c.step(5)
+    c.set_form(mcs.FORM_MIXED)
+    assert c.form() == mcs.FORM_MIXED
+    c.step(4)
+    c.set_form(mcs.FORM_SPECTRAL)
+    c.step(3)
+    pa = a.get_psi()
+    assert rel_l2(b.get_psi(), pa) < tol
+    assert rel_l2(c.get_psi(), pa) < tol
+    sa, ma, ka = a.get_state()
+    sb, mb, kb = b.get_state()
+    assert ma == mb and ka == kb == 12
+    want = np.fft.ifft(sa.astype(np.complex128), axis=-1)
+    assert np.linalg.norm(sb - want) / np.linalg.norm(want) < tol * 10
+    d = mcs.Stepper(shp, [H] * 3, 0.3, 0.25, form=mcs.FORM_MIXED, **kw)
+    d.set_state(sb, mb, kb)
+    d.step(3)
+    b.step(3)
+    assert np.array_equal(d.get_state()[0].view(np.uint8), b.get_state()[0].view(np.uint8))
 
 
 def test_config1_2d_256_fp64_1000_steps(mcs, orc):
