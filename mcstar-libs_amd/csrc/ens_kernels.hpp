@@ -19,49 +19,63 @@ template <typename F, int P, int SPLIT, bool WAVE, typename FP = RunPlan, bool N
 __global__ void __launch_bounds__(MCS_XPASS_MAX_THREADS) k_ens_xupd(const EnsRep<F>* __restrict__ reps, int nr, int units,
                                                   const int2* __restrict__ pairs, int npairs, int G, FftPlanDev pl,
                                                   const C2<F>* __restrict__ tw, const C2<F>* __restrict__ wph,
-                                                  const int64_t* __restrict__ d_step, int draw_off) {
+                                                  const int64_t* __restrict__ d_step, int draw_off, int twn) {
     extern __shared__ unsigned char smem_raw[];
     C2<F>* lds = reinterpret_cast<C2<F>*>(smem_raw);
     const uint32_t draw = NOISE ? (uint32_t)(*d_step + draw_off) : 0u;
     const long total = (long)nr * units;
+    // the twiddle table behind the exchange area (twn entries; 0: read from memory), copied once
+    // per workgroup
+    const int L = FP::fixed ? FP::L : pl.L;
+    const C2<F>* twl = stage_twiddles<F>(tw, twn, lds + G * 2 * L);
     for (long it = blockIdx.x; it < total; it += gridDim.x) {
         const int r = (int)(it / units), u = (int)(it % units);
         const EnsRep<F>& R = reps[r];
-        xupd_body<F, P, SPLIT, WAVE, FP, NOISE>(R.W, R.U, pairs, npairs, G, R.xa, pl, tw, wph, draw, lds, u,
+        xupd_body<F, P, SPLIT, WAVE, FP, NOISE>(R.W, R.U, pairs, npairs, G, R.xa, pl, twl, wph, draw, lds, u,
                                                 opaque_tid());
         __syncthreads();
     }
 }
 
+// Column work units are dealt in the tile order swz of xcd_tile over the grid of nr * nouter rows
+// of tiles: the launch grid is a multiple of 8 (or the unit count), so unit `it` runs on XCD it % 8
+// as a workgroup index would.
 template <typename F, int P, int DIR, typename FP = RunPlan>
 __global__ void __launch_bounds__(MCS_REG_MAX_THREADS(P, F)) k_ens_col(const EnsRep<F>* __restrict__ reps, int nr, ColGeom g, int G,
-                                                  FftPlanDev pl, const C2<F>* __restrict__ tw) {
+                                                  FftPlanDev pl, const C2<F>* __restrict__ tw, int swz, int twn) {
     extern __shared__ unsigned char smem_raw[];
     C2<F>* lds = reinterpret_cast<C2<F>*>(smem_raw);
     const int tilesx = (g.nxc + G - 1) / G;
     const long units = (long)tilesx * g.nouter;
     const long total = (long)nr * units;
-    for (long it = blockIdx.x; it < total; it += gridDim.x) {
+    const int L = FP::fixed ? FP::L : pl.L;
+    const C2<F>* twl = stage_twiddles<F>(tw, twn, lds + G * L);
+    for (long it0 = blockIdx.x; it0 < total; it0 += gridDim.x) {
+        const long it = xcd_tile((unsigned)it0, (unsigned)total, swz, (unsigned)tilesx, (unsigned)(nr * g.nouter));
         const int r = (int)(it / units);
         const long u = it % units;
         C2<F>* W = reps[r].W;
-        col_body<F, P, DIR, FP>(W, W, g, G, pl, tw, lds, (int)(u % tilesx), (int)(u / tilesx), opaque_tid());
+        col_body<F, P, DIR, FP>(W, W, g, G, pl, twl, lds, (int)(u % tilesx), (int)(u / tilesx), opaque_tid());
         __syncthreads();
     }
 }
 
 template <typename F, int P, int SPLIT, typename FP = RunPlan>
 __global__ void __launch_bounds__(MCS_REG_MAX_THREADS(P, F)) k_ens_col_nl(const EnsRep<F>* __restrict__ reps, int nr, ColGeom g, int G,
-                                                     FftPlanDev pl, const C2<F>* __restrict__ tw, F scale) {
+                                                     FftPlanDev pl, const C2<F>* __restrict__ tw, F scale, int swz,
+                                                     int twn) {
     extern __shared__ unsigned char smem_raw[];
     C2<F>* lds = reinterpret_cast<C2<F>*>(smem_raw);
     const int tilesx = (g.nxc + G - 1) / G;
     const long units = (long)tilesx * g.nouter;
     const long total = (long)nr * units;
-    for (long it = blockIdx.x; it < total; it += gridDim.x) {
+    const int L = FP::fixed ? FP::L : pl.L;
+    const C2<F>* twl = stage_twiddles<F>(tw, twn, lds + G * L);
+    for (long it0 = blockIdx.x; it0 < total; it0 += gridDim.x) {
+        const long it = xcd_tile((unsigned)it0, (unsigned)total, swz, (unsigned)tilesx, (unsigned)(nr * g.nouter));
         const int r = (int)(it / units);
         const long u = it % units;
-        col_nl_body<F, P, SPLIT, FP>(reps[r].W, reps[r].W, g, G, pl, tw, scale, lds, (int)(u % tilesx), (int)(u / tilesx),
+        col_nl_body<F, P, SPLIT, FP>(reps[r].W, reps[r].W, g, G, pl, twl, scale, lds, (int)(u % tilesx), (int)(u / tilesx),
                                      opaque_tid());
         __syncthreads();
     }

@@ -119,12 +119,28 @@ static int upload_reps(mcs_ensemble* e) {
     return MCS_OK;
 }
 
-// resident workgroups per CU from the thread (2048 per CU) and LDS (160 KB) budgets, times 2 for slack
+// resident workgroups per CU from the thread (2048 per CU) and LDS (160 KB) budgets, times 2 for slack;
+// a multiple of 8 below the unit count (so that a unit's XCD follows from its index)
 static unsigned persistent_grid(mcs_ensemble* e, int threads, size_t lds, long total) {
     long per = 2048 / std::max(threads, 64);
     if (lds > 0) per = std::min<long>(per, (long)(MCS_LDS_MAX / lds));
     per = std::max<long>(per, 1) * 2;
-    return (unsigned)std::max<long>(1, std::min<long>(total, per * e->cus));
+    long grid = per * e->cus;
+    if (grid < total) grid = std::max<long>(8, grid & ~7L);
+    return (unsigned)std::max<long>(1, std::min<long>(total, grid));
+}
+
+static int ens_lds_attr(const void* fn, size_t bytes) {
+    if (bytes > 65536) MCS_HIP_TRY(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+    return MCS_OK;
+}
+
+// the tile order of a work-table column pass: where neighbouring tiles share cache lines (tiles
+// narrower than a 128-byte line, or rows not aligned to one), the XCD-aware order of
+// col_tile_order keeps a shared line in one L2; elsewhere plain order
+static int ens_tile_order(const ColGeom& g, int G, long esz) {
+    const bool shared = (G * esz) % 128 != 0 || (g.nxc * esz) % 128 != 0 || (g.os * esz) % 128 != 0;
+    return shared ? col_tile_order(g, G, esz) : 0;
 }
 
 // one x pass over members [first, first + nr) of group g
@@ -133,8 +149,10 @@ static int ens_xupd_t(mcs_ensemble* e, EnsGroup* g, int first, int nr, int draw_
     const LinePlan& lp = g->sg.px;
     const int G = xupd_pairs_per_wg(lp);
     const int S = lp.L / P;
-    const size_t lds = (size_t)G * 2 * lp.L * sizeof(C2<F>);
+    const size_t exch = (size_t)G * 2 * lp.L * sizeof(C2<F>);
     const int units = (g->sg.npairs + G - 1) / G;
+    const int twn = twiddles_in_lds(lp, exch, (long)nr * units);
+    const size_t lds = exch + (size_t)twn * sizeof(C2<F>);  // exchange area, twiddles
     bool noise = false;  // the noise-free kernels carry no noise draw
     for (int i = first; i < first + nr; ++i) noise = noise || e->reps[g->members[i]].temp > 0;
     auto fn = noise ? ((64 % S == 0) ? k_ens_xupd<F, P, SPLIT, true, RunPlan, true>
@@ -142,9 +160,11 @@ static int ens_xupd_t(mcs_ensemble* e, EnsGroup* g, int first, int nr, int draw_
                     : ((64 % S == 0) ? k_ens_xupd<F, P, SPLIT, true, RunPlan, false>
                                      : k_ens_xupd<F, P, SPLIT, false, RunPlan, false>);
     const unsigned grid = persistent_grid(e, G * S, lds, (long)nr * units);
+    int st = ens_lds_attr((const void*)fn, lds);
+    if (st) return st;
     hipLaunchKernelGGL(fn, dim3(grid), dim3(G * S), lds, s, (const EnsRep<F>*)g->d_reps + first, nr, units,
                        g->sg.pairs, g->sg.npairs, G, lp.dev, (const C2<F>*)lp.tw, (const C2<F>*)g->sg.wph,
-                       (const int64_t*)e->d_step, draw_off);
+                       (const int64_t*)e->d_step, draw_off, twn);
     MCS_HIP_TRY(hipGetLastError());
     return MCS_OK;
 }
@@ -154,8 +174,10 @@ static int ens_col_t(mcs_ensemble* e, EnsGroup* g, const LinePlan& lp, const Col
                      hipStream_t s) {
     const int G = col_lines_per_wg(lp);
     const int S = lp.L / P;
-    const size_t lds = (size_t)G * lp.L * sizeof(C2<F>);
+    const size_t exch = (size_t)G * lp.L * sizeof(C2<F>);
     const long total = (long)nr * ((cg.nxc + G - 1) / G) * cg.nouter;
+    int twn = twiddles_in_lds(lp, exch, total);
+    const size_t lds = exch + (size_t)twn * sizeof(C2<F>);  // exchange area, twiddles
     const void* fx = fixed_ens_col_kernel(lp, DIR, 0);
     const void* fn = fx ? fx : (const void*)k_ens_col<F, P, DIR>;
     const unsigned grid = persistent_grid(e, G * S, lds, total);
@@ -164,8 +186,15 @@ static int ens_col_t(mcs_ensemble* e, EnsGroup* g, const LinePlan& lp, const Col
     const C2<F>* tw = (const C2<F>*)lp.tw;
     ColGeom geo = cg;
     int gg = G, n = nr;
-    void* args[] = {&reps, &n, &geo, &gg, &pl, &tw};
-    MCS_HIP_TRY(hipLaunchKernel(fn, dim3(grid), dim3(G * S), args, lds, s));
+    int swz = ens_tile_order(cg, G, (long)sizeof(C2<F>));
+    MCS_CHECK(total < (1L << 32), MCS_ERR_UNSUPPORTED, "too many column units in one launch");
+    int st = ens_lds_attr(fn, lds);
+    if (st) return st;
+    // the fixed-plan kernels have the runtime-plan kernel's signature: launched through its type,
+    // so that the arguments are checked against the parameters when this compiles
+    const auto kfn = reinterpret_cast<decltype(&k_ens_col<F, P, DIR>)>(const_cast<void*>(fn));
+    hipLaunchKernelGGL(kfn, dim3(grid), dim3(G * S), lds, s, reps, n, geo, gg, pl, tw, swz, twn);
+    MCS_HIP_TRY(hipGetLastError());
     return MCS_OK;
 }
 
@@ -174,8 +203,10 @@ static int ens_col_nl_t(mcs_ensemble* e, EnsGroup* g, const LinePlan& lp, const 
                         hipStream_t s) {
     const int G = col_lines_per_wg(lp);
     const int S = lp.L / P;
-    const size_t lds = (size_t)G * lp.L * sizeof(C2<F>);
+    const size_t exch = (size_t)G * lp.L * sizeof(C2<F>);
     const long total = (long)nr * ((cg.nxc + G - 1) / G) * cg.nouter;
+    int twn = twiddles_in_lds(lp, exch, total);
+    const size_t lds = exch + (size_t)twn * sizeof(C2<F>);  // exchange area, twiddles
     const void* fx = fixed_ens_col_kernel(lp, 0, SPLIT);
     const void* fn = fx ? fx : (const void*)k_ens_col_nl<F, P, SPLIT>;
     const unsigned grid = persistent_grid(e, G * S, lds, total);
@@ -185,8 +216,13 @@ static int ens_col_nl_t(mcs_ensemble* e, EnsGroup* g, const LinePlan& lp, const 
     ColGeom geo = cg;
     int gg = G, n = nr;
     F scale = (F)(2.0 / ((double)g->sg.nx * g->sg.ny * g->sg.nz));
-    void* args[] = {&reps, &n, &geo, &gg, &pl, &tw, &scale};
-    MCS_HIP_TRY(hipLaunchKernel(fn, dim3(grid), dim3(G * S), args, lds, s));
+    int swz = ens_tile_order(cg, G, (long)sizeof(C2<F>));
+    MCS_CHECK(total < (1L << 32), MCS_ERR_UNSUPPORTED, "too many column units in one launch");
+    int st = ens_lds_attr(fn, lds);
+    if (st) return st;
+    const auto kfn = reinterpret_cast<decltype(&k_ens_col_nl<F, P, SPLIT>)>(const_cast<void*>(fn));
+    hipLaunchKernelGGL(kfn, dim3(grid), dim3(G * S), lds, s, reps, n, geo, gg, pl, tw, scale, swz, twn);
+    MCS_HIP_TRY(hipGetLastError());
     return MCS_OK;
 }
 
@@ -334,22 +370,22 @@ static int cls_launch_t(mcs_ensemble* e, const EnsClass& c, int draw_off, hipStr
     const bool ref = e->split == MCS_SPLIT_REFERENCE;
     long total = c.total;
     if (c.kind == CLS_X) {
-        fn = c.noise ? (const void*)k_cls_xupd<F, TIER, true> : (const void*)k_cls_xupd<F, TIER, false>;
+        const auto kfn = c.noise ? k_cls_xupd<F, TIER, true> : k_cls_xupd<F, TIER, false>;
+        fn = (const void*)kfn;
         int st = cls_lds_attr(fn, c.lds);
         if (st) return st;
-        const int64_t* ds = e->d_step;
-        void* args[] = {&d, &total, &ds, &draw_off};
-        MCS_HIP_TRY(hipLaunchKernel(fn, dim3(grid), dim3(c.T), args, c.lds, s));
+        hipLaunchKernelGGL(kfn, dim3(grid), dim3(c.T), c.lds, s, d, total, (const int64_t*)e->d_step, draw_off);
+        MCS_HIP_TRY(hipGetLastError());
         return MCS_OK;
     }
-    if (c.kind == CLS_COL_NL)
-        fn = ref ? (const void*)k_cls_col_nl<F, TIER, MCS_SPLIT_REFERENCE> : (const void*)k_cls_col_nl<F, TIER, MCS_SPLIT_NORTHSTAR>;
-    else
-        fn = c.kind == CLS_COL_INV ? (const void*)k_cls_col<F, TIER, +1> : (const void*)k_cls_col<F, TIER, -1>;
+    const auto kfn = c.kind == CLS_COL_NL ? (ref ? k_cls_col_nl<F, TIER, MCS_SPLIT_REFERENCE>
+                                                 : k_cls_col_nl<F, TIER, MCS_SPLIT_NORTHSTAR>)
+                                          : (c.kind == CLS_COL_INV ? k_cls_col<F, TIER, +1> : k_cls_col<F, TIER, -1>);
+    fn = (const void*)kfn;
     int st = cls_lds_attr(fn, c.lds);
     if (st) return st;
-    void* args[] = {&d, &total};
-    MCS_HIP_TRY(hipLaunchKernel(fn, dim3(grid), dim3(c.T), args, c.lds, s));
+    hipLaunchKernelGGL(kfn, dim3(grid), dim3(c.T), c.lds, s, d, total);
+    MCS_HIP_TRY(hipGetLastError());
     return MCS_OK;
 }
 

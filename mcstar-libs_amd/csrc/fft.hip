@@ -18,7 +18,7 @@
 // whole tile before it writes).
 template <typename F, int P, int DIR>
 __global__ void __launch_bounds__(MCS_REG_MAX_THREADS(P, F)) k_col_reg(const C2<F>* src, C2<F>* dst, ColGeom g, int G, FftPlanDev pl,
-                                                  const C2<F>* __restrict__ tw, int swz) {
+                                                  const C2<F>* __restrict__ tw, int swz, int twn) {
     extern __shared__ unsigned char smem_raw[];
     C2<F>* lds = reinterpret_cast<C2<F>*>(smem_raw);
     const int S = pl.L / P;
@@ -32,7 +32,8 @@ __global__ void __launch_bounds__(MCS_REG_MAX_THREADS(P, F)) k_col_reg(const C2<
     C2<F> a[P];
 #pragma unroll
     for (int m = 0; m < P; ++m) a[m] = valid ? src[base + (long)(t + m * S) * g.ls] : mk<F>(0, 0);
-    reg_fft<F, P, DIR>(a, pl, tw, t, lds, c, G);
+    const C2<F>* twl = stage_twiddles<F>(tw, twn, lds + G * pl.L);  // behind the exchange area
+    reg_fft<F, P, DIR>(a, pl, twl, t, lds, c, G);
     if (valid) {
 #pragma unroll
         for (int m = 0; m < P; ++m) dst[base + (long)(t + m * S) * g.ls] = a[m];
@@ -181,14 +182,15 @@ static int launch_col_reg_t(const LinePlan& lp, const void* src, void* dst, cons
     const int S = lp.L / P;
     const int esz = (int)sizeof(C2<F>);
     const int G = col_lines_per_wg(lp);
-    const size_t lds = (size_t)G * lp.L * esz;
+    const long tiles = (long)((g.nxc + G - 1) / G) * g.nouter;
+    const int twn = twiddles_in_lds(lp, (size_t)G * lp.L * esz, tiles);
+    const size_t lds = ((size_t)G * lp.L + twn) * esz;  // exchange area, twiddles
     auto fn = k_col_reg<F, P, DIR>;
     int st = set_lds((const void*)fn, lds);
     if (st) return st;
-    const long tiles = (long)((g.nxc + G - 1) / G) * g.nouter;
     const int swz = col_tile_order(g, G, esz);
     hipLaunchKernelGGL(fn, dim3((unsigned)tiles), dim3(G * S), lds, s, (const C2<F>*)src, (C2<F>*)dst, g, G, lp.dev,
-                       (const C2<F>*)lp.tw, swz);
+                       (const C2<F>*)lp.tw, swz, twn);
     MCS_HIP_TRY(hipGetLastError());
     return MCS_OK;
 }

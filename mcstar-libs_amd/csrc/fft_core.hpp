@@ -282,6 +282,26 @@ MCS_DEV void reg_fft(C2<F> (&a)[P], const FftPlanDev& pl, const C2<F>* __restric
     }
 }
 
+// entries of a runtime plan's twiddle table (passes with Ns > 1 hold Ns (R - 1) each)
+MCS_DEV int plan_tw_count(const FftPlanDev& pl) {
+    int n = 0;
+    for (int p = 0; p < pl.np; ++p)
+        if (pl.ns[p] > 1) n = max(n, pl.twoff[p] + pl.ns[p] * (pl.radix[p] - 1));
+    return n;
+}
+
+// Copies a twiddle table of n entries into LDS at dst with every thread of the workgroup and
+// returns dst (ends with a barrier); n = 0 (the launcher found no room for the table in LDS)
+// returns tw itself.  Each pass of a transform reads its twiddles in the middle of the
+// transform; from global memory that is an L2 round trip per pass, a chain that short launches
+// (few workgroups, nothing to hide it behind) wait on in full.
+template <typename F> MCS_DEV const C2<F>* stage_twiddles(const C2<F>* __restrict__ tw, int n, C2<F>* dst) {
+    if (n == 0) return tw;
+    for (int i = (int)threadIdx.x; i < n; i += (int)blockDim.x) dst[i] = tw[i];
+    __syncthreads();
+    return dst;
+}
+
 // ------------------------------------------------------------ lane exchanges
 // A swap stage of a register/lane transpose within a wavefront: the lanes whose bit D is clear
 // take the lo value of their partner lane l ^ D into hi; the lanes whose bit D is set take the
@@ -445,6 +465,12 @@ template <int Ns, int R, int... Rest> constexpr int fix_tw_count_from() {
     else return (Ns > 1 ? Ns * (R - 1) : 0) + fix_tw_count_from<Ns * R, Rest...>();
 }
 template <int L, int... Rs> constexpr int fix_tw_count(FixPlan<L, Rs...>) { return fix_tw_count_from<1, Rs...>(); }
+
+// twiddle entries of the runtime plan pl or of the fixed plan FP
+template <typename FP> MCS_DEV int line_tw_count(const FftPlanDev& pl) {
+    if constexpr (FP::fixed) return fix_tw_count(FP{});
+    else return plan_tw_count(pl);
+}
 
 // the line transform of a pass body: runtime plan pl, or the fixed plan FP (pl unused); SHUF:
 // a fixed plan's last exchange through lane shuffles where the plan allows it

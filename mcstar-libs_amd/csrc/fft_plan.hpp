@@ -56,3 +56,16 @@ int make_phase_table(void** dev, int n, int count, int prec);
 
 // the library's maximum dynamic LDS per workgroup
 #define MCS_LDS_MAX (160 * 1024)
+
+// twiddle entries a launch of nblocks workgroups with `exchange` bytes of exchange LDS each stages
+// into LDS behind that area: the plan's whole table where it fits the maximum and either adds at
+// most a quarter to the workgroup's LDS (hardly fewer workgroups fit a CU) or the launch is short
+// (up to two workgroups per CU, where the chain of twiddle loads, not occupancy, sets the time);
+// else none (the kernel reads them from memory).  A 4096-point line with its 32 KB table behind
+// 64 KB of exchange area ran 30% slower staged.
+inline int twiddles_in_lds(const LinePlan& lp, size_t exchange, long nblocks) {
+    const size_t esz = lp.prec == 64 ? 16 : 8;
+    const size_t tw = lp.twcount * esz;
+    if (exchange + tw > (size_t)MCS_LDS_MAX) return 0;
+    return (4 * tw <= exchange || nblocks <= 512) ? (int)lp.twcount : 0;
+}

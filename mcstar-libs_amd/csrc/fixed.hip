@@ -110,12 +110,13 @@ __global__ void __launch_bounds__(MCS_REG_MAX_THREADS(P, F)) k_col_fx(const C2<F
 template <typename F, int P, int SPLIT, typename FP>
 __global__ void __launch_bounds__(MCS_REG_MAX_THREADS(P, F)) k_col_nl_fx(const C2<F>* src, C2<F>* W, ColGeom g,
                                                                         int G, const C2<F>* __restrict__ tw, F scale,
-                                                                        int64_t* d_step) {
+                                                                        int64_t* d_step, int swz) {
     extern __shared__ unsigned char smem_raw[];
     C2<F>* lds = reinterpret_cast<C2<F>*>(smem_raw);
     constexpr int S = FP::L / P;
     const int tilesx = (g.nxc + G - 1) / G;
-    const int tile = blockIdx.x % tilesx, outer = blockIdx.x / tilesx;
+    const int blk = (int)xcd_tile(blockIdx.x, gridDim.x, swz, tilesx, g.nouter);
+    const int tile = blk % tilesx, outer = blk / tilesx;
     const int c = threadIdx.x % G, t = threadIdx.x / G;
     const bool valid = tile * G + c < g.nxc;
     const char* sb = reinterpret_cast<const char*>(src + (long)outer * g.os + (long)tile * G);
@@ -154,6 +155,18 @@ __global__ void __launch_bounds__(MCS_XPASS_MAX_THREADS) k_xupd_fx(C2<F>* __rest
     xupd_body<F, P, SPLIT, WAVE, FP, NOISE, SHUF>(W, U, pairs, npairs, G, ar, unused, tw, wph, draw,
                                             reinterpret_cast<C2<F>*>(smem_raw), blockIdx.x, threadIdx.x);
 }
+
+// Signatures of the kernel families the tables hold (every instance of a family has its
+// family's parameters): launches go through these types, so that the arguments are checked
+// against the parameters when the library compiles.
+template <typename F> using ColFxFn = void (*)(const C2<F>*, C2<F>*, ColGeom, int, const C2<F>*, int);
+template <typename F> using ColNlFxFn = void (*)(const C2<F>*, C2<F>*, ColGeom, int, const C2<F>*, F, int64_t*, int);
+template <typename F>
+using XupdFxFn = void (*)(C2<F>*, C2<F>*, const int2*, int, int, XArgs<F>, const C2<F>*, const C2<F>*,
+                          const int64_t*);
+template <typename F>
+using RowFxFn = void (*)(const C2<F>*, C2<F>*, long, int, FftPlanDev, const C2<F>*, const C2<F>*);
+template <typename T> static T typed(const void* fn) { return reinterpret_cast<T>(const_cast<void*>(fn)); }
 
 // ------------------------------------------------------------------ table
 // Each entry: precision, points per thread, length, radices in pass order (the
@@ -285,9 +298,13 @@ int fixed_col(const LinePlan& lp, const void* src, void* dst, const ColGeom& g, 
     int st = set_lds_fx(fn, lds);
     if (st) return st;
     const long tiles = (long)((g.nxc + G - 1) / G) * g.nouter;
-    const void* tw = lp.tw;
-    void* args[] = {(void*)&src, &dst, (void*)&g, (void*)&G, (void*)&tw, (void*)&swz};
-    MCS_HIP_TRY(hipLaunchKernel(fn, dim3((unsigned)tiles), dim3(G * S), args, lds, s));
+    if (lp.prec == 64)
+        hipLaunchKernelGGL(typed<ColFxFn<double>>(fn), dim3((unsigned)tiles), dim3(G * S), lds, s,
+                           (const C2<double>*)src, (C2<double>*)dst, g, G, (const C2<double>*)lp.tw, swz);
+    else
+        hipLaunchKernelGGL(typed<ColFxFn<float>>(fn), dim3((unsigned)tiles), dim3(G * S), lds, s,
+                           (const C2<float>*)src, (C2<float>*)dst, g, G, (const C2<float>*)lp.tw, swz);
+    MCS_HIP_TRY(hipGetLastError());
     *done = true;
     return MCS_OK;
 }
@@ -306,16 +323,15 @@ int fixed_step_col(const LinePlan& lp, const void* src, void* W, const ColGeom& 
     int st = set_lds_fx(fn, lds);
     if (st) return st;
     const long tiles = (long)((g.nxc + G - 1) / G) * g.nouter;
-    void* tw = lp.tw;
-    if (lp.prec == 64) {
-        double sc = scale;
-        void* args[] = {(void*)&src, &W, (void*)&g, (void*)&G, &tw, &sc, &d_step};
-        MCS_HIP_TRY(hipLaunchKernel(fn, dim3((unsigned)tiles), dim3(G * S), args, lds, s));
-    } else {
-        float sc = (float)scale;
-        void* args[] = {(void*)&src, &W, (void*)&g, (void*)&G, &tw, &sc, &d_step};
-        MCS_HIP_TRY(hipLaunchKernel(fn, dim3((unsigned)tiles), dim3(G * S), args, lds, s));
-    }
+    const int swz = col_tile_order(g, G, esz);
+    if (lp.prec == 64)
+        hipLaunchKernelGGL(typed<ColNlFxFn<double>>(fn), dim3((unsigned)tiles), dim3(G * S), lds, s,
+                           (const C2<double>*)src, (C2<double>*)W, g, G, (const C2<double>*)lp.tw, scale, d_step, swz);
+    else
+        hipLaunchKernelGGL(typed<ColNlFxFn<float>>(fn), dim3((unsigned)tiles), dim3(G * S), lds, s,
+                           (const C2<float>*)src, (C2<float>*)W, g, G, (const C2<float>*)lp.tw, (float)scale, d_step,
+                           swz);
+    MCS_HIP_TRY(hipGetLastError());
     *done = true;
     return MCS_OK;
 }
@@ -332,13 +348,10 @@ static int fixed_xupd_t(const XEntry& e, const StepGeom& sg, const StepPhys& ph,
     const void* fn = e.fn[sh ? 1 : 0][ph.split == MCS_SPLIT_REFERENCE ? 1 : 0][ar.noise ? 1 : 0];
     int st = set_lds_fx(fn, lds);
     if (st) return st;
-    const int2* pairs = sg.pairs;
-    int npairs = sg.npairs, g = G;
-    const void* tw = lp.tw;
-    const void* wph = sg.wph;
-    const int64_t* ds = sg.d_step;
-    void* args[] = {&W, &U, (void*)&pairs, &npairs, &g, &ar, (void*)&tw, (void*)&wph, (void*)&ds};
-    MCS_HIP_TRY(hipLaunchKernel(fn, dim3((unsigned)((sg.npairs + G - 1) / G)), dim3(G * S), args, lds, s));
+    hipLaunchKernelGGL(typed<XupdFxFn<F>>(fn), dim3((unsigned)((sg.npairs + G - 1) / G)), dim3(G * S), lds, s,
+                       (C2<F>*)W, (C2<F>*)U, sg.pairs, sg.npairs, G, ar, (const C2<F>*)lp.tw, (const C2<F>*)sg.wph,
+                       (const int64_t*)sg.d_step);
+    MCS_HIP_TRY(hipGetLastError());
     return MCS_OK;
 }
 
@@ -461,11 +474,14 @@ int fixed_row(int mode, const LinePlan& lp, const void* wph, const void* src, vo
     const void* fn = mode == 0 ? e->r2c : e->c2r;
     int st = set_lds_fx(fn, lds);
     if (st) return st;
-    FftPlanDev pl = lp.dev;
-    const void* tw = lp.tw;
-    int g = G;
-    void* args[] = {(void*)&src, &dst, &nrows, &g, &pl, (void*)&tw, (void*)&wph};
-    MCS_HIP_TRY(hipLaunchKernel(fn, dim3((unsigned)((nrows + G - 1) / G)), dim3(G * (lp.L / lp.P)), args, lds, s));
+    const dim3 grid((unsigned)((nrows + G - 1) / G)), block(G * (lp.L / lp.P));
+    if (lp.prec == 64)
+        hipLaunchKernelGGL(typed<RowFxFn<double>>(fn), grid, block, lds, s, (const C2<double>*)src, (C2<double>*)dst,
+                           nrows, G, lp.dev, (const C2<double>*)lp.tw, (const C2<double>*)wph);
+    else
+        hipLaunchKernelGGL(typed<RowFxFn<float>>(fn), grid, block, lds, s, (const C2<float>*)src, (C2<float>*)dst,
+                           nrows, G, lp.dev, (const C2<float>*)lp.tw, (const C2<float>*)wph);
+    MCS_HIP_TRY(hipGetLastError());
     *done = true;
     return MCS_OK;
 }

@@ -12,11 +12,13 @@ template <typename F, int P, int SPLIT, bool WAVE, bool NOISE>
 __global__ void __launch_bounds__(MCS_XPASS_MAX_THREADS)
     k_xupd_mx(const C2<F>* __restrict__ W, C2<F>* __restrict__ Sx, const int2* __restrict__ pairs, int npairs, int G,
               XArgs<F> ar, FftPlanDev pl, const C2<F>* __restrict__ tw, const C2<F>* __restrict__ wph,
-              const int64_t* __restrict__ d_step, F invL) {
+              const int64_t* __restrict__ d_step, F invL, int twn) {
     extern __shared__ unsigned char smem_raw[];
+    C2<F>* lds = reinterpret_cast<C2<F>*>(smem_raw);
     const uint32_t draw = NOISE && ar.noise ? (uint32_t)*d_step : 0u;
-    xupd_mixed_body<F, P, SPLIT, WAVE, RunPlan, NOISE>(W, Sx, pairs, npairs, G, ar, pl, tw, wph, draw, invL,
-                                                       reinterpret_cast<C2<F>*>(smem_raw), blockIdx.x, threadIdx.x);
+    const C2<F>* twl = stage_twiddles<F>(tw, twn, lds + G * 3 * pl.L);  // behind the exchange area
+    xupd_mixed_body<F, P, SPLIT, WAVE, RunPlan, NOISE>(W, Sx, pairs, npairs, G, ar, pl, twl, wph, draw, invL, lds,
+                                                       blockIdx.x, threadIdx.x);
 }
 
 // SHUF: the last exchange of every line transform through lane shuffles (plans with fix_shuffle_ok)
@@ -129,15 +131,14 @@ static int launch_mx_fx(const MxEntry& e, const StepGeom& sg, const XArgs<F>& ar
     const void* fn = e.fn[use_shuffle(e, lp) ? 1 : 0][split == MCS_SPLIT_REFERENCE ? 1 : 0][ar.noise ? 1 : 0];
     int st = set_lds_mx(fn, lds);
     if (st) return st;
-    const int2* pairs = sg.pairs;
-    int npairs = sg.npairs, g = G;
-    XArgs<F> a = ar;
-    const void* tw = lp.tw;
-    const void* wph = sg.wph;
-    const int64_t* ds = sg.d_step;
-    F invL = (F)(1.0 / lp.L);
-    void* args[] = {(void*)&W, &Sx, (void*)&pairs, &npairs, &g, &a, (void*)&tw, (void*)&wph, (void*)&ds, &invL};
-    MCS_HIP_TRY(hipLaunchKernel(fn, dim3((unsigned)((sg.npairs + G - 1) / G)), dim3(G * S), args, lds, s));
+    // every fixed-plan kernel of the table has this signature: launched through it, so that the
+    // arguments are checked against the parameters when the library compiles
+    using MxFn = void (*)(const C2<F>*, C2<F>*, const int2*, int, int, XArgs<F>, const C2<F>*, const C2<F>*,
+                          const int64_t*, F);
+    hipLaunchKernelGGL(reinterpret_cast<MxFn>(const_cast<void*>(fn)), dim3((unsigned)((sg.npairs + G - 1) / G)),
+                       dim3(G * S), lds, s, (const C2<F>*)W, (C2<F>*)Sx, sg.pairs, sg.npairs, G, ar,
+                       (const C2<F>*)lp.tw, (const C2<F>*)sg.wph, (const int64_t*)sg.d_step, (F)(1.0 / lp.L));
+    MCS_HIP_TRY(hipGetLastError());
     return MCS_OK;
 }
 
@@ -146,7 +147,8 @@ static int launch_mx_t(const StepGeom& sg, const XArgs<F>& ar, const void* W, vo
     const LinePlan& lp = sg.px;
     const int S = lp.L / P;
     const int G = xupd_mixed_pairs_per_wg(lp);
-    const size_t lds = mixed_lds(lp, G);
+    const int twn = twiddles_in_lds(lp, mixed_lds(lp, G), (sg.npairs + G - 1) / G);
+    const size_t lds = mixed_lds(lp, G) + (size_t)twn * sizeof(C2<F>);  // exchange area, twiddles
     // the noise-free kernels carry no noise draw
     auto fn = ar.noise ? ((64 % S == 0) ? k_xupd_mx<F, P, SPLIT, true, true> : k_xupd_mx<F, P, SPLIT, false, true>)
                        : ((64 % S == 0) ? k_xupd_mx<F, P, SPLIT, true, false> : k_xupd_mx<F, P, SPLIT, false, false>);
@@ -154,7 +156,7 @@ static int launch_mx_t(const StepGeom& sg, const XArgs<F>& ar, const void* W, vo
     if (st) return st;
     hipLaunchKernelGGL(fn, dim3((unsigned)((sg.npairs + G - 1) / G)), dim3(G * S), lds, s, (const C2<F>*)W,
                        (C2<F>*)Sx, sg.pairs, sg.npairs, G, ar, lp.dev, (const C2<F>*)lp.tw, (const C2<F>*)sg.wph,
-                       sg.d_step, (F)(1.0 / lp.L));
+                       sg.d_step, (F)(1.0 / lp.L), twn);
     MCS_HIP_TRY(hipGetLastError());
     return MCS_OK;
 }

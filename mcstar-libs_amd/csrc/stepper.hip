@@ -35,13 +35,17 @@ __global__ void __launch_bounds__(MCS_REG_MAX_THREADS(P, F)) k_row_reg(const C2<
     row_body<F, P, DIR>(src, dst, nrows, G, pl, tw, reinterpret_cast<C2<F>*>(smem_raw), blockIdx.x, threadIdx.x);
 }
 
+// swz: the tile order of xcd_tile (col_tile_order)
 template <typename F, int P, int SPLIT>
 __global__ void __launch_bounds__(MCS_REG_MAX_THREADS(P, F)) k_col_nl(const C2<F>* src, C2<F>* W, ColGeom g, int G,
-                                                 FftPlanDev pl, const C2<F>* __restrict__ tw, F scale, int64_t* d_step) {
+                                                 FftPlanDev pl, const C2<F>* __restrict__ tw, F scale, int64_t* d_step,
+                                                 int swz, int twn) {
     extern __shared__ unsigned char smem_raw[];
+    C2<F>* lds = reinterpret_cast<C2<F>*>(smem_raw);
     const int tilesx = (g.nxc + G - 1) / G;
-    col_nl_body<F, P, SPLIT>(src, W, g, G, pl, tw, scale, reinterpret_cast<C2<F>*>(smem_raw), blockIdx.x % tilesx,
-                             blockIdx.x / tilesx, threadIdx.x);
+    const unsigned blk = xcd_tile(blockIdx.x, gridDim.x, swz, tilesx, g.nouter);
+    const C2<F>* twl = stage_twiddles<F>(tw, twn, lds + G * pl.L);  // behind the exchange area
+    col_nl_body<F, P, SPLIT>(src, W, g, G, pl, twl, scale, lds, blk % tilesx, blk / tilesx, threadIdx.x);
     if (d_step && blockIdx.x == 0 && threadIdx.x == 0) *d_step += 1;
 }
 
@@ -49,11 +53,13 @@ template <typename F, int P, int SPLIT, bool WAVE, bool NOISE>
 __global__ void __launch_bounds__(MCS_XPASS_MAX_THREADS) k_xupd(C2<F>* __restrict__ W, C2<F>* __restrict__ U, const int2* __restrict__ pairs,
                                               int npairs, int G, XArgs<F> ar, FftPlanDev pl,
                                               const C2<F>* __restrict__ tw, const C2<F>* __restrict__ wph,
-                                              const int64_t* __restrict__ d_step) {
+                                              const int64_t* __restrict__ d_step, int twn) {
     extern __shared__ unsigned char smem_raw[];
+    C2<F>* lds = reinterpret_cast<C2<F>*>(smem_raw);
     const uint32_t draw = NOISE && ar.noise ? (uint32_t)*d_step : 0u;
-    xupd_body<F, P, SPLIT, WAVE, RunPlan, NOISE>(W, U, pairs, npairs, G, ar, pl, tw, wph, draw,
-                                                 reinterpret_cast<C2<F>*>(smem_raw), blockIdx.x, threadIdx.x);
+    const C2<F>* twl = stage_twiddles<F>(tw, twn, lds + G * 2 * pl.L);  // behind the exchange area
+    xupd_body<F, P, SPLIT, WAVE, RunPlan, NOISE>(W, U, pairs, npairs, G, ar, pl, twl, wph, draw, lds, blockIdx.x,
+                                                 threadIdx.x);
 }
 
 // ------------------------------------------------------------------ helpers
@@ -125,13 +131,14 @@ static int launch_col_nl_t(const LinePlan& lp, const void* src, void* W, const C
     const int S = lp.L / P;
     const size_t esz = sizeof(C2<F>);
     const int G = col_lines_per_wg(lp);
-    const size_t lds = (size_t)G * lp.L * esz;
+    const long tiles = (long)((g.nxc + G - 1) / G) * g.nouter;
+    const int twn = twiddles_in_lds(lp, (size_t)G * lp.L * esz, tiles);
+    const size_t lds = ((size_t)G * lp.L + twn) * esz;  // exchange area, twiddles
     auto fn = k_col_nl<F, P, SPLIT>;
     int st = set_lds_attr((const void*)fn, lds);
     if (st) return st;
-    const long tiles = (long)((g.nxc + G - 1) / G) * g.nouter;
     hipLaunchKernelGGL(fn, dim3((unsigned)tiles), dim3(G * S), lds, s, (const C2<F>*)src, (C2<F>*)W, g, G, lp.dev,
-                       (const C2<F>*)lp.tw, (F)scale, d_step);
+                       (const C2<F>*)lp.tw, (F)scale, d_step, col_tile_order(g, G, (long)esz), twn);
     MCS_HIP_TRY(hipGetLastError());
     return MCS_OK;
 }
@@ -187,14 +194,16 @@ static int launch_xupd_t(const StepGeom& sg, void* W, void* U, const XArgs<F>& a
     const int S = lp.L / P;
     const size_t esz = sizeof(C2<F>);
     const int G = xupd_pairs_per_wg(lp);
-    const size_t lds = (size_t)G * 2 * lp.L * esz;
+    const int twn = twiddles_in_lds(lp, (size_t)G * 2 * lp.L * esz, (sg.npairs + G - 1) / G);
+    const size_t lds = ((size_t)G * 2 * lp.L + twn) * esz;  // exchange area, twiddles
     // the noise-free kernels carry no noise draw
     auto fn = ar.noise ? ((64 % S == 0) ? k_xupd<F, P, SPLIT, true, true> : k_xupd<F, P, SPLIT, false, true>)
                        : ((64 % S == 0) ? k_xupd<F, P, SPLIT, true, false> : k_xupd<F, P, SPLIT, false, false>);
     int st = set_lds_attr((const void*)fn, lds);
     if (st) return st;
     hipLaunchKernelGGL(fn, dim3((unsigned)((sg.npairs + G - 1) / G)), dim3(G * S), lds, s, (C2<F>*)W, (C2<F>*)U,
-                       sg.pairs, sg.npairs, G, ar, lp.dev, (const C2<F>*)lp.tw, (const C2<F>*)sg.wph, sg.d_step);
+                       sg.pairs, sg.npairs, G, ar, lp.dev, (const C2<F>*)lp.tw, (const C2<F>*)sg.wph, sg.d_step,
+                       twn);
     MCS_HIP_TRY(hipGetLastError());
     return MCS_OK;
 }

@@ -18,6 +18,7 @@
 //   --lowpass DIVS       (2D: amplitude_<step>.npy and lowpass_<step>.gz at every output,
 //                         Gaussian width 1.1 DIVS h)
 //   --plan Px,Gx,Py,Gy,Pz,Gz  (explicit plans; default: tuned)   --no-tune
+//   --form spectral|mixed     (the state's form; a resumed run takes the checkpoint's)
 #include <cuda/array_abstraction.hpp>
 #include <cuda/cufft_wrapper.hpp>
 #include <cuda/math.hpp>
@@ -51,6 +52,7 @@ namespace {
 struct Opts {
     int3 dims{64, 64, 64};
     double h = M_PI / 4, eps = 0.3, dt = 0.5, psi0 = -0.25, amp = 0.05, temp = 0, ths = 0, rlim = 5.5, lowpass = 0;
+    int form = -1;  // -1: the library's default
     uint64_t seed = 11, noise_seed = 0;
     int split = MCS_SPLIT_NORTHSTAR, steps = 100, every = 0;
     std::string out = ".", load, resume;
@@ -111,6 +113,11 @@ Opts parse(int argc, char** argv) {
         else if (k == "--rlim") o.rlim = std::atof(v.c_str());
         else if (k == "--lowpass") o.lowpass = std::atof(v.c_str());
         else if (k == "--plan") o.plan = parse_list(v);
+        else if (k == "--form") {
+            if (v == "spectral") o.form = MCS_FORM_SPECTRAL;
+            else if (v == "mixed") o.form = MCS_FORM_MIXED;
+            else usage("--form takes spectral or mixed");
+        }
         else usage("unknown flag " + k);
     }
     if (o.steps < 0 || o.every < 0) usage("--steps and --every must be >= 0");
@@ -256,6 +263,7 @@ int main(int argc, char** argv) {
     GPUArray psi(host);
     std::unique_ptr<Lowpass> lp(o.lowpass > 0 ? new Lowpass(o) : nullptr);
     int64_t step0 = 0;
+    if (!resuming && o.form >= 0) CUERR(mcs_stepper_set_form(st, o.form));
     if (resuming) {
         CUERR(mcs_stepper_set_form(st, res.form));
         CUERR(mcs_stepper_set_state(st, res.spec.data(), res.mass, res.step));

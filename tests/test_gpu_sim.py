@@ -60,6 +60,24 @@ def test_resume_continues_bit_for_bit(tmp_path, prec):
     assert "step 16" in open(os.path.join(b, "checkpoint.txt")).read()
 
 
+def test_mixed_form_checkpoint_resumes_bit_for_bit(tmp_path):
+    # the checkpoint names the state's form; a resumed run restores it (whatever the library's
+    # default) and continues bit for bit
+    a, b = str(tmp_path / "a"), str(tmp_path / "b")
+    os.makedirs(a)
+    os.makedirs(b)
+    common = ["--dims", "64x48x32", "--eps", 0.3, "--dt", 0.25, "--temp", 1e-3, "--noise-seed", 4]
+    sim("f32", *common, "--form", "mixed", "--steps", 12, "--out", a)
+    sim("f32", *common, "--form", "mixed", "--steps", 5, "--out", b)
+    assert "form 1" in open(os.path.join(b, "checkpoint.txt")).read()
+    sim("f32", "--resume", b, "--steps", 7, "--out", b)
+    assert np.array_equal(np.load(os.path.join(a, "psi_00000012.npy")), np.load(os.path.join(b, "psi_00000012.npy")))
+    assert "form 1" in open(os.path.join(b, "checkpoint.txt")).read()
+    r = subprocess.run([os.path.join(BIN, "mcs_sim_f32"), "--form", "exact"], capture_output=True, text=True,
+                       timeout=60)
+    assert r.returncode == 2 and "--form takes" in r.stderr
+
+
 @pytest.mark.parametrize("dims", ["2048x2048", "2560x768"])
 def test_resume_without_tuning_continues_bit_for_bit(tmp_path, dims):
     """--no-tune runs the default tiles (2 lines per workgroup for the 2048-point FP32 y pass; 3 row
