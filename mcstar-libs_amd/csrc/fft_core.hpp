@@ -372,6 +372,49 @@ template <typename F, int P, int NS, int I = 1> MCS_DEV void lane_transpose(C2<F
     }
 }
 
+// Two lines transformed together pass by pass (the x pass's row and mirror row, both owned by
+// the same threads): every pass writes both lines' butterfly outputs, then one synchronisation,
+// then both read back, so a pair takes half the synchronisations (and their latency) of two
+// reg_fft calls.  Each line sees the same operations as in reg_fft, so the results are the same.
+template <typename F, int P, int R, int DIR>
+MCS_DEV void rpass2(C2<F> (&a)[P], C2<F> (&b)[P], int t, int S, int Ns, unsigned nsm, const C2<F>* __restrict__ tw,
+                    C2<F>* lds, int la, int lb, int lps, bool last) {
+    rpass<F, P, R, DIR>(a, t, S, Ns, nsm, tw, lds, la, lps, last);
+    rpass<F, P, R, DIR>(b, t, S, Ns, nsm, tw, lds, lb, lps, last);
+}
+
+template <typename F, int P, int DIR, bool WAVE = false>
+MCS_DEV void reg_fft2(C2<F> (&a)[P], C2<F> (&b)[P], const FftPlanDev& pl, const C2<F>* __restrict__ tw, int t,
+                      C2<F>* lds, int la, int lb, int lps) {
+    const int S = pl.L / P;
+    for (int p = 0; p < pl.np; ++p) {
+        const bool last = (p == pl.np - 1);
+        const int R = pl.radix[p];
+        const int Ns = pl.ns[p];
+        const unsigned M = pl.nsm[p];
+        const C2<F>* twp = tw + pl.twoff[p];
+        switch (R) {
+            case 2: if constexpr (P % 2 == 0) rpass2<F, P, 2, DIR>(a, b, t, S, Ns, M, twp, lds, la, lb, lps, last); break;
+            case 3: if constexpr (P % 3 == 0) rpass2<F, P, 3, DIR>(a, b, t, S, Ns, M, twp, lds, la, lb, lps, last); break;
+            case 4: if constexpr (P % 4 == 0) rpass2<F, P, 4, DIR>(a, b, t, S, Ns, M, twp, lds, la, lb, lps, last); break;
+            case 5: if constexpr (P % 5 == 0) rpass2<F, P, 5, DIR>(a, b, t, S, Ns, M, twp, lds, la, lb, lps, last); break;
+            case 8: if constexpr (P % 8 == 0) rpass2<F, P, 8, DIR>(a, b, t, S, Ns, M, twp, lds, la, lb, lps, last); break;
+            case 16: if constexpr (P % 16 == 0) rpass2<F, P, 16, DIR>(a, b, t, S, Ns, M, twp, lds, la, lb, lps, last); break;
+            case 32: if constexpr (P % 32 == 0) rpass2<F, P, 32, DIR>(a, b, t, S, Ns, M, twp, lds, la, lb, lps, last); break;
+            default: break;
+        }
+        if (!last) {
+            lds_sync<WAVE>();
+#pragma unroll
+            for (int m = 0; m < P; ++m) {
+                a[m] = lds[la + (t + m * S) * lps];
+                b[m] = lds[lb + (t + m * S) * lps];
+            }
+            lds_sync<WAVE>();
+        }
+    }
+}
+
 // ------------------------------------------------------------ fixed plans
 // The register engine with the whole plan known at compile time: length L and
 // radices Rs... in pass order (sub-lengths and twiddle-table offsets follow as
@@ -446,6 +489,71 @@ MCS_DEV void fix_pass(C2<F> (&a)[P], const C2<F>* __restrict__ tw, int t, C2<F>*
     }
 }
 
+// the butterflies of one fixed pass over one line: outputs to LDS, or back to a on the last pass
+template <typename F, int P, int DIR, int L, int Ns, int TwOff, int R, bool LAST>
+MCS_DEV void fix_bfly(C2<F> (&a)[P], const C2<F>* __restrict__ tw, int t, C2<F>* lds, int lbase, int lps) {
+    constexpr int S = L / P;
+    constexpr int B = P / R;
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+        C2<F> v[R];
+#pragma unroll
+        for (int q = 0; q < R; ++q) v[q] = a[b + q * B];
+        const int j = t + b * S;
+        const int k = Ns > 1 ? j % Ns : 0;
+        if constexpr (Ns > 1) apply_tw<F, R, DIR>(v, tw + TwOff, k);
+        dft<F, R, DIR>(v);
+        if constexpr (LAST) {
+#pragma unroll
+            for (int q = 0; q < R; ++q) a[b + q * B] = v[q];
+        } else {
+            const int base = (j - k) * R + k;
+#pragma unroll
+            for (int q = 0; q < R; ++q) lds[lbase + (base + q * Ns) * lps] = v[q];
+        }
+    }
+}
+
+// two lines together pass by pass, as reg_fft2 (SHUF as for fix_pass)
+template <typename F, int P, int DIR, bool WAVE, bool SHUF, int L, int Ns, int TwOff, int R, int... Rest>
+MCS_DEV void fix_pass2(C2<F> (&a)[P], C2<F> (&b)[P], const C2<F>* __restrict__ tw, int t, C2<F>* lds, int la, int lb,
+                       int lps) {
+    constexpr int S = L / P;
+    constexpr bool last = sizeof...(Rest) == 0;
+    static_assert(P % R == 0, "every radix of a register plan divides P");
+    if constexpr (!last) {
+        constexpr bool shuffle = SHUF && WAVE && sizeof...(Rest) == 1 && R == P && first_radix<Rest...>() == P &&
+                                 64 % S == 0;
+        if constexpr (shuffle) {
+            const int k = Ns > 1 ? t % Ns : 0;
+            if constexpr (Ns > 1) {
+                apply_tw<F, R, DIR>(a, tw + TwOff, k);
+                apply_tw<F, R, DIR>(b, tw + TwOff, k);
+            }
+            dft<F, R, DIR>(a);
+            dft<F, R, DIR>(b);
+            lane_transpose<F, P, Ns>(a);
+            lane_transpose<F, P, Ns>(b);
+            fix_pass2<F, P, DIR, WAVE, SHUF, L, Ns * R, TwOff + (Ns > 1 ? Ns * (R - 1) : 0), Rest...>(a, b, tw, t, lds,
+                                                                                                la, lb, lps);
+            return;
+        }
+    }
+    fix_bfly<F, P, DIR, L, Ns, TwOff, R, last>(a, tw, t, lds, la, lps);
+    fix_bfly<F, P, DIR, L, Ns, TwOff, R, last>(b, tw, t, lds, lb, lps);
+    if constexpr (!last) {
+        lds_sync<WAVE>();
+#pragma unroll
+        for (int m = 0; m < P; ++m) {
+            a[m] = lds[la + (t + m * S) * lps];
+            b[m] = lds[lb + (t + m * S) * lps];
+        }
+        lds_sync<WAVE>();
+        fix_pass2<F, P, DIR, WAVE, SHUF, L, Ns * R, TwOff + (Ns > 1 ? Ns * (R - 1) : 0), Rest...>(a, b, tw, t, lds, la,
+                                                                                            lb, lps);
+    }
+}
+
 template <typename F, int P, int DIR, bool WAVE, int L, int... Rs>
 MCS_DEV void fix_fft(C2<F> (&a)[P], const C2<F>* __restrict__ tw, int t, C2<F>* lds, int lbase, int lps,
                      FixPlan<L, Rs...>) {
@@ -465,6 +573,20 @@ template <int Ns, int R, int... Rest> constexpr int fix_tw_count_from() {
     else return (Ns > 1 ? Ns * (R - 1) : 0) + fix_tw_count_from<Ns * R, Rest...>();
 }
 template <int L, int... Rs> constexpr int fix_tw_count(FixPlan<L, Rs...>) { return fix_tw_count_from<1, Rs...>(); }
+
+template <typename F, int P, int DIR, bool WAVE, bool SHUF, int L, int... Rs>
+MCS_DEV void fix_fft2(C2<F> (&a)[P], C2<F> (&b)[P], const C2<F>* __restrict__ tw, int t, C2<F>* lds, int la, int lb,
+                      int lps, FixPlan<L, Rs...>) {
+    fix_pass2<F, P, DIR, WAVE, SHUF, L, 1, 0, Rs...>(a, b, tw, t, lds, la, lb, lps);
+}
+
+// two lines through line_fft's transform together, pass by pass (LDS areas la and lb)
+template <typename F, int P, int DIR, bool WAVE, typename FP, bool SHUF = false>
+MCS_DEV void line_fft2(C2<F> (&a)[P], C2<F> (&b)[P], const FftPlanDev& pl, const C2<F>* __restrict__ tw, int t,
+                       C2<F>* lds, int la, int lb, int lps) {
+    if constexpr (FP::fixed) fix_fft2<F, P, DIR, WAVE, SHUF>(a, b, tw, t, lds, la, lb, lps, FP{});
+    else reg_fft2<F, P, DIR, WAVE>(a, b, pl, tw, t, lds, la, lb, lps);
+}
 
 // twiddle entries of the runtime plan pl or of the fixed plan FP
 template <typename FP> MCS_DEV int line_tw_count(const FftPlanDev& pl) {

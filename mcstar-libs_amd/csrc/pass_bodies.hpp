@@ -191,8 +191,7 @@ MCS_DEV void xupd_body(C2<F>* __restrict__ W, C2<F>* __restrict__ U, const int2*
         a[m] = valid ? W[rc * L + p] : mk<F>(0, 0);
         b[m] = valid ? W[rm * L + p] : mk<F>(0, 0);
     }
-    line_fft<F, P, -1, WAVE, FP, SHUF>(a, pl, tw, t, lds, base, 1);
-    line_fft<F, P, -1, WAVE, FP, SHUF>(b, pl, tw, t, lds, base + L, 1);
+    line_fft2<F, P, -1, WAVE, FP, SHUF>(a, b, pl, tw, t, lds, base, base + L, 1);
 #pragma unroll
     for (int m = 0; m < P; ++m) {
         const int p = t + m * S;
@@ -235,8 +234,7 @@ MCS_DEV void xupd_body(C2<F>* __restrict__ W, C2<F>* __restrict__ U, const int2*
 #pragma unroll
     for (int m = 0; m < P; ++m) b[m] = self ? a[m] : Nm[t + m * S];
     lds_sync<WAVE>();
-    line_fft<F, P, +1, WAVE, FP, SHUF>(a, pl, tw, t, lds, base, 1);
-    line_fft<F, P, +1, WAVE, FP, SHUF>(b, pl, tw, t, lds, base + L, 1);
+    line_fft2<F, P, +1, WAVE, FP, SHUF>(a, b, pl, tw, t, lds, base, base + L, 1);
     if (valid) {
 #pragma unroll
         for (int m = 0; m < P; ++m) {
