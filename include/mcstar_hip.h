@@ -302,9 +302,10 @@ typedef struct mcs_ensemble mcs_ensemble;
                              pass fall in one register tier share one persistent work-table launch) or one
                              launch per pass and size group, whichever ran faster when created */
 #define MCS_ENS_GRAPH 2   /* replayed from a captured HIP graph of 8 steps: either the plan-class launches
-                             (the classes of one pass on parallel branches) or one branch per size group,
-                             whichever replayed faster when the ensemble was created (both compute the same
-                             bits); remainder steps run batched */
+                             (the classes of one pass on parallel branches) or the size groups on parallel
+                             branches (one per group, or packed onto fewer), whichever replayed faster when
+                             the ensemble was created (all compute the same bits); remainder steps run
+                             batched */
 int mcs_ensemble_create(mcs_ensemble** e, const mcs_replica* reps, int nrep, int split, int prec);
 /* seeded initial fields for every replica */
 int mcs_ensemble_init(mcs_ensemble* e);
@@ -335,6 +336,14 @@ int mcs_ensemble_launch_info(mcs_ensemble* e, int* groups, int* classes_per_step
  * the step graph through the plan classes (1) or per size group (0); the results are the same bits
  * either way.  MCS_ERR_UNSUPPORTED when the ensemble has no classes. */
 int mcs_ensemble_set_launch_form(mcs_ensemble* e, int batched_classes, int graph_classes);
+/* Instrumentation (no reference counterpart): the branches of the per-group step graph.  The size
+ * groups are dealt onto them longest first by their step time alone at creation, and each branch
+ * runs its groups one after the other; a process has 4 hardware queues by default, so this chooses
+ * which groups share one.  At creation one branch per group and packings onto 4, 3 and 2 branches
+ * are timed and the fastest kept.  set: 0 = one branch per group, n > 0 = packed onto n branches
+ * (the same bits either way; the graphs are captured again on their next use). */
+int mcs_ensemble_graph_branches(mcs_ensemble* e, int* branches);
+int mcs_ensemble_set_graph_branches(mcs_ensemble* e, int branches);
 
 /* ---------------------------------------------------------------- peaks
  * call_kernel_findpeaks / _2d (cuda/kernels/findpeaks.hpp:7-63,

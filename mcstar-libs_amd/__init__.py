@@ -138,6 +138,8 @@ _SIGS = {
                                   ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                                   ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
     "mcs_ensemble_set_launch_form": ([_vp, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+    "mcs_ensemble_graph_branches": ([_vp, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
+    "mcs_ensemble_set_graph_branches": ([_vp, ctypes.c_int], ctypes.c_int),
     "mcs_findpeaks": ([_vp, _ip, ctypes.c_double, _i64p, _dp, ctypes.c_int64, _i64p, ctypes.c_int], ctypes.c_int),
     "mcs_bond_order_cutoff": ([_dp, ctypes.c_int, ctypes.c_double, ctypes.c_double, _ip, _dp], ctypes.c_int),
     "mcs_bond_order_knn": ([_ip, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, _ip, _dp, _i64p,
@@ -825,13 +827,19 @@ class Ensemble:
         check(lib().mcs_ensemble_launch_info(self.ptr, ctypes.byref(g), ctypes.byref(c), ctypes.byref(bc),
                                              ctypes.byref(gc), ms), "mcs_ensemble_launch_info")
         names = ["batched_per_group", "batched_classes", "graph_per_group", "graph_classes"]
+        nb = ctypes.c_int()
+        check(lib().mcs_ensemble_graph_branches(self.ptr, ctypes.byref(nb)), "mcs_ensemble_graph_branches")
         return {"size_groups": g.value, "class_launches_per_step": c.value, "batched_uses_classes": bool(bc.value),
-                "graph_replays_classes": bool(gc.value),
+                "graph_replays_classes": bool(gc.value), "graph_branches": nb.value,
                 "ms_per_step_at_create": {n: round(ms[i], 4) for i, n in enumerate(names)}}
 
     def set_launch_form(self, batched_classes, graph_classes):
         check(lib().mcs_ensemble_set_launch_form(self.ptr, int(batched_classes), int(graph_classes)),
               "mcs_ensemble_set_launch_form")
+
+    def set_graph_branches(self, branches):
+        """per-group step graph on `branches` packed branches (0: one per size group)"""
+        check(lib().mcs_ensemble_set_graph_branches(self.ptr, int(branches)), "mcs_ensemble_set_graph_branches")
 
     def run(self, nsteps, ths, k=12, nbins=64):
         """nsteps graph-replayed steps with peaks and q4/q6 histograms captured after the last one"""

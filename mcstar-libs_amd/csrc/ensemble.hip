@@ -86,6 +86,10 @@ struct mcs_ensemble {
     double strat_ms[4] = {-1, -1, -1, -1};   // per step: eager per group, eager classes, graph per group, graph classes
     std::vector<hipStream_t> cside;          // fork streams of the class launches of one pass
     std::vector<hipEvent_t> cev;             // [0] fork, [1 + i] join of class i
+    // branches of the per-group step graph: the size groups each branch runs one after the other
+    // (empty: one branch per group)
+    std::vector<std::vector<int>> packs;
+    std::vector<double> gcost;               // per size group: ms per step launched alone (at creation)
 };
 
 
@@ -528,18 +532,43 @@ template <typename F> static int ens_branches_t(mcs_ensemble* e, int nsteps, hip
         if (!st) st = ens_advance(e, nsteps, s);
         return st;
     }
+    std::vector<std::vector<int>> packs = e->packs;
+    if (packs.empty())
+        for (int gi = 0; gi < ng; ++gi) packs.push_back({gi});
     if (hipEventRecord(e->side_ev[0], s) != hipSuccess) return MCS_ERR_HIP;
-    for (int gi = 0; gi < ng && !st; ++gi) {
-        hipStream_t b = e->side[gi];
+    for (size_t bi = 0; bi < packs.size() && !st; ++bi) {
+        hipStream_t b = e->side[bi];
         if (hipStreamWaitEvent(b, e->side_ev[0], 0) != hipSuccess) return MCS_ERR_HIP;
-        EnsGroup* g = e->groups[gi];
-        for (int it = 0; it < nsteps && !st; ++it) st = ens_step_group<F>(e, g, 0, (int)g->members.size(), it, b);
-        if (!st && analysis) st = ens_analysis_group(e, gi, b);
-        if (!st && hipEventRecord(e->side_ev[1 + gi], b) != hipSuccess) st = MCS_ERR_HIP;
-        if (!st && hipStreamWaitEvent(s, e->side_ev[1 + gi], 0) != hipSuccess) st = MCS_ERR_HIP;
+        for (int gi : packs[bi]) {
+            EnsGroup* g = e->groups[gi];
+            for (int it = 0; it < nsteps && !st; ++it) st = ens_step_group<F>(e, g, 0, (int)g->members.size(), it, b);
+            if (!st && analysis) st = ens_analysis_group(e, gi, b);
+        }
+        if (!st && hipEventRecord(e->side_ev[1 + bi], b) != hipSuccess) st = MCS_ERR_HIP;
+        if (!st && hipStreamWaitEvent(s, e->side_ev[1 + bi], 0) != hipSuccess) st = MCS_ERR_HIP;
     }
     if (!st) st = ens_advance(e, nsteps, s);
     return st;
+}
+
+// The size groups dealt onto nb branches, longest first, each to the branch with the least work
+// so far (cost: a group's milliseconds per step launched alone).  A process has few hardware
+// queues (4 by default) and branches beyond them share one in order, so with one branch per
+// group which groups wait on each other is left to the runtime; packed branches choose it.
+static std::vector<std::vector<int>> ens_pack(const std::vector<double>& cost, int nb) {
+    std::vector<int> order(cost.size());
+    for (size_t i = 0; i < order.size(); ++i) order[i] = (int)i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+    std::vector<std::vector<int>> packs(nb);
+    std::vector<double> load(nb, 0.0);
+    for (int gi : order) {
+        const int b = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        packs[b].push_back(gi);
+        load[b] += cost[gi];
+    }
+    packs.erase(std::remove_if(packs.begin(), packs.end(), [](const std::vector<int>& v) { return v.empty(); }),
+                packs.end());
+    return packs;
 }
 
 static constexpr int GRAPH_STEPS = 8;
@@ -589,35 +618,85 @@ static int ens_choose_graph(mcs_ensemble* e, hipStream_t s) {
     MCS_HIP_TRY(hipEventCreate(&b));
     double best = -1;
     int st = MCS_OK;
-    for (int use = 0; use <= 1 && !st; ++use) {
-        if (use && e->cls.empty()) break;
-        e->graph_classes = use != 0;
+    // each size group's steps launched alone, the cost the branch packing balances
+    const int ng = (int)e->groups.size();
+    std::vector<double> cost(ng, 1.0);
+    for (int gi = 0; gi < ng && !st; ++gi) {
+        EnsGroup* g = e->groups[gi];
+        const int nm = (int)g->members.size();
+        float el = 0;
+        st = e->prec == 64 ? ens_step_group<double>(e, g, 0, nm, 0, s) : ens_step_group<float>(e, g, 0, nm, 0, s);
+        if (!st && hipEventRecord(a, s) != hipSuccess) st = MCS_ERR_HIP;
+        for (int it = 0; it < 4 && !st; ++it)
+            st = e->prec == 64 ? ens_step_group<double>(e, g, 0, nm, it, s) : ens_step_group<float>(e, g, 0, nm, it, s);
+        if (!st && (hipEventRecord(b, s) != hipSuccess || hipEventSynchronize(b) != hipSuccess ||
+                    hipEventElapsedTime(&el, a, b) != hipSuccess))
+            st = MCS_ERR_HIP;
+        if (!st && el > 0) cost[gi] = el / 4.0;
+    }
+    e->gcost = cost;
+    // the step graph's candidates: one branch per group, the groups packed onto 4, 3 and 2
+    // branches, and the class launches
+    std::vector<std::vector<std::vector<int>>> forms = {{}};
+    for (int nb : {4, 3, 2})
+        if (nb < ng) forms.push_back(ens_pack(cost, nb));
+    const size_t nforms = forms.size();
+    // every candidate captured, then timed in interleaved rounds (3 replays each), keeping each
+    // one's fastest round: single timings of one candidate after another varied by about 10%
+    std::vector<hipGraph_t> gs;
+    std::vector<hipGraphExec_t> xs;
+    std::vector<double> ms;
+    for (size_t use = 0; use <= nforms && !st; ++use) {
+        if (use == nforms && e->cls.empty()) break;
+        e->graph_classes = use == nforms;
+        if (use < nforms) e->packs = forms[use];
         hipGraph_t g = nullptr;
         hipGraphExec_t x = nullptr;
         st = ens_capture(e, s, false, &g, &x);
+        e->graph_classes = false;
         if (st) break;
-        double ms = -1;
-        float el = 0;
-        if (hipGraphLaunch(x, s) == hipSuccess && hipEventRecord(a, s) == hipSuccess) {
-            bool ok = true;
-            for (int i = 0; i < 3 && ok; ++i) ok = hipGraphLaunch(x, s) == hipSuccess;
+        gs.push_back(g);
+        xs.push_back(x);
+        ms.push_back(-1);
+        if (hipGraphLaunch(x, s) != hipSuccess) st = MCS_ERR_HIP;
+    }
+    for (int round = 0; round < 3 && !st; ++round)
+        for (size_t i = 0; i < xs.size() && !st; ++i) {
+            float el = 0;
+            bool ok = hipEventRecord(a, s) == hipSuccess;
+            for (int k = 0; k < 3 && ok; ++k) ok = hipGraphLaunch(xs[i], s) == hipSuccess;
             if (ok && hipEventRecord(b, s) == hipSuccess && hipEventSynchronize(b) == hipSuccess &&
-                hipEventElapsedTime(&el, a, b) == hipSuccess)
-                ms = el;
+                hipEventElapsedTime(&el, a, b) == hipSuccess && el > 0) {
+                if (ms[i] < 0 || el < ms[i]) ms[i] = el;
+            }
         }
-        if (ms > 0) e->strat_ms[2 + use] = ms / (3.0 * GRAPH_STEPS);
-        if (ms > 0 && (best < 0 || ms < best)) {
-            best = ms;
-            if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
-            if (e->graph) (void)hipGraphDestroy(e->graph);
-            e->gexec = x;
-            e->graph = g;
-            e->gsteps = GRAPH_STEPS;
-        } else {
-            (void)hipGraphExecDestroy(x);
-            (void)hipGraphDestroy(g);
-            if (use) e->graph_classes = false;
-        }
+    size_t besti = xs.size(), best_pg = 0;
+    for (size_t i = 0; i < xs.size(); ++i) {
+        if (ms[i] <= 0) continue;
+        const int slot = i == nforms ? 3 : 2;
+        const double per = ms[i] / (3.0 * GRAPH_STEPS);
+        if (e->strat_ms[slot] < 0 || per < e->strat_ms[slot]) e->strat_ms[slot] = per;
+        if (besti == xs.size() || ms[i] < ms[besti]) besti = i;
+        if (i < nforms && (ms[best_pg] <= 0 || ms[i] < ms[best_pg])) best_pg = i;
+    }
+    // the per-group form keeps its fastest packing also when the class launches won
+    if (!st && !forms.empty()) e->packs = forms[best_pg];
+    for (size_t i = 0; i < xs.size(); ++i) {
+        if (i == besti) continue;
+        (void)hipGraphExecDestroy(xs[i]);
+        (void)hipGraphDestroy(gs[i]);
+    }
+    if (!st && besti < xs.size()) {
+        best = ms[besti];
+        if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
+        if (e->graph) (void)hipGraphDestroy(e->graph);
+        e->gexec = xs[besti];
+        e->graph = gs[besti];
+        e->gsteps = GRAPH_STEPS;
+        e->graph_classes = besti == nforms;
+    } else if (besti < xs.size()) {
+        (void)hipGraphExecDestroy(xs[besti]);
+        (void)hipGraphDestroy(gs[besti]);
     }
     // eager batched steps: per-group launches against class launches, the same way
     double best_eager = -1;
@@ -868,6 +947,31 @@ extern "C" int mcs_ensemble_set_launch_form(mcs_ensemble* e, int batched_classes
         e->aexec = nullptr;
         e->agraph = nullptr;
     }
+    return MCS_OK;
+}
+
+extern "C" int mcs_ensemble_graph_branches(mcs_ensemble* e, int* branches) {
+    MCS_CHECK(e && branches, MCS_ERR_INVALID, "null argument");
+    *branches = e->packs.empty() ? (int)e->groups.size() : (int)e->packs.size();
+    return MCS_OK;
+}
+
+extern "C" int mcs_ensemble_set_graph_branches(mcs_ensemble* e, int branches) {
+    MCS_CHECK(e && branches >= 0, MCS_ERR_INVALID, "branches must be >= 0");
+    MCS_HIP_TRY(hipStreamSynchronize(mcs_stream()));
+    const int ng = (int)e->groups.size();
+    std::vector<double> cost = e->gcost;
+    cost.resize(ng, 1.0);
+    e->packs = (branches == 0 || branches >= ng) ? std::vector<std::vector<int>>{} : ens_pack(cost, branches);
+    // the graphs are captured again on their next use
+    if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
+    if (e->graph) (void)hipGraphDestroy(e->graph);
+    e->gexec = nullptr;
+    e->graph = nullptr;
+    if (e->aexec) (void)hipGraphExecDestroy(e->aexec);
+    if (e->agraph) (void)hipGraphDestroy(e->agraph);
+    e->aexec = nullptr;
+    e->agraph = nullptr;
     return MCS_OK;
 }
 

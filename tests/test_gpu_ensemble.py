@@ -57,6 +57,27 @@ def test_launch_forms_agree_bit_for_bit(mcs, form):
         assert np.array_equal(a.get_psi(r), b.get_psi(r)), r
 
 
+@pytest.mark.parametrize("branches", [1, 2, 3])
+def test_packed_graph_branches_agree_bit_for_bit(mcs, branches):
+    # the per-group step graph with the size groups packed onto fewer branches (each running its
+    # groups one after the other) against the looping launches
+    reps = small_3d() + small_2d()
+    a = mcs.Ensemble(reps, dtype=np.float32)
+    a.init()
+    a.step(19, mcs.ENS_LOOP)
+    b = mcs.Ensemble(reps, dtype=np.float32)
+    b.set_launch_form(0, 0)
+    b.set_graph_branches(branches)
+    b.init()
+    b.step(3, mcs.ENS_BATCHED)
+    b.step(16, mcs.ENS_GRAPH)
+    info = b.launch_info()
+    assert not info["graph_replays_classes"]
+    assert info["graph_branches"] == min(branches, info["size_groups"])
+    for r in range(len(reps)):
+        assert np.array_equal(a.get_psi(r), b.get_psi(r)), r
+
+
 def test_graph_replays_between_eager_steps(mcs):
     # the captured graph holds one branch per size group; replays interleaved with eager steps and host
     # synchronisations must continue the same noise sequence as the looping launches
