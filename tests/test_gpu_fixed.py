@@ -19,6 +19,10 @@ CASES = [
     ((256, 256, 256), np.float32, 1e-3, 1, [16, 8, 8, 32, 16, 8]),
     ((256, 256, 256), np.float64, 1e-3, 0, [8, 4, 8, 8, 16, 8]),
     ((256, 256), np.float64, 0.0, 0, [8, 4, 8, 8]),
+    # 4-point x plans (a row pair per wavefront)
+    ((512, 512, 512), np.float32, 0.0, 0, [4, 2, 32, 16, 32, 16]),
+    ((256, 256, 256), np.float64, 1e-3, 1, [4, 4, 8, 8, 16, 8]),
+    ((256, 256), np.float32, 1e-3, 0, [4, 2, 8, 8]),
     ((256, 256, 256), np.float32, 1e-3, 0, None),
     ((512, 512, 512), np.float64, 0.0, 0, None),
 ]
@@ -99,13 +103,16 @@ def test_fixed_boundary_fft_bit_identical_to_runtime_plans(mcs, shp, dtype):
 
 
 SHUFFLE_CASES = [
-    # shape, dtype, temp, split, plan: x plans whose last two passes have radix P (4*8*8 and 16*16 for
-    # nx/2 = 256, 2*8*8 for 128), so the mixed form's fixed x kernels have a lane-shuffle form
+    # shape, dtype, temp, split, plan: x plans whose last two passes have radix P (4*8*8, 16*16 and
+    # 4*4*4*4 for nx/2 = 256, 2*8*8 and 2*4*4*4 for 128), so the fixed x kernels have a lane-shuffle form
     ((512, 512, 512), np.float32, 0.0, 0, [8, 2, 8, 16, 32, 16]),
     ((512, 512, 512), np.float32, 0.0, 1, [16, 4, 8, 16, 32, 16]),
     ((256, 256, 256), np.float32, 1e-3, 0, [8, 4, 16, 16, 8, 16]),
     ((512, 512, 512), np.float64, 0.0, 0, [8, 2, 8, 8, 16, 8]),
     ((256, 256), np.float64, 1e-3, 1, [8, 4, 8, 8]),
+    # 4*4*4*4 and 2*4*4*4: a 4 x 4 transpose over lanes 16 and 32 apart (v_permlane16/32_swap)
+    ((512, 512, 512), np.float32, 0.0, 0, [4, 2, 32, 16, 32, 16]),
+    ((256, 256, 256), np.float32, 1e-3, 0, [4, 4, 16, 16, 8, 16]),
 ]
 
 
