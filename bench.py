@@ -99,10 +99,15 @@ def measured_traffic(pass_name, n):
     return None
 
 
-def ensemble_bench(mcs, kind, steps):
-    """replica-steps/s of the batched, graph-replayed ensemble against looping single-replica launches"""
+def ensemble_bench(mcs, kind, steps, pin=None):
+    """replica-steps/s of the batched, graph-replayed ensemble against looping single-replica launches;
+    pin (profiling): the step graph replays per-group launches on that many packed branches (0: one
+    per size group) instead of the form the ensemble timed fastest at creation"""
     reps = mcs.ensemble_config(kind)
     e = mcs.Ensemble(reps, dtype=np.float32)
+    if pin is not None:
+        e.set_launch_form(e.launch_info()["batched_uses_classes"], False)
+        e.set_graph_branches(pin)
     e.init()
     e.step(8, mcs.ENS_GRAPH)
     cells = sum(int(np.prod(r["shape"])) for r in reps)
@@ -250,6 +255,9 @@ def main():
     ap.add_argument("--no-noise-config", action="store_true", help="skip the 256^3 noise-on line")
     ap.add_argument("--only-ensemble", action="store_true", help="run only the ensemble lines (profiling)")
     ap.add_argument("--only-fft", action="store_true", help="run only the boundary FFT lines (profiling)")
+    ap.add_argument("--ens-pin", default=None,
+                    help="with --only-ensemble (profiling): pin each step graph to per-group launches on N packed "
+                         "branches, e.g. 2d=0,3d=2 (0: one branch per size group)")
     ap.add_argument("--only-analysis", action="store_true",
                     help="run only the 500-step FP32 field and its peak / bond-order analysis (profiling)")
     ap.add_argument("--no-other-form", action="store_true", help="skip the other state form's comparison line")
@@ -281,7 +289,10 @@ def main():
         mcs.check(mcs.lib().mcs_memcpy(dst.ptr, src.ptr, nb, mcs.D2D), "mcs_memcpy")
         del src, dst
     if a.only_ensemble:
-        print(json.dumps({"ensemble_2d": ensemble_bench(mcs, "2d", 40), "ensemble_3d": ensemble_bench(mcs, "3d", 32)}))
+        pins = dict(kv.split("=") for kv in a.ens_pin.split(",")) if a.ens_pin else {}
+        pin = {k: int(v) for k, v in pins.items()}
+        print(json.dumps({"ensemble_2d": ensemble_bench(mcs, "2d", 40, pin.get("2d")),
+                          "ensemble_3d": ensemble_bench(mcs, "3d", 32, pin.get("3d"))}))
         return
     if a.only_fft:
         print(json.dumps({"boundary_fft_512": fft_bench(mcs, a.n, 5),
