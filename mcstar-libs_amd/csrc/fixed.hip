@@ -150,9 +150,14 @@ __global__ void __launch_bounds__(MCS_XPASS_MAX_THREADS) k_xupd_fx(C2<F>* __rest
     extern __shared__ unsigned char smem_raw[];
     const uint32_t draw = NOISE && ar.noise ? (uint32_t)*d_step : 0u;
     const FftPlanDev unused{};
-    // (twiddles staged in LDS, also with a persistent grid that copies them once per
-    // workgroup, did not make this pass faster: it keeps the global tables)
-    xupd_body<F, P, SPLIT, WAVE, FP, NOISE, SHUF>(W, U, pairs, npairs, G, ar, unused, tw, wph, draw,
+    // the twiddle table in LDS behind the two rows per pair, read by both transforms of every
+    // row (with the 4-point plan's three twiddled passes per transform, the 512^3 FP32 pass
+    // went from 380 to 374 us)
+    constexpr int NTW = fix_tw_count(FP{});
+    C2<F>* twl = reinterpret_cast<C2<F>*>(smem_raw) + G * 2 * FP::L;
+    for (int i = threadIdx.x; i < NTW; i += blockDim.x) twl[i] = tw[i];
+    __syncthreads();
+    xupd_body<F, P, SPLIT, WAVE, FP, NOISE, SHUF>(W, U, pairs, npairs, G, ar, unused, twl, wph, draw,
                                             reinterpret_cast<C2<F>*>(smem_raw), blockIdx.x, threadIdx.x);
 }
 
@@ -182,6 +187,7 @@ struct ColEntry {
 struct XEntry {
     int prec, P, L, np;
     int rad[4];
+    int ntw;  // twiddle table entries, staged in LDS behind the rows
     // [shuffle][split][noise]: split 0 psi^3, 1 psi^3 - psi; noise 0 without the draw (noise off);
     // the lane-shuffle forms are null where the plan has none
     const void* fn[2][2][2];
@@ -211,6 +217,7 @@ template <typename F, int P, int L, int... Rs> static XEntry x_entry() {
     e.prec = sizeof(F) == 8 ? 64 : 32;
     e.P = P;
     e.L = L;
+    e.ntw = fix_tw_count(FP{});
     const int r[] = {Rs...};
     e.np = (int)sizeof...(Rs);
     for (int i = 0; i < e.np; ++i) e.rad[i] = r[i];
@@ -346,7 +353,7 @@ static int fixed_xupd_t(const XEntry& e, const StepGeom& sg, const StepPhys& ph,
     XArgs<F> ar = make_xargs<F>(sg, ph);
     const int S = lp.L / lp.P;
     const int G = xupd_pairs_per_wg(lp);
-    const size_t lds = (size_t)G * 2 * lp.L * sizeof(C2<F>);
+    const size_t lds = ((size_t)G * 2 * lp.L + e.ntw) * sizeof(C2<F>);  // two rows per pair, twiddles
     const int mode = g_shuffle.load();
     const bool sh = e.fn[1][0][0] && (mode == 2 || (mode == 1 && lp.shuf));
     const void* fn = e.fn[sh ? 1 : 0][ph.split == MCS_SPLIT_REFERENCE ? 1 : 0][ar.noise ? 1 : 0];
