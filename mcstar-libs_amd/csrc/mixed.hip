@@ -30,7 +30,12 @@ __global__ void __launch_bounds__(MCS_XPASS_MAX_THREADS)
     extern __shared__ unsigned char smem_raw[];
     const uint32_t draw = NOISE && ar.noise ? (uint32_t)*d_step : 0u;
     const FftPlanDev unused{};
-    xupd_mixed_body<F, P, SPLIT, WAVE, FP, NOISE, SHUF>(W, Sx, pairs, npairs, G, ar, unused, tw, wph, draw, invL,
+    // the twiddle table in LDS behind the three rows per pair, read by all three transforms
+    constexpr int NTW = fix_tw_count(FP{});
+    C2<F>* twl = reinterpret_cast<C2<F>*>(smem_raw) + G * 3 * FP::L;
+    for (int i = threadIdx.x; i < NTW; i += blockDim.x) twl[i] = tw[i];
+    __syncthreads();
+    xupd_mixed_body<F, P, SPLIT, WAVE, FP, NOISE, SHUF>(W, Sx, pairs, npairs, G, ar, unused, twl, wph, draw, invL,
                                                         reinterpret_cast<C2<F>*>(smem_raw), blockIdx.x, threadIdx.x);
 }
 
@@ -54,6 +59,7 @@ static int set_lds_mx(const void* fn, size_t bytes) {
 struct MxEntry {
     int prec, P, L, np;
     int rad[4];
+    int ntw;  // twiddle table entries, staged in LDS behind the rows
     // [shuffle][split][noise]: split 0 psi^3, 1 psi^3 - psi; noise 0 without the draw (noise off);
     // the shuffle forms are null where the plan has none
     const void* fn[2][2][2];
@@ -67,6 +73,7 @@ template <typename F, int P, int L, int... Rs> static MxEntry mx_entry() {
     e.prec = sizeof(F) == 8 ? 64 : 32;
     e.P = P;
     e.L = L;
+    e.ntw = fix_tw_count(FP{});
     const int r[] = {Rs...};
     e.np = (int)sizeof...(Rs);
     for (int i = 0; i < e.np; ++i) e.rad[i] = r[i];
@@ -87,6 +94,9 @@ static const MxEntry* mx_table(int* n) {
     static const MxEntry t[] = {
         mx_entry<float, 8, 256, 4, 8, 8>(),  mx_entry<float, 16, 256, 16, 16>(), mx_entry<float, 8, 128, 2, 8, 8>(),
         mx_entry<float, 16, 128, 8, 16>(),   mx_entry<double, 8, 256, 4, 8, 8>(), mx_entry<double, 8, 128, 2, 8, 8>(),
+        // 4 points per thread, as in the spectral form's table
+        mx_entry<float, 4, 256, 4, 4, 4, 4>(), mx_entry<float, 4, 128, 2, 4, 4, 4>(),
+        mx_entry<double, 4, 256, 4, 4, 4, 4>(), mx_entry<double, 4, 128, 2, 4, 4, 4>(),
     };
     *n = (int)(sizeof(t) / sizeof(t[0]));
     return t;
@@ -127,7 +137,7 @@ static int launch_mx_fx(const MxEntry& e, const StepGeom& sg, const XArgs<F>& ar
     const LinePlan& lp = sg.px;
     const int S = lp.L / lp.P;
     const int G = xupd_mixed_pairs_per_wg(lp);
-    const size_t lds = mixed_lds(lp, G);
+    const size_t lds = mixed_lds(lp, G) + (size_t)e.ntw * sizeof(C2<F>);  // rows, twiddles
     const void* fn = e.fn[use_shuffle(e, lp) ? 1 : 0][split == MCS_SPLIT_REFERENCE ? 1 : 0][ar.noise ? 1 : 0];
     int st = set_lds_mx(fn, lds);
     if (st) return st;
