@@ -443,6 +443,7 @@ const void* fixed_ens_col_kernel(const LinePlan& lp, int kind, int split) {
 struct RowEntry {
     int prec, P, L, np;
     int rad[4];
+    int ntw;  // twiddle table entries, staged in LDS behind the rows
     const void* r2c;
     const void* c2r;
 };
@@ -454,6 +455,7 @@ template <typename F, int P, int L, int... Rs> static RowEntry row_entry() {
     e.prec = sizeof(F) == 8 ? 64 : 32;
     e.P = P;
     e.L = L;
+    e.ntw = fix_tw_count(FP{});
     const int r[] = {Rs...};
     e.np = (int)sizeof...(Rs);
     for (int i = 0; i < e.np; ++i) e.rad[i] = r[i];
@@ -465,6 +467,8 @@ template <typename F, int P, int L, int... Rs> static RowEntry row_entry() {
 static const RowEntry* row_table(int* n) {
     static const RowEntry t[] = {
         row_entry<float, 16, 128, 8, 16>(),   row_entry<float, 16, 256, 16, 16>(),
+        row_entry<float, 8, 256, 4, 8, 8>(),  row_entry<float, 8, 128, 2, 8, 8>(),
+        row_entry<float, 8, 512, 8, 8, 8>(),
         row_entry<float, 32, 512, 16, 32>(),  row_entry<double, 8, 128, 2, 8, 8>(),
         row_entry<double, 8, 256, 4, 8, 8>(), row_entry<double, 8, 512, 8, 8, 8>(),
     };
@@ -483,6 +487,7 @@ int fixed_row(int mode, const LinePlan& lp, const void* wph, const void* src, vo
         if (plan_matches(t[i], lp)) e = &t[i];
     if (!e) return MCS_OK;
     const void* fn = mode == 0 ? e->r2c : e->c2r;
+    lds += (size_t)e->ntw * (lp.prec == 64 ? 16 : 8);  // the twiddle table behind the rows
     int st = set_lds_fx(fn, lds);
     if (st) return st;
     const dim3 grid((unsigned)((nrows + G - 1) / G)), block(G * (lp.L / lp.P));

@@ -10,6 +10,20 @@
 //        read back through LDS; k = L (Nyquist) comes from k = 0 with w^L = -1
 //   C2R: Z_k = (X_k + X*_{L-k}) + i (X_k - X*_{L-k}) w^-k with the mirror element
 //        read back through LDS, inverse FFT in registers
+// a fixed plan's twiddle table copied into LDS behind the G rows (the launcher sizes the LDS for
+// it); runtime plans read the global table
+template <typename F, typename FP> MCS_DEV const C2<F>* row_twiddles(const C2<F>* tw, C2<F>* behind) {
+    if constexpr (FP::fixed) {
+        constexpr int NTW = fix_tw_count(FP{});
+        for (int i = threadIdx.x; i < NTW; i += blockDim.x) behind[i] = tw[i];
+        __syncthreads();
+        return behind;
+    } else {
+        (void)behind;
+        return tw;
+    }
+}
+
 template <typename F, int P, bool WAVE, typename FP = RunPlan>
 __global__ void __launch_bounds__(256) k_row_r2c_reg(const C2<F>* __restrict__ src, C2<F>* __restrict__ dst,
                                                      long nrows, int G, FftPlanDev pl,
@@ -21,10 +35,11 @@ __global__ void __launch_bounds__(256) k_row_r2c_reg(const C2<F>* __restrict__ s
     const long row = (long)blockIdx.x * G + g;
     const bool valid = row < nrows && g < G;
     const int base = (g < G ? g : 0) * L;
+    const C2<F>* twr = row_twiddles<F, FP>(tw, lds + G * L);
     C2<F> a[P];
 #pragma unroll
     for (int m = 0; m < P; ++m) a[m] = valid ? src[row * L + t + m * S] : mk<F>(0, 0);
-    line_fft<F, P, -1, WAVE, FP>(a, pl, tw, t, lds, base, 1);
+    line_fft<F, P, -1, WAVE, FP>(a, pl, twr, t, lds, base, 1);
     lds_sync<WAVE>();
 #pragma unroll
     for (int m = 0; m < P; ++m) lds[base + t + m * S] = a[m];
@@ -54,6 +69,7 @@ __global__ void __launch_bounds__(256) k_row_c2r_reg(const C2<F>* __restrict__ s
     const long row = (long)blockIdx.x * G + g;
     const bool valid = row < nrows && g < G;
     const int base = (g < G ? g : 0) * L;
+    const C2<F>* twr = row_twiddles<F, FP>(tw, lds + G * L);
     const C2<F>* in = src + (valid ? row : 0) * (L + 1);
     // the row is read once; the mirror X_{L-k} comes back through LDS (X_L, the
     // mirror of k = 0, is read by the row's first thread)
@@ -75,7 +91,7 @@ __global__ void __launch_bounds__(256) k_row_c2r_reg(const C2<F>* __restrict__ s
         a[m] = cadd(sm, cmuli<+1>(d));
     }
     lds_sync<WAVE>();  // every mirror is read before the transform's exchanges reuse the LDS
-    line_fft<F, P, +1, WAVE, FP>(a, pl, tw, t, lds, base, 1);
+    line_fft<F, P, +1, WAVE, FP>(a, pl, twr, t, lds, base, 1);
     if (valid) {
 #pragma unroll
         for (int m = 0; m < P; ++m) dst[row * L + t + m * S] = a[m];
