@@ -4,12 +4,6 @@
 
 #include "fft_kernels.hpp"
 
-// x pass of the boundary transform for even nx on the register engine: G rows of
-// L = nx/2 packed points per workgroup, S = L/P threads per row.
-//   R2C: packed FFT in registers, then X_k = E_k + w^k O_k with the mirror Z_{L-k}
-//        read back through LDS; k = L (Nyquist) comes from k = 0 with w^L = -1
-//   C2R: Z_k = (X_k + X*_{L-k}) + i (X_k - X*_{L-k}) w^-k with the mirror element
-//        read back through LDS, inverse FFT in registers
 // a fixed plan's twiddle table copied into LDS behind the G rows (the launcher sizes the LDS for
 // it); runtime plans read the global table
 template <typename F, typename FP> MCS_DEV const C2<F>* row_twiddles(const C2<F>* tw, C2<F>* behind) {
@@ -24,6 +18,12 @@ template <typename F, typename FP> MCS_DEV const C2<F>* row_twiddles(const C2<F>
     }
 }
 
+// x pass of the boundary transform for even nx on the register engine: G rows of
+// L = nx/2 packed points per workgroup, S = L/P threads per row.
+//   R2C: packed FFT in registers, then X_k = E_k + w^k O_k with the mirror Z_{L-k}
+//        read back through LDS; k = L (Nyquist) comes from k = 0 with w^L = -1
+//   C2R: Z_k = (X_k + X*_{L-k}) + i (X_k - X*_{L-k}) w^-k with the mirror element
+//        read back through LDS, inverse FFT in registers
 template <typename F, int P, bool WAVE, typename FP = RunPlan>
 __global__ void __launch_bounds__(256) k_row_r2c_reg(const C2<F>* __restrict__ src, C2<F>* __restrict__ dst,
                                                      long nrows, int G, FftPlanDev pl,

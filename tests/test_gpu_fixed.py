@@ -110,7 +110,7 @@ SHUFFLE_CASES = [
     ((256, 256, 256), np.float32, 1e-3, 0, [8, 4, 16, 16, 8, 16]),
     ((512, 512, 512), np.float64, 0.0, 0, [8, 2, 8, 8, 16, 8]),
     ((256, 256), np.float64, 1e-3, 1, [8, 4, 8, 8]),
-    # 4*4*4*4 and 2*4*4*4: a 4 x 4 transpose over lanes 16 and 32 apart (v_permlane16/32_swap)
+    # 4*4*4*4 and 2*4*4*4: a 4 x 4 transpose over lanes 16 and 32 apart (v_permlane16/32_swap), 8 and 16
     ((512, 512, 512), np.float32, 0.0, 0, [4, 2, 32, 16, 32, 16]),
     ((256, 256, 256), np.float32, 1e-3, 0, [4, 4, 16, 16, 8, 16]),
 ]
