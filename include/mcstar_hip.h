@@ -112,7 +112,7 @@ int mcs_norm(void* real, const int ext[3], int prec);
 int mcs_update_eom(void* kpsi, const void* knonlin, const void* knoise, const int ext[3], double namp,
                    double dt, double eps, const double lens[3], int prec);
 /* PFC split selector for the fused paths */
-#define MCS_SPLIT_NORTHSTAR 0 /* N = psi^3,       D = 1 + dt k^2 (-eps + (1-k^2)^2) */
+#define MCS_SPLIT_CUBIC 0 /* N = psi^3,       D = 1 + dt k^2 (-eps + (1-k^2)^2) */
 #define MCS_SPLIT_REFERENCE 1 /* N = psi^3 - psi, D = 1 + dt k^2 (1 - eps + (1-k^2)^2) */
 /* the same update with conserved noise drawn in registers:
  *   eta_k = sqrt(2 T dt / dV) sqrt(N) |k| xi_k(seed, draw), xi_k = mcs_rng_fill_kspace's values. */

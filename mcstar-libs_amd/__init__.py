@@ -21,7 +21,7 @@ HEADER = os.path.join(os.path.dirname(_HERE), "include", "mcstar_hip.h")
 OK, ERR_INVALID, ERR_HIP, ERR_UNSUPPORTED, ERR_NOMEM, ERR_CAPACITY = 0, 1, 2, 3, 4, 5
 H2D, D2H, D2D, H2H = 1, 2, 3, 4
 R2C, C2R = 0, 1
-SPLIT_NORTHSTAR, SPLIT_REFERENCE = 0, 1
+SPLIT_CUBIC, SPLIT_REFERENCE = 0, 1
 FORM_SPECTRAL, FORM_MIXED = 0, 1
 
 _vp = ctypes.c_void_p
@@ -592,7 +592,7 @@ def pair_histogram(xyz, box, rmax, nbins):
 class Stepper:
     """The fused PFC stepper (mcs_stepper_*)."""
 
-    def __init__(self, shape_zyx, h, eps, dt, temp=0.0, seed=0, split=SPLIT_NORTHSTAR, dtype=np.float32,
+    def __init__(self, shape_zyx, h, eps, dt, temp=0.0, seed=0, split=SPLIT_CUBIC, dtype=np.float32,
                  plan=None, form=None):
         """plan: optional explicit [P_x, pairs/wg, P_y, lines/wg, P_z, lines/wg] instead of the tuned plans;
         form: FORM_SPECTRAL or FORM_MIXED (default: the library's default form, mcs_set_step_form)"""
@@ -777,7 +777,7 @@ class Ensemble:
     """Batched replicas (mcs_ensemble_*).  Each replica is a dict with keys
     shape (numpy order), h, eps, dt, temp, psi0, amp, init_seed, noise_seed."""
 
-    def __init__(self, replicas, split=SPLIT_NORTHSTAR, dtype=np.float32):
+    def __init__(self, replicas, split=SPLIT_CUBIC, dtype=np.float32):
         self.dtype = np.dtype(dtype)
         self.shapes = [tuple(r["shape"]) for r in replicas]
         arr = (ReplicaParams * len(replicas))()

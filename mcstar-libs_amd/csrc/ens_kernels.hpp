@@ -156,10 +156,10 @@ __global__ void __launch_bounds__(MCS_XPASS_MAX_THREADS) k_cls_xupd(const EnsCla
         const bool wave = 64 % (c.pl.L / c.P) == 0;
         with_p<TIER>(cls_p(c), [&](auto PC) {
             if (wave)
-                xupd_body<F, decltype(PC)::value, MCS_SPLIT_NORTHSTAR, true, RunPlan, NOISE>(
+                xupd_body<F, decltype(PC)::value, MCS_SPLIT_CUBIC, true, RunPlan, NOISE>(
                     R.W, R.U, c.pairs, c.npairs, c.G, R.xa, c.pl, c.tw, c.wph, draw, lds, u, opaque_tid());
             else
-                xupd_body<F, decltype(PC)::value, MCS_SPLIT_NORTHSTAR, false, RunPlan, NOISE>(
+                xupd_body<F, decltype(PC)::value, MCS_SPLIT_CUBIC, false, RunPlan, NOISE>(
                     R.W, R.U, c.pairs, c.npairs, c.G, R.xa, c.pl, c.tw, c.wph, draw, lds, u, opaque_tid());
         });
         __syncthreads();

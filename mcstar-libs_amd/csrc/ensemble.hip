@@ -247,7 +247,7 @@ static int ens_col_nl_t(mcs_ensemble* e, EnsGroup* g, const LinePlan& lp, const 
 template <typename F>
 static int ens_xupd(mcs_ensemble* e, EnsGroup* g, int first, int nr, int draw_off, hipStream_t s) {
     if (e->split == MCS_SPLIT_REFERENCE) { MCS_P_DISPATCH(g->sg.px.P, (ens_xupd_t<F, PP, MCS_SPLIT_REFERENCE>(e, g, first, nr, draw_off, s))) }
-    MCS_P_DISPATCH(g->sg.px.P, (ens_xupd_t<F, PP, MCS_SPLIT_NORTHSTAR>(e, g, first, nr, draw_off, s)))
+    MCS_P_DISPATCH(g->sg.px.P, (ens_xupd_t<F, PP, MCS_SPLIT_CUBIC>(e, g, first, nr, draw_off, s)))
 }
 
 template <typename F>
@@ -261,7 +261,7 @@ template <typename F>
 static int ens_col_nl(mcs_ensemble* e, EnsGroup* g, const LinePlan& lp, const ColGeom& cg, int first, int nr,
                       hipStream_t s) {
     if (e->split == MCS_SPLIT_REFERENCE) { MCS_P_DISPATCH(lp.P, (ens_col_nl_t<F, PP, MCS_SPLIT_REFERENCE>(e, g, lp, cg, first, nr, s))) }
-    MCS_P_DISPATCH(lp.P, (ens_col_nl_t<F, PP, MCS_SPLIT_NORTHSTAR>(e, g, lp, cg, first, nr, s)))
+    MCS_P_DISPATCH(lp.P, (ens_col_nl_t<F, PP, MCS_SPLIT_CUBIC>(e, g, lp, cg, first, nr, s)))
 }
 
 // one step of members [first, first+nr) of g, drawing noise number d_step + draw_off
@@ -383,7 +383,7 @@ static int cls_launch_t(mcs_ensemble* e, const EnsClass& c, int draw_off, hipStr
         return MCS_OK;
     }
     const auto kfn = c.kind == CLS_COL_NL ? (ref ? k_cls_col_nl<F, TIER, MCS_SPLIT_REFERENCE>
-                                                 : k_cls_col_nl<F, TIER, MCS_SPLIT_NORTHSTAR>)
+                                                 : k_cls_col_nl<F, TIER, MCS_SPLIT_CUBIC>)
                                           : (c.kind == CLS_COL_INV ? k_cls_col<F, TIER, +1> : k_cls_col<F, TIER, -1>);
     fn = (const void*)kfn;
     int st = cls_lds_attr(fn, c.lds);
@@ -799,7 +799,7 @@ extern "C" int mcs_ensemble_destroy(mcs_ensemble* e) {
 extern "C" int mcs_ensemble_create(mcs_ensemble** out, const mcs_replica* reps, int nrep, int split, int prec) {
     MCS_CHECK(out && reps && nrep > 0, MCS_ERR_INVALID, "bad argument");
     MCS_CHECK(prec == 32 || prec == 64, MCS_ERR_INVALID, "precision must be 32 or 64");
-    MCS_CHECK(split == MCS_SPLIT_NORTHSTAR || split == MCS_SPLIT_REFERENCE, MCS_ERR_INVALID, "bad split");
+    MCS_CHECK(split == MCS_SPLIT_CUBIC || split == MCS_SPLIT_REFERENCE, MCS_ERR_INVALID, "bad split");
     for (int r = 0; r < nrep; ++r) {
         const mcs_replica& p = reps[r];
         MCS_CHECK(p.ext[0] >= 1 && p.ext[1] >= 1 && p.ext[2] >= 1, MCS_ERR_INVALID, "extent must be >=1");

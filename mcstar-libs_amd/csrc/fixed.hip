@@ -204,7 +204,7 @@ template <typename F, int P, int L, int... Rs> static ColEntry col_entry() {
     for (int i = 0; i < e.np; ++i) e.rad[i] = r[i];
     e.fwd = (const void*)k_col_fx<F, P, -1, FP>;
     e.inv = (const void*)k_col_fx<F, P, +1, FP>;
-    e.nl_ns = (const void*)k_col_nl_fx<F, P, MCS_SPLIT_NORTHSTAR, FP>;
+    e.nl_ns = (const void*)k_col_nl_fx<F, P, MCS_SPLIT_CUBIC, FP>;
     e.nl_ref = (const void*)k_col_nl_fx<F, P, MCS_SPLIT_REFERENCE, FP>;
     return e;
 }
@@ -221,14 +221,14 @@ template <typename F, int P, int L, int... Rs> static XEntry x_entry() {
     const int r[] = {Rs...};
     e.np = (int)sizeof...(Rs);
     for (int i = 0; i < e.np; ++i) e.rad[i] = r[i];
-    e.fn[0][0][1] = (const void*)k_xupd_fx<F, P, MCS_SPLIT_NORTHSTAR, wave, FP, true, false>;
+    e.fn[0][0][1] = (const void*)k_xupd_fx<F, P, MCS_SPLIT_CUBIC, wave, FP, true, false>;
     e.fn[0][1][1] = (const void*)k_xupd_fx<F, P, MCS_SPLIT_REFERENCE, wave, FP, true, false>;
-    e.fn[0][0][0] = (const void*)k_xupd_fx<F, P, MCS_SPLIT_NORTHSTAR, wave, FP, false, false>;
+    e.fn[0][0][0] = (const void*)k_xupd_fx<F, P, MCS_SPLIT_CUBIC, wave, FP, false, false>;
     e.fn[0][1][0] = (const void*)k_xupd_fx<F, P, MCS_SPLIT_REFERENCE, wave, FP, false, false>;
     if constexpr (shuf) {
-        e.fn[1][0][1] = (const void*)k_xupd_fx<F, P, MCS_SPLIT_NORTHSTAR, wave, FP, true, true>;
+        e.fn[1][0][1] = (const void*)k_xupd_fx<F, P, MCS_SPLIT_CUBIC, wave, FP, true, true>;
         e.fn[1][1][1] = (const void*)k_xupd_fx<F, P, MCS_SPLIT_REFERENCE, wave, FP, true, true>;
-        e.fn[1][0][0] = (const void*)k_xupd_fx<F, P, MCS_SPLIT_NORTHSTAR, wave, FP, false, true>;
+        e.fn[1][0][0] = (const void*)k_xupd_fx<F, P, MCS_SPLIT_CUBIC, wave, FP, false, true>;
         e.fn[1][1][0] = (const void*)k_xupd_fx<F, P, MCS_SPLIT_REFERENCE, wave, FP, false, true>;
     }
     return e;
@@ -403,7 +403,7 @@ template <typename F, int P, int L, int... Rs> static EnsColEntry ens_col_entry(
     for (int i = 0; i < e.np; ++i) e.rad[i] = r[i];
     e.fwd = (const void*)k_ens_col<F, P, -1, FP>;
     e.inv = (const void*)k_ens_col<F, P, +1, FP>;
-    e.nl_ns = (const void*)k_ens_col_nl<F, P, MCS_SPLIT_NORTHSTAR, FP>;
+    e.nl_ns = (const void*)k_ens_col_nl<F, P, MCS_SPLIT_CUBIC, FP>;
     e.nl_ref = (const void*)k_ens_col_nl<F, P, MCS_SPLIT_REFERENCE, FP>;
     return e;
 }

@@ -77,14 +77,14 @@ template <typename F, int P, int L, int... Rs> static MxEntry mx_entry() {
     const int r[] = {Rs...};
     e.np = (int)sizeof...(Rs);
     for (int i = 0; i < e.np; ++i) e.rad[i] = r[i];
-    e.fn[0][0][1] = (const void*)k_xupd_mx_fx<F, P, MCS_SPLIT_NORTHSTAR, wave, FP, true, false>;
+    e.fn[0][0][1] = (const void*)k_xupd_mx_fx<F, P, MCS_SPLIT_CUBIC, wave, FP, true, false>;
     e.fn[0][1][1] = (const void*)k_xupd_mx_fx<F, P, MCS_SPLIT_REFERENCE, wave, FP, true, false>;
-    e.fn[0][0][0] = (const void*)k_xupd_mx_fx<F, P, MCS_SPLIT_NORTHSTAR, wave, FP, false, false>;
+    e.fn[0][0][0] = (const void*)k_xupd_mx_fx<F, P, MCS_SPLIT_CUBIC, wave, FP, false, false>;
     e.fn[0][1][0] = (const void*)k_xupd_mx_fx<F, P, MCS_SPLIT_REFERENCE, wave, FP, false, false>;
     if constexpr (shuf) {
-        e.fn[1][0][1] = (const void*)k_xupd_mx_fx<F, P, MCS_SPLIT_NORTHSTAR, wave, FP, true, true>;
+        e.fn[1][0][1] = (const void*)k_xupd_mx_fx<F, P, MCS_SPLIT_CUBIC, wave, FP, true, true>;
         e.fn[1][1][1] = (const void*)k_xupd_mx_fx<F, P, MCS_SPLIT_REFERENCE, wave, FP, true, true>;
-        e.fn[1][0][0] = (const void*)k_xupd_mx_fx<F, P, MCS_SPLIT_NORTHSTAR, wave, FP, false, true>;
+        e.fn[1][0][0] = (const void*)k_xupd_mx_fx<F, P, MCS_SPLIT_CUBIC, wave, FP, false, true>;
         e.fn[1][1][0] = (const void*)k_xupd_mx_fx<F, P, MCS_SPLIT_REFERENCE, wave, FP, false, true>;
     }
     return e;
@@ -192,7 +192,7 @@ static int launch_mx(const StepGeom& sg, const StepPhys& ph, const void* W, void
     const XArgs<F> ar = make_xargs<F>(sg, ph);
     if (const MxEntry* e = mx_match(sg.px)) return launch_mx_fx<F>(*e, sg, ar, ph.split, W, Sx, s);
     return ph.split == MCS_SPLIT_REFERENCE ? launch_mx_d<F, MCS_SPLIT_REFERENCE>(sg, ar, W, Sx, s)
-                                           : launch_mx_d<F, MCS_SPLIT_NORTHSTAR>(sg, ar, W, Sx, s);
+                                           : launch_mx_d<F, MCS_SPLIT_CUBIC>(sg, ar, W, Sx, s);
 }
 
 int launch_xupd_mixed(const StepGeom& sg, const StepPhys& ph, const void* W, void* Sx, hipStream_t s) {

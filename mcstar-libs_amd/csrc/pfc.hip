@@ -3,7 +3,7 @@
 //   call_kernel_norm        (cuda/kernels/norm.hpp:1-10)
 //   call_kernel_update_eom  (cuda/kernels/eom.hpp:60-86)
 //   NoiseHostApi            (cuda/noise.hpp:93-150), on Philox4x32-10
-// plus the north-star update with in-register noise, the seeded initial
+// plus the cubic-split update with in-register noise, the seeded initial
 // field and FP64 reductions (mean density, free energy).
 // All are streaming kernels: one read (and write) of each operand per call.
 #include "fft_kernels.hpp"
@@ -156,7 +156,7 @@ extern "C" int mcs_update_pfc(void* kpsi, const void* knl, const int ext[3], con
                               double eps, double temp, uint64_t seed, uint32_t draw, int split, int prec) {
     MCS_CHECK(kpsi && knl && lens && ext_ok(ext), MCS_ERR_INVALID, "bad argument");
     MCS_CHECK(prec == 32 || prec == 64, MCS_ERR_INVALID, "precision must be 32 or 64");
-    MCS_CHECK(split == MCS_SPLIT_NORTHSTAR || split == MCS_SPLIT_REFERENCE, MCS_ERR_INVALID, "bad split");
+    MCS_CHECK(split == MCS_SPLIT_CUBIC || split == MCS_SPLIT_REFERENCE, MCS_ERR_INVALID, "bad split");
     const long nc = (long)(ext[0] / 2 + 1) * ext[1] * ext[2];
     const double h[3] = {lens[0] / ext[0], lens[1] / ext[1], lens[2] / ext[2]};
     const double amp = pfc_noise_amp(ext, h, dt, temp);

@@ -11,7 +11,7 @@ template <typename F> MCS_HD F kwave(int i, int n, F len) {
 
 // semi-implicit conserved update of one mode:
 //   psi' = (psi - dt k^2 nl + amp |k| xi) / (1 + dt k^2 (lin0 + (1 - k^2)^2))
-// lin0 = -eps (north-star split, nl = psi^3) or 1 - eps (reference split, nl = psi^3 - psi)
+// lin0 = -eps (cubic split, nl = psi^3) or 1 - eps (reference split, nl = psi^3 - psi)
 // In FP32 the denominator is inverted with the hardware reciprocal (1 ulp) and
 // |k| is taken only when the noise amplitude is non-zero: the exact divide and
 // square root were a fifth of the fused x pass's vector instructions.

@@ -183,9 +183,9 @@ int launch_col_nl(const LinePlan& lp, const void* src, void* W, const ColGeom& g
     if (lp.prec == 64)
         return split == MCS_SPLIT_REFERENCE
                    ? launch_col_nl_d<double, MCS_SPLIT_REFERENCE>(lp, src, W, g, scale, d_step, s)
-                   : launch_col_nl_d<double, MCS_SPLIT_NORTHSTAR>(lp, src, W, g, scale, d_step, s);
+                   : launch_col_nl_d<double, MCS_SPLIT_CUBIC>(lp, src, W, g, scale, d_step, s);
     return split == MCS_SPLIT_REFERENCE ? launch_col_nl_d<float, MCS_SPLIT_REFERENCE>(lp, src, W, g, scale, d_step, s)
-                                        : launch_col_nl_d<float, MCS_SPLIT_NORTHSTAR>(lp, src, W, g, scale, d_step, s);
+                                        : launch_col_nl_d<float, MCS_SPLIT_CUBIC>(lp, src, W, g, scale, d_step, s);
 }
 
 template <typename F, int P, int SPLIT>
@@ -231,11 +231,11 @@ int launch_xupd(const StepGeom& sg, const StepPhys& ph, void* W, void* U, hipStr
     if (sg.prec == 64) {
         const XArgs<double> ar = make_xargs<double>(sg, ph);
         return ph.split == MCS_SPLIT_REFERENCE ? launch_xupd_d<double, MCS_SPLIT_REFERENCE>(sg, W, U, ar, s)
-                                               : launch_xupd_d<double, MCS_SPLIT_NORTHSTAR>(sg, W, U, ar, s);
+                                               : launch_xupd_d<double, MCS_SPLIT_CUBIC>(sg, W, U, ar, s);
     }
     const XArgs<float> ar = make_xargs<float>(sg, ph);
     return ph.split == MCS_SPLIT_REFERENCE ? launch_xupd_d<float, MCS_SPLIT_REFERENCE>(sg, W, U, ar, s)
-                                           : launch_xupd_d<float, MCS_SPLIT_NORTHSTAR>(sg, W, U, ar, s);
+                                           : launch_xupd_d<float, MCS_SPLIT_CUBIC>(sg, W, U, ar, s);
 }
 
 // ------------------------------------------------------------------ geometry
@@ -479,7 +479,7 @@ static int stepper_create(mcs_stepper** out, const mcs_pfc_params* p, const int*
     MCS_CHECK(out && p, MCS_ERR_INVALID, "null argument");
     MCS_CHECK(p->ext[0] >= 1 && p->ext[1] >= 1 && p->ext[2] >= 1, MCS_ERR_INVALID, "extent must be >=1");
     MCS_CHECK(p->prec == 32 || p->prec == 64, MCS_ERR_INVALID, "precision must be 32 or 64");
-    MCS_CHECK(p->split == MCS_SPLIT_NORTHSTAR || p->split == MCS_SPLIT_REFERENCE, MCS_ERR_INVALID, "bad split");
+    MCS_CHECK(p->split == MCS_SPLIT_CUBIC || p->split == MCS_SPLIT_REFERENCE, MCS_ERR_INVALID, "bad split");
     MCS_CHECK(p->dt > 0 && p->h[0] > 0 && p->h[1] > 0 && p->h[2] > 0, MCS_ERR_INVALID, "dt and h must be positive");
     if (plan) {
         const int rc = step_plan_validate(p->ext, p->prec, plan);

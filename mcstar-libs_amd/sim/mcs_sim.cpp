@@ -12,7 +12,7 @@
 //
 //   --dims NxNxN       grid (nz = 1 for NxN)          --h 0.7853981633974483
 //   --eps 0.3 --dt 0.5 --psi0 -0.25 --amp 0.05 --seed 11
-//   --temp 0 --noise-seed 0                          --split northstar|reference
+//   --temp 0 --noise-seed 0                          --split cubic|reference
 //   --steps 100 --every 0 --out DIR                  --load psi.npy | --resume DIR
 //   --ths 0 --rlim 5.5   (peaks >= ths and bond order within rlim at every output)
 //   --lowpass DIVS       (2D: amplitude_<step>.npy and lowpass_<step>.gz at every output,
@@ -54,7 +54,7 @@ struct Opts {
     double h = M_PI / 4, eps = 0.3, dt = 0.5, psi0 = -0.25, amp = 0.05, temp = 0, ths = 0, rlim = 5.5, lowpass = 0;
     int form = -1;  // -1: the library's default
     uint64_t seed = 11, noise_seed = 0;
-    int split = MCS_SPLIT_NORTHSTAR, steps = 100, every = 0;
+    int split = MCS_SPLIT_CUBIC, steps = 100, every = 0;
     std::string out = ".", load, resume;
     std::vector<int> plan;
     bool tune = true;
@@ -101,9 +101,9 @@ Opts parse(int argc, char** argv) {
         else if (k == "--temp") o.temp = std::atof(v.c_str());
         else if (k == "--noise-seed") o.noise_seed = std::strtoull(v.c_str(), nullptr, 10);
         else if (k == "--split") {
-            if (v == "northstar") o.split = MCS_SPLIT_NORTHSTAR;
+            if (v == "cubic") o.split = MCS_SPLIT_CUBIC;
             else if (v == "reference") o.split = MCS_SPLIT_REFERENCE;
-            else usage("--split takes northstar or reference");
+            else usage("--split takes cubic or reference");
         } else if (k == "--steps") o.steps = std::atoi(v.c_str());
         else if (k == "--every") o.every = std::atoi(v.c_str());
         else if (k == "--out") o.out = v;

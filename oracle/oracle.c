@@ -335,7 +335,7 @@ void orc_init_field(double* psi, const int n[3], double psi0, double amp, uint64
 
 /* ------------------------------------------------------------------------ */
 /* PFC update.  Restates cuda/kernels/eom.hpp:60-72 (reference split) and   */
-/* the north-star split: N = psi^3, D = 1 + dt k^2 (-eps + (1-k^2)^2).       */
+/* the cubic split: N = psi^3, D = 1 + dt k^2 (-eps + (1-k^2)^2).       */
 /* ------------------------------------------------------------------------ */
 
 static inline double kval(int i, int nn, double len) {

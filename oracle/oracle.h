@@ -49,7 +49,7 @@ void orc_stepper_noise(double* xi, const int n[3], uint64_t seed, uint32_t step)
 void orc_init_field(double* psi, const int n[3], double psi0, double amp, uint64_t seed);
 
 /* ---- PFC semi-implicit update on arrays in the half spectrum ---- */
-enum { ORC_SPLIT_NORTHSTAR = 0, ORC_SPLIT_REFERENCE = 1 };
+enum { ORC_SPLIT_CUBIC = 0, ORC_SPLIT_REFERENCE = 1 };
 /* kpsi <- (kpsi - dt k^2 knl + noise_amp*|k|*xi) / D(k); xi may be NULL (noise off) */
 void orc_update(double* kpsi, const double* knl, const double* xi, const int n[3], const double lens[3],
                 double dt, double eps, double noise_amp, int split);

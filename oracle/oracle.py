@@ -18,7 +18,7 @@ _i = ctypes.POINTER(ctypes.c_int)
 _i64 = ctypes.POINTER(ctypes.c_int64)
 _u32 = ctypes.POINTER(ctypes.c_uint32)
 
-SPLIT_NORTHSTAR = 0
+SPLIT_CUBIC = 0
 SPLIT_REFERENCE = 1
 
 
@@ -155,7 +155,7 @@ def update(kpsi, knl, xi, shape_zyx, lens, dt, eps, noise_amp, split):
     return k
 
 
-def pfc_run(psi, h, eps, dt, temp, seed, step0, nsteps, split=SPLIT_NORTHSTAR, want_mass=False):
+def pfc_run(psi, h, eps, dt, temp, seed, step0, nsteps, split=SPLIT_CUBIC, want_mass=False):
     p = np.ascontiguousarray(psi, dtype=np.float64).copy()
     hh = np.ascontiguousarray(h, dtype=np.float64)
     mass = np.empty(nsteps + 1, dtype=np.float64) if want_mass else None

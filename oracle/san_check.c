@@ -30,7 +30,7 @@ static int run_shape(const int n[3]) {
     orc_kspace_noise(x1, n, 5, 1);
     if (nx % 2 == 0) orc_stepper_noise(x2, n, 5, 1);
     memcpy(k2, k1, 2 * Nc * sizeof(double));
-    orc_update(k2, k1, x1, n, lens, 0.3, 0.25, 0.1, ORC_SPLIT_NORTHSTAR);
+    orc_update(k2, k1, x1, n, lens, 0.3, 0.25, 0.1, ORC_SPLIT_CUBIC);
     orc_update(k2, k1, NULL, n, lens, 0.3, 0.25, 0.0, ORC_SPLIT_REFERENCE);
     orc_update_eom_fen(k2, k1, x1, n, lens, 0.1, 0.3, 0.25);
     (void)orc_calc_fen(psi, out, n, h);
@@ -46,7 +46,7 @@ static int run_shape(const int n[3]) {
     (void)orc_free_energy(psi, n, h, 0.25);
     orc_gaussian_init(k3, n, lens, h, 1.5);
     double mass[4];
-    if (orc_pfc_run(psi, n, h, 0.25, 0.5, 1e-3, 7, 0, 3, ORC_SPLIT_NORTHSTAR, mass)) return 1;
+    if (orc_pfc_run(psi, n, h, 0.25, 0.5, 1e-3, 7, 0, 3, ORC_SPLIT_CUBIC, mass)) return 1;
     int64_t* idx = (int64_t*)malloc(N * sizeof(int64_t));
     const int64_t np = orc_findpeaks(psi, n, -10.0, idx, (int64_t)N);
     const int64_t np2 = orc_findpeaks(psi, n, -10.0, idx, np > 0 ? np - 1 : 0);  /* capacity short by one */

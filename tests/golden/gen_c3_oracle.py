@@ -3,7 +3,7 @@ oracle's field after the 512^3 precision study (run from the repo root; about 15
 cores, two hours on 8 slower ones).
 
 The run: 3D 512^3, eps = 0.3, dx = pi/4, dt = 0.5, psi0 = -0.25 + U[-0.05, 0.05] from seed 11
-(the oracle's own Philox initial field), noise off, northstar split, 500 semi-implicit steps of
+(the oracle's own Philox initial field), noise off, cubic split, 500 semi-implicit steps of
 oracle/oracle.c's orc_pfc_run.  The whole field is 1.07 GB in FP64, so the fixture holds:
 
   sample     psi[::8, ::8, ::8]                       every 8th cell per axis, 64^3 FP64
