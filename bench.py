@@ -274,8 +274,9 @@ def main():
                     help="also run a 1 GiB device-to-device copy (reference dispatch for HBM counters)")
     a = ap.parse_args()
     if a.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (the PFC step runs on a single "
-              f"MI355X; see DESIGN.md for how it would shard)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported: the PFC step runs on a single "
+              f"MI355X (sharding the grid in z slabs with an all-to-all transpose per step is not built)",
+              file=sys.stderr)
         sys.exit(2)
 
     mcs = load_pkg()
